@@ -1,0 +1,188 @@
+"""Pins of the FP64 CPU oracle (CPU only).
+
+The oracle restates GTODirectTPIContraction::directScaffoldNew
+(include/particleintegrals/contract/direct.hpp:1412-2123).  Each check here
+is chosen so that a plausible slip -- a dropped degeneracy factor, a wrong
+digestion index, a transposed density, a wrong solid-harmonic sign or a
+broken recursion -- fails at least one of them:
+
+* ChronusQ's own golden data: contract.hdf5 (tests/func/contract.cxx,
+  tolerance 1e-9) and G = F - H of five SCF restart files;
+* /INTS/OVERLAP of the same files (basis order, normalisation, pure signs);
+* mpmath Boys values;
+* closed forms, permutational symmetry, brute-force N^4 contraction.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import (CONTRACT_CASES, GOLDEN, SCF_FIXTURES, fixture_molecule,
+                      load_fixture)
+from oracle.oracle import COULOMB, EXCHANGE, OracleBasis, boys
+
+
+def _basis(meta):
+    mol, gbs = fixture_molecule(meta)
+    return OracleBasis(gbs, mol.Z, mol.xyz)
+
+
+def test_boys_against_mpmath():
+    d = np.load(os.path.join(GOLDEN, "boys_mpmath.npz"))
+    for T, F in zip(d["T"], d["F"]):
+        got = boys(F.size - 1, T)
+        assert np.max(np.abs(got - F) / F) < 5e-15, T
+
+
+@pytest.mark.parametrize("name", SCF_FIXTURES)
+def test_overlap_against_chronusq(name):
+    meta, d = load_fixture("scf_" + name)
+    S = _basis(meta).overlap()
+    assert S.shape == d["OVERLAP"].shape
+    assert np.abs(S - d["OVERLAP"]).max() < 1e-13
+
+
+@pytest.mark.parametrize("case", sorted(CONTRACT_CASES))
+def test_contract_hdf5(case):
+    """tests/func/contract.cxx:97-182: max|AX - AX_ref| < 1e-9."""
+    meta, d = load_fixture("contract_water_ccpvtz")
+    ob = _basis(meta)
+    assert ob.nbf == 58
+    t, h = CONTRACT_CASES[case]
+    AX = ob.contract([d[case + "_X"]], [t], [h])[0]
+    assert np.abs(AX - d[case + "_AX"]).max() < 1e-9
+
+
+@pytest.mark.parametrize("name", SCF_FIXTURES)
+def test_fock_fixture_G(name):
+    """G = F - H = 2J[S] - K[S] (scalar), -K[M] (magnetisation), the
+    assembly of FockBuilder::formRawGDInBatches
+    (include/fockbuilder/impl.hpp:139-237); E2 = 1/4 sum_c Re<D_c, G_c>."""
+    meta, d = load_fixture("scf_" + name)
+    ob = _basis(meta)
+    comps = [c for c in ("SCALAR", "MZ", "MY", "MX") if "PDM_" + c in d]
+    Ds = [d["PDM_" + c] for c in comps]
+    if any(np.iscomplexobj(x) for x in Ds):
+        Ds = [x.astype(np.complex128) for x in Ds]
+    out = ob.contract([Ds[0]] + Ds, [COULOMB] + [EXCHANGE] * len(Ds),
+                      [True] * (len(Ds) + 1))
+    J, Ks = out[0], out[1:]
+    e2 = 0.0
+    for i, c in enumerate(comps):
+        G = (2.0 * J if i == 0 else 0.0) - Ks[i]
+        ref = d["FOCK_" + c] - d["HCORE_" + c]
+        assert np.abs(G - ref).max() < 1e-9, c
+        e2 += 0.25 * np.real(np.sum(np.conj(Ds[i]) * G))
+    assert abs(e2 - d["MANY_BODY_ENERGY"][0]) < 1e-9
+
+
+def test_ssss_closed_form():
+    """(ss|ss) = 2 pi^{5/2} / (p q sqrt(p+q)) K_AB K_CD F_0(T)."""
+    gbs = "****\nH 0\nS 1 1.00\n0.8 1.0\n****\nHe 0\nS 1 1.00\n1.7 1.0\n****\n"
+    xyz = np.array([[0.0, 0.0, 0.0], [0.3, -0.4, 1.1]])
+    ob = OracleBasis(gbs, [1, 2], xyz)
+    a, b = 0.8, 1.7
+    na = (2 * a / np.pi) ** 0.75
+    nb = (2 * b / np.pi) ** 0.75
+    # (s_A s_B | s_A s_B)
+    p = q = a + b
+    AB2 = np.sum((xyz[0] - xyz[1]) ** 2)
+    K = np.exp(-a * b / p * AB2)
+    ref = 2 * np.pi ** 2.5 / (p * q * np.sqrt(p + q)) * K * K * (na * nb) ** 2
+    assert abs(ob.eri_quartet(1, 0, 1, 0)[0] - ref) < 1e-14 * ref
+    # (s_A s_A | s_B s_B)
+    p, q = 2 * a, 2 * b
+    T = p * q / (p + q) * AB2
+    from math import erf
+    F0 = 0.5 * np.sqrt(np.pi / T) * erf(np.sqrt(T))
+    ref = 2 * np.pi ** 2.5 / (p * q * np.sqrt(p + q)) * F0 * na ** 2 * nb ** 2
+    assert abs(ob.eri_quartet(0, 0, 1, 1)[0] - ref) < 1e-14 * ref
+
+
+def test_eri_permutational_symmetry_through_f():
+    """(12|34) = (21|34) = (12|43) = (34|12): the four index orders run
+    the recursions along different paths (bra/ket VRR, HRR direction)."""
+    from chronusq_public_amd.chem import WATER, Molecule, basis_text
+    mol = Molecule.from_angstrom(WATER)
+    ob = OracleBasis(basis_text("cc-pVTZ"), mol.Z, mol.xyz)
+    sh = ob.shells()
+    n = [2 * s["l"] + 1 if s["pure"] else (s["l"] + 1) * (s["l"] + 2) // 2 for s in sh]
+    rng = np.random.Generator(np.random.PCG64(3))
+    picks = [(9, 7, 15, 4), (9, 9, 21, 13), (7, 4, 9, 2), (15, 9, 21, 4)]
+    picks += [tuple(rng.integers(0, len(sh), 4)) for _ in range(8)]
+    for a, b, c, d in picks:
+        I = ob.eri_quartet(a, b, c, d).reshape(n[a], n[b], n[c], n[d])
+        I2 = ob.eri_quartet(b, a, c, d).reshape(n[b], n[a], n[c], n[d])
+        I3 = ob.eri_quartet(a, b, d, c).reshape(n[a], n[b], n[d], n[c])
+        I4 = ob.eri_quartet(c, d, a, b).reshape(n[c], n[d], n[a], n[b])
+        scale = max(1.0, np.abs(I).max())
+        assert np.abs(I - I2.transpose(1, 0, 2, 3)).max() < 1e-12 * scale
+        assert np.abs(I - I3.transpose(0, 1, 3, 2)).max() < 1e-12 * scale
+        assert np.abs(I - I4.transpose(2, 3, 0, 1)).max() < 1e-12 * scale
+
+
+def _small():
+    from chronusq_public_amd.chem import basis_text, water_cluster
+    mol = water_cluster(2, 4.0, seed=11)
+    return OracleBasis(basis_text("6-31G(d)"), mol.Z, mol.xyz)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("her", [True, False])
+def test_direct_matches_bruteforce(cplx, her):
+    """8-fold screened loop vs the full N^4 tensor contracted directly:
+    J_mn = sum_kl (mn|kl) X_lk, K_mn = sum_kl (mk|nl) X_kl (Hermitian X);
+    non-Hermitian X: J of the symmetric part, plain K (direct.hpp:1925-1998)."""
+    ob = _small()
+    n = ob.nbf
+    rng = np.random.Generator(np.random.PCG64(5))
+    X = rng.normal(size=(n, n))
+    if cplx:
+        X = X + 1j * rng.normal(size=(n, n))
+    if her:
+        X = 0.5 * (X + X.conj().T)
+    eri = ob.eri_full()
+    if her:
+        Jb = np.einsum("mnkl,lk->mn", eri, X.real if cplx else X)
+        Kb = np.einsum("mknl,kl->mn", eri, X)
+    else:
+        Xs = 0.5 * (X + X.T)
+        Jb = np.einsum("mnkl,lk->mn", eri, Xs)
+        # the eight updates of direct.hpp:1954-1998 summed over the 8-fold
+        # orbit, halved at the end (:2056-2062), give the plain K of X
+        Kb = np.einsum("mknl,kl->mn", eri, X)
+    for screen in (False, True):
+        J, K = ob.contract([X, X], [COULOMB, EXCHANGE], [her, her], screen=screen)
+        assert np.abs(J - Jb).max() < 1e-10
+        assert np.abs(K - Kb).max() < 1e-10
+
+
+def test_thread_count_invariance():
+    """Per-thread accumulators + reduction (direct.hpp:1495-1513, 2036-2066)
+    give the 1-thread result for Hermitian and non-Hermitian X."""
+    ob = _small()
+    n = ob.nbf
+    rng = np.random.Generator(np.random.PCG64(9))
+    X = rng.normal(size=(n, n))
+    for her in (True, False):
+        Xh = 0.5 * (X + X.T) if her else X
+        a = ob.contract([Xh, Xh], [COULOMB, EXCHANGE], [her, her], nthreads=1)
+        b = ob.contract([Xh, Xh], [COULOMB, EXCHANGE], [her, her], nthreads=3)
+        for u, v in zip(a, b):
+            assert np.abs(u - v).max() < 1e-11
+
+
+def test_translation_invariance_and_positivity():
+    from chronusq_public_amd.chem import basis_text, water_cluster
+    mol = water_cluster(2, 4.0, seed=11)
+    g = basis_text("6-31G(d)")
+    a = OracleBasis(g, mol.Z, mol.xyz)
+    b = OracleBasis(g, mol.Z, mol.xyz + np.array([1.3, -2.1, 0.7]))
+    rng = np.random.Generator(np.random.PCG64(2))
+    C = rng.normal(size=(a.nbf, 6))
+    D = C @ C.T
+    Ja, Ka = a.contract([D, D], [COULOMB, EXCHANGE], [True, True])
+    Jb, Kb = b.contract([D, D], [COULOMB, EXCHANGE], [True, True])
+    assert np.abs(Ja - Jb).max() < 1e-10 and np.abs(Ka - Kb).max() < 1e-10
+    assert np.sum(D * Ja) > 0  # Coulomb metric is positive definite
+    assert np.sum(D * Ka) > 0
