@@ -1,0 +1,303 @@
+"""Python mirror of ChronusQ's direct J/K interface over the cqx C ABI.
+
+Names and argument meaning follow the reference classes:
+
+* ``TwoBodyContraction``  -- include/particleintegrals/twopints.hpp:59-75
+* ``BasisSet``            -- include/basisset/basisset_def.hpp:120-223
+* ``DirectTPI``           -- include/particleintegrals/twopints/gtodirecttpi.hpp:30-141
+* ``GTODirectTPIContraction.twoBodyContract``
+                          -- gtodirecttpi.hpp:143-228 (-> direct.hpp:1412-2123)
+* ``FockBuilder.formGD``  -- include/fockbuilder/impl.hpp:75-244
+
+Errors raise ``CErr`` (a RuntimeError), as ChronusQ's CErr throws
+std::runtime_error("FATAL") (include/cerr.hpp:33-51).  Every call goes to
+libcqx.so on the GPU; there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .chem import Molecule, basis_text
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libcqx.so")
+
+COULOMB, EXCHANGE = 0, 1
+HOST, DEVICE = 0, 1
+
+
+class CErr(RuntimeError):
+    """ChronusQ's fatal error."""
+
+
+_lib = None
+
+
+class _Contraction(C.Structure):
+    _fields_ = [("X", C.c_void_p), ("AX", C.c_void_p), ("her", C.c_int), ("type", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("n_candidates", C.c_longlong), ("n_quartets", C.c_longlong),
+                ("n_prim_quartets_max", C.c_longlong), ("flops_model", C.c_double),
+                ("ms_total", C.c_double), ("ms_eri", C.c_double),
+                ("n_launch_eri", C.c_int)]
+
+
+SYMBOLS = ["cqx_basis_create", "cqx_basis_from_g94", "cqx_basis_destroy",
+           "cqx_basis_dims", "cqx_basis_shell", "cqx_basis_shell_prims",
+           "cqx_ctx_create", "cqx_ctx_destroy", "cqx_last_error", "cqx_ctx_schwarz",
+           "cqx_ctx_shell_pairs", "cqx_two_body_contract", "cqx_eri_quartets",
+           "cqx_last_stats", "cqx_set_option", "cqx_device_alloc", "cqx_device_free",
+           "cqx_memcpy", "cqx_device_zero"]
+
+
+def lib():
+    """Load libcqx.so; fails loudly when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise CErr(f"{LIBPATH} is missing: build it with chronusq_public_amd/build.py")
+        L = C.CDLL(LIBPATH)
+        P, I, D = C.c_void_p, C.c_int, C.c_double
+        L.cqx_last_error.restype = C.c_char_p
+        L.cqx_last_error.argtypes = [P]
+        L.cqx_basis_from_g94.argtypes = [C.c_char_p, I, P, P, I, C.POINTER(P)]
+        L.cqx_basis_create.argtypes = [I, P, P, P, P, P, P, P, C.POINTER(P)]
+        L.cqx_basis_destroy.argtypes = [P]
+        L.cqx_basis_dims.argtypes = [P, P, P, P, P]
+        L.cqx_basis_shell.argtypes = [P, I, P, P, P, P, P]
+        L.cqx_basis_shell_prims.argtypes = [P, I, P, P, P]
+        L.cqx_ctx_create.argtypes = [P, D, I, C.POINTER(P)]
+        L.cqx_ctx_destroy.argtypes = [P]
+        L.cqx_ctx_schwarz.argtypes = [P, P]
+        L.cqx_ctx_shell_pairs.argtypes = [P, P, P, P]
+        L.cqx_two_body_contract.argtypes = [P, I, I, P, I, I]
+        L.cqx_eri_quartets.argtypes = [P, I, P, P, P, P, P, P]
+        L.cqx_last_stats.argtypes = [P, C.POINTER(Stats)]
+        L.cqx_set_option.argtypes = [P, C.c_char_p, D]
+        L.cqx_device_alloc.argtypes = [P, C.c_size_t, C.POINTER(P)]
+        L.cqx_device_free.argtypes = [P, P]
+        L.cqx_memcpy.argtypes = [P, P, P, C.c_size_t, I]
+        L.cqx_device_zero.argtypes = [P, P, C.c_size_t]
+        _lib = L
+    return _lib
+
+
+def _check(rc, handle=None):
+    if rc != 0:
+        msg = lib().cqx_last_error(handle)
+        raise CErr(f"FATAL (cqx status {rc}): {msg.decode() if msg else ''}")
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class TwoBodyContraction:
+    """{X, AX, HER, contType} (twopints.hpp:59-75).  X / AX are NB x NB
+    numpy arrays (column-major semantics: element (i, j) is M[i, j]) or
+    device tensors exposing ``data_ptr()``."""
+
+    def __init__(self, X, AX, HER=True, contType=COULOMB):
+        self.X, self.AX, self.HER, self.contType = X, AX, bool(HER), int(contType)
+
+
+class BasisSet:
+    """Shell set on a molecule (ReferenceBasisSet + BasisSet::update)."""
+
+    def __init__(self, name_or_g94, mol, force_cart=False):
+        text = name_or_g94 if "****" in name_or_g94 else basis_text(name_or_g94)
+        self.mol = mol
+        Z = np.ascontiguousarray(mol.Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(mol.xyz, dtype=np.float64)
+        h = C.c_void_p()
+        _check(lib().cqx_basis_from_g94(text.encode(), len(Z), _ptr(Z), _ptr(xyz),
+                                        int(force_cart), C.byref(h)))
+        self._h = h
+        dims = [C.c_int() for _ in range(4)]
+        _check(lib().cqx_basis_dims(self._h, *[C.byref(d) for d in dims]))
+        self.nShell, self.nBasis, self.maxPrim, self.maxL = (d.value for d in dims)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.cqx_basis_destroy(self._h)
+            self._h = None
+
+    def shell(self, s):
+        v = [C.c_int() for _ in range(5)]
+        _check(lib().cqx_basis_shell(self._h, s, *[C.byref(x) for x in v]))
+        l, pure, nprim, bf, atom = (x.value for x in v)
+        a, c, o = np.zeros(nprim), np.zeros(nprim), np.zeros(3)
+        _check(lib().cqx_basis_shell_prims(self._h, s, _ptr(a), _ptr(c), _ptr(o)))
+        return dict(l=l, pure=pure, nprim=nprim, bf=bf, atom=atom, alpha=a, coef=c, origin=o)
+
+
+class DirectTPI:
+    """Device-resident DirectTPI<double>(basis, basis, mol, threshSchwarz)."""
+
+    def __init__(self, basis, thresh_schwarz=1e-12, device=-1):
+        self.basis = basis
+        self.threshSchwarz = thresh_schwarz
+        h = C.c_void_p()
+        _check(lib().cqx_ctx_create(basis._h, float(thresh_schwarz), int(device), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.cqx_ctx_destroy(self._h)
+            self._h = None
+
+    def schwarz(self):
+        ns = self.basis.nShell
+        Q = np.zeros((ns, ns), order="F")
+        _check(lib().cqx_ctx_schwarz(self._h, _ptr(Q)), self._h)
+        return Q
+
+    def shell_pairs(self):
+        n = C.c_int()
+        _check(lib().cqx_ctx_shell_pairs(self._h, C.byref(n), None, None), self._h)
+        s1 = np.zeros(n.value, dtype=np.int32)
+        s2 = np.zeros(n.value, dtype=np.int32)
+        _check(lib().cqx_ctx_shell_pairs(self._h, C.byref(n), _ptr(s1), _ptr(s2)), self._h)
+        return s1, s2
+
+    def eri_quartets(self, quartets):
+        q = np.asarray(quartets, dtype=np.int32).reshape(-1, 4)
+        cols = [np.ascontiguousarray(q[:, k]) for k in range(4)]
+        nf = []
+        for s in range(self.basis.nShell):
+            sh = self.basis.shell(s)
+            nf.append(2 * sh["l"] + 1 if sh["pure"] else (sh["l"] + 1) * (sh["l"] + 2) // 2)
+        total = sum(nf[a] * nf[b] * nf[c] * nf[d] for a, b, c, d in q)
+        out = np.zeros(max(total, 1))
+        off = np.zeros(len(q) + 1, dtype=np.uint64)
+        _check(lib().cqx_eri_quartets(self._h, len(q), *[_ptr(c) for c in cols], _ptr(out),
+                                      _ptr(off)), self._h)
+        return [out[int(off[i]):int(off[i + 1])] for i in range(len(q))]
+
+    def set_option(self, name, value):
+        _check(lib().cqx_set_option(self._h, name.encode(), float(value)), self._h)
+
+    def last_stats(self):
+        st = Stats()
+        _check(lib().cqx_last_stats(self._h, C.byref(st)), self._h)
+        return {k: getattr(st, k) for k, _ in Stats._fields_}
+
+
+class DeviceMatrix:
+    """An NB x NB matrix (float64 or complex128, column-major) resident in
+    the HBM of a DirectTPI's GPU."""
+
+    def __init__(self, tpi, nb, complex_=False):
+        self.tpi, self.nb, self.cplx = tpi, nb, bool(complex_)
+        self.nbytes = nb * nb * (16 if complex_ else 8)
+        p = C.c_void_p()
+        _check(lib().cqx_device_alloc(tpi._h, self.nbytes, C.byref(p)), tpi._h)
+        self._p = p
+
+    def data_ptr(self):
+        return self._p.value
+
+    def is_complex(self):
+        return self.cplx
+
+    def upload(self, M):
+        a = np.asfortranarray(M, dtype=np.complex128 if self.cplx else np.float64)
+        _check(lib().cqx_memcpy(self.tpi._h, self._p, a.ctypes.data_as(C.c_void_p), self.nbytes, 0),
+               self.tpi._h)
+        return self
+
+    def download(self):
+        a = np.zeros((self.nb, self.nb), dtype=np.complex128 if self.cplx else np.float64, order="F")
+        _check(lib().cqx_memcpy(self.tpi._h, a.ctypes.data_as(C.c_void_p), self._p, self.nbytes, 1),
+               self.tpi._h)
+        return a
+
+    def zero(self):
+        _check(lib().cqx_device_zero(self.tpi._h, self._p, self.nbytes), self.tpi._h)
+        return self
+
+    def __del__(self):
+        if getattr(self, "_p", None) and _lib is not None and getattr(self.tpi, "_h", None):
+            _lib.cqx_device_free(self.tpi._h, self._p)
+            self._p = None
+
+
+class GTODirectTPIContraction:
+    """TPIContractions implementation for the DIRECT algorithm."""
+
+    def __init__(self, tpi):
+        if not isinstance(tpi, DirectTPI):
+            raise CErr("GTODirectTPIContraction expects a DirectTPI")
+        self.tpi = tpi
+
+    def twoBodyContract(self, contractions, screen=True):
+        """AX += J[X] / K[X] for every contraction (callers zero AX)."""
+        if not contractions:
+            raise CErr("Nothing specified to compute")
+        dev = [hasattr(c.X, "data_ptr") for c in contractions]
+        if any(dev) != all(dev):
+            raise CErr("contraction list mixes host and device matrices")
+        arr = (_Contraction * len(contractions))()
+        if all(dev):
+            cplx = contractions[0].X.is_complex()
+            for i, c in enumerate(contractions):
+                arr[i].X, arr[i].AX = c.X.data_ptr(), c.AX.data_ptr()
+                arr[i].her, arr[i].type = int(c.HER), c.contType
+            _check(lib().cqx_two_body_contract(self.tpi._h, int(screen), len(contractions),
+                                               C.cast(arr, C.c_void_p), int(cplx), DEVICE),
+                   self.tpi._h)
+            return
+        cplx = any(np.iscomplexobj(c.X) or np.iscomplexobj(c.AX) for c in contractions)
+        dt = np.complex128 if cplx else np.float64
+        keep = []
+        for i, c in enumerate(contractions):
+            X = np.asfortranarray(c.X, dtype=dt)
+            if not (isinstance(c.AX, np.ndarray) and c.AX.dtype == dt and c.AX.flags.f_contiguous):
+                raise CErr("AX must be a Fortran-ordered array of the contraction dtype")
+            keep.append(X)
+            arr[i].X, arr[i].AX = X.ctypes.data, c.AX.ctypes.data
+            arr[i].her, arr[i].type = int(c.HER), c.contType
+        _check(lib().cqx_two_body_contract(self.tpi._h, int(screen), len(contractions),
+                                           C.cast(arr, C.c_void_p), int(cplx), HOST),
+               self.tpi._h)
+
+
+class SingleSlaterState:
+    """The pieces of SingleSlater that FockBuilder::formGD reads and writes:
+    onePDM as Pauli components [S] (1C) or [S, Z, Y, X] (2C), and the
+    outputs coulombMatrix, exchangeMatrix[c], twoeH[c]."""
+
+    def __init__(self, onePDM, TPI):
+        self.onePDM = [np.asfortranarray(d) for d in onePDM]
+        self.TPI = TPI
+        nb = self.onePDM[0].shape[0]
+        dt = np.result_type(*[d.dtype for d in self.onePDM])
+        self.coulombMatrix = np.zeros((nb, nb), dtype=dt, order="F")
+        self.exchangeMatrix = [np.zeros((nb, nb), dtype=dt, order="F") for _ in self.onePDM]
+        self.twoeH = [np.zeros((nb, nb), dtype=dt, order="F") for _ in self.onePDM]
+
+
+class FockBuilder:
+    """FockBuilder<MatsT,IntsT>::formGD (include/fockbuilder/impl.hpp:75-244)
+    for electronic, non-incremental builds: J on S, K on S (+ Z, Y, X for
+    two-component densities), G = 2J - xHFX K componentwise."""
+
+    @staticmethod
+    def formGD(ss, xHFX=1.0, HerDen=True):
+        dt = ss.coulombMatrix.dtype
+        D = [np.asfortranarray(d, dtype=dt) for d in ss.onePDM]
+        J = np.zeros_like(ss.coulombMatrix, order="F")
+        K = [np.zeros_like(ss.coulombMatrix, order="F") for _ in D]
+        cont = [TwoBodyContraction(D[0], J, HerDen, COULOMB)]
+        doK = abs(xHFX) > 1e-12
+        if doK:
+            cont += [TwoBodyContraction(D[c], K[c], HerDen, EXCHANGE) for c in range(len(D))]
+        ss.TPI.twoBodyContract(cont)
+        ss.coulombMatrix[...] = J
+        for c in range(len(D)):
+            ss.exchangeMatrix[c][...] = K[c]
+            ss.twoeH[c][...] = (2.0 * J if c == 0 else 0.0) - (xHFX * K[c] if doK else 0.0)
+        return ss

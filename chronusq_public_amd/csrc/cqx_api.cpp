@@ -1,0 +1,814 @@
+// C-ABI of the MI355X direct J/K engine: context set-up (shell pairs,
+// primitive pairs, Schwarz bounds on the GPU) and the two-body contraction
+// driver that replaces GTODirectTPIContraction::directScaffoldNew
+// (include/particleintegrals/contract/direct.hpp:1412-2123).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cqx.h"
+#include "basis.hpp"
+#include "device_types.hpp"
+#include "kernels.hpp"
+
+namespace {
+
+const double kRysTable[] = {
+#include "rys_table_data.inc"
+};
+
+std::string g_error;
+
+}  // namespace
+
+struct cqx_basis {
+  cqx::Basis b;
+};
+
+using cqx::PairRec;
+using cqx::PrimPair;
+using cqx::Quartet;
+
+namespace {
+
+constexpr int kNumClass = 10;  // (la, lb), la >= lb, la <= 3
+inline int class_of(int la, int lb) { return la * (la + 1) / 2 + lb; }
+inline void class_ls(int c, int &la, int &lb) {
+  la = 0;
+  while ((la + 1) * (la + 2) / 2 <= c) ++la;
+  lb = c - la * (la + 1) / 2;
+}
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess)                                                           \
+      throw HipError(std::string(#x) + ": " + hipGetErrorString(e_));               \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  void ensure(size_t m) {
+    if (m <= n) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (m == 0) return;
+    HIPCHK(hipMalloc((void **)&p, sizeof(T) * m));
+    n = m;
+  }
+  void upload(const std::vector<T> &v, hipStream_t s) {
+    ensure(v.size());
+    if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s));
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+}  // namespace
+
+struct cqx_ctx {
+  cqx::Basis basis;
+  double thresh = 1e-12;
+  double prim_thresh = 1e-18;
+  int fast_max_nc = 108;  // classes up to this Cartesian size use the fast kernels
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  int ns = 0, nb = 0, nbc = 0;
+  std::vector<std::pair<int, int>> ref_pairs;  // (s1 >= s2) reference order
+  std::vector<std::vector<PairRec>> cls;       // per class, sorted by Q desc
+  std::vector<PrimPair> prims;
+  std::vector<double> schwarz;                 // NS x NS
+
+  DevBuf<PairRec> d_pairs[kNumClass];
+  DevBuf<PrimPair> d_prims;
+  DevBuf<double> d_rys, d_sph;
+  DevBuf<int> d_cart_shell, d_cart_comp, d_sh_bf, d_sh_bfc, d_sh_l, d_sh_pure, d_sh_nf,
+      d_bf_shell, d_bf_comp;
+  cqx::CartMap cmap{};
+
+  // per-call workspace
+  DevBuf<double> d_X, d_AX, d_Bm, d_Bg, d_planes, d_acc;
+  DevBuf<int> d_her;
+  DevBuf<long long> d_cand;
+  DevBuf<Quartet> d_quart;
+  DevBuf<unsigned long long> d_count;
+  DevBuf<double> d_psum;
+  DevBuf<double> d_store;
+  DevBuf<long long> d_store_off;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  cqx_stats stats{};
+};
+
+namespace {
+
+int fail(cqx_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  g_error = msg;
+  return code;
+}
+
+// Cartesian -> pure transform of a contracted Cartesian quartet (host side,
+// used for the Schwarz diagonals and the ERI inspection entry point).
+std::vector<double> cart_to_pure(const cqx::Basis &B, const int sh[4], const double *cart) {
+  int l[4], nc[4], nf[4];
+  for (int k = 0; k < 4; ++k) {
+    l[k] = B.shells[sh[k]].l;
+    nc[k] = cqx::ncart(l[k]);
+    nf[k] = B.shells[sh[k]].nfunc();
+  }
+  std::vector<double> cur(cart, cart + (size_t)nc[0] * nc[1] * nc[2] * nc[3]);
+  int dims[4] = {nc[0], nc[1], nc[2], nc[3]};
+  for (int k = 0; k < 4; ++k) {
+    if (!B.shells[sh[k]].pure) continue;
+    auto T = cqx::solid_harmonics(l[k]);
+    int pre = 1, post = 1;
+    for (int q = 0; q < k; ++q) pre *= dims[q];
+    for (int q = k + 1; q < 4; ++q) post *= dims[q];
+    std::vector<double> nxt((size_t)pre * nf[k] * post, 0.0);
+    for (int a = 0; a < pre; ++a)
+      for (int m = 0; m < nf[k]; ++m)
+        for (int c = 0; c < nc[k]; ++c) {
+          const double t = T[(size_t)m * nc[k] + c];
+          if (t == 0.0) continue;
+          for (int p = 0; p < post; ++p)
+            nxt[((size_t)a * nf[k] + m) * post + p] += t * cur[((size_t)a * nc[k] + c) * post + p];
+        }
+    cur.swap(nxt);
+    dims[k] = nf[k];
+  }
+  return cur;
+}
+
+// Algorithmic flop model of one class pair (Rys quadrature form):
+//   per primitive quartet  n_r (30 + 12 (la+lb+1)(lc+ld+1)
+//                                + 6 (la+1)(lb+1)(lc+1)(ld+1) + 3 N_c) + 2 N_c
+//   per contracted quartet 8 N_c N_s^(1/4) + N_s w (4 n_J + 8 n_K)
+// with N_c / N_s the Cartesian / pure quartet sizes, w = 2 for complex.
+double class_pair_flops(int la, int lb, int lc, int ld, bool pure_d, double prim_sum, double nq,
+                        int w, int nJ, int nK) {
+  const int L = la + lb + lc + ld, nr = L / 2 + 1;
+  auto nc = [](int l) { return (double)cqx::ncart(l); };
+  auto nf = [&](int l) { return (l > 1 && pure_d) ? 2.0 * l + 1.0 : nc(l); };
+  const double Nc = nc(la) * nc(lb) * nc(lc) * nc(ld);
+  const double Ns = nf(la) * nf(lb) * nf(lc) * nf(ld);
+  const double per_prim = nr * (30.0 + 12.0 * (la + lb + 1) * (lc + ld + 1) +
+                                6.0 * (la + 1) * (lb + 1) * (lc + 1) * (ld + 1) + 3.0 * Nc) +
+                          2.0 * Nc;
+  const double per_q = 8.0 * Nc * std::pow(Ns, 0.25) + Ns * w * (4.0 * nJ + 8.0 * nK);
+  return prim_sum * per_prim + nq * per_q;
+}
+
+// primitive pairs of shells (a, b) in kernel orientation
+void make_prims(const cqx::Shell &A, const cqx::Shell &B, std::vector<PrimPair> &out,
+                std::vector<double> *bound) {
+  double AB2 = 0;
+  for (int d = 0; d < 3; ++d) AB2 += (A.origin[d] - B.origin[d]) * (A.origin[d] - B.origin[d]);
+  for (int i = 0; i < A.nprim(); ++i)
+    for (int j = 0; j < B.nprim(); ++j) {
+      const double a = A.alpha[i], b = B.alpha[j], p = a + b;
+      PrimPair pp;
+      pp.p = p;
+      for (int d = 0; d < 3; ++d) {
+        pp.P[d] = (a * A.origin[d] + b * B.origin[d]) / p;
+        pp.PA[d] = pp.P[d] - A.origin[d];
+      }
+      pp.K = A.coef[i] * B.coef[j] * std::exp(-a * b / p * AB2);
+      out.push_back(pp);
+      // |(pp|qq)| <= U_p U_q with U = sqrt(2) pi^(5/4) |K| p^(-5/4)
+      if (bound) bound->push_back(std::sqrt(2.0) * std::pow(M_PI, 1.25) * std::fabs(pp.K) * std::pow(p, -1.25));
+    }
+}
+
+// Run the generic kernel in STORE mode over (bra[q] | ket[q]) and return
+// the contracted Cartesian quartets back to back.
+std::vector<double> eri_store(cqx_ctx *c, const std::vector<PairRec> &bra,
+                              const std::vector<PairRec> &ket, int la, int lb, int lc, int ld,
+                              const std::vector<PrimPair> &prims) {
+  const long long nq = (long long)bra.size();
+  const int ncq = cqx::ncart(la) * cqx::ncart(lb) * cqx::ncart(lc) * cqx::ncart(ld);
+  DevBuf<PairRec> db, dk;
+  DevBuf<PrimPair> dp;
+  DevBuf<Quartet> dq;
+  DevBuf<long long> doff;
+  DevBuf<double> dout;
+  db.upload(bra, c->stream);
+  dk.upload(ket, c->stream);
+  dp.upload(prims, c->stream);
+  std::vector<Quartet> q(nq);
+  std::vector<long long> off(nq);
+  for (long long i = 0; i < nq; ++i) {
+    q[i].bra = (uint32_t)i;
+    q[i].ket_mask = (uint32_t)i;
+    off[i] = i * ncq;
+  }
+  dq.upload(q, c->stream);
+  doff.upload(off, c->stream);
+  dout.ensure((size_t)nq * ncq);
+  cqx::EriLaunch L{};
+  L.la = la; L.lb = lb; L.lc = lc; L.ld = ld;
+  L.quartets = dq.p;
+  L.nq = nq;
+  L.bra = db.p;
+  L.ket = dk.p;
+  L.prims = dp.p;
+  L.rys = c->d_rys.p;
+  L.store = dout.p;
+  L.store_off = doff.p;
+  L.store_mode = 1;
+  HIPCHK(cqx::launch_eri_team(L, c->stream));
+  std::vector<double> h((size_t)nq * ncq);
+  HIPCHK(hipMemcpyAsync(h.data(), dout.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return h;
+}
+
+PairRec make_pair_rec(const cqx::Basis &B, int a, int b, int s1, int s2, int pp_off, int npp) {
+  PairRec r{};
+  r.a = a; r.b = b; r.s1 = s1; r.s2 = s2;
+  r.pp_off = pp_off; r.npp = npp;
+  r.bfa = B.bf_cart[a]; r.bfb = B.bf_cart[b];
+  r.Q = 0.0;
+  for (int d = 0; d < 3; ++d) r.AB[d] = B.shells[a].origin[d] - B.shells[b].origin[d];
+  r.kk = B.shells[a].nprim() * B.shells[b].nprim();
+  return r;
+}
+
+void setup_device_maps(cqx_ctx *c) {
+  const cqx::Basis &B = c->basis;
+  std::vector<int> cart_shell, cart_comp, bf_shell, bf_comp, sh_bf, sh_bfc, sh_l, sh_pure, sh_nf;
+  for (int s = 0; s < c->ns; ++s) {
+    const auto &sh = B.shells[s];
+    for (int k = 0; k < cqx::ncart(sh.l); ++k) { cart_shell.push_back(s); cart_comp.push_back(k); }
+    for (int k = 0; k < sh.nfunc(); ++k) { bf_shell.push_back(s); bf_comp.push_back(k); }
+    sh_bf.push_back(B.bf[s]);
+    sh_bfc.push_back(B.bf_cart[s]);
+    sh_l.push_back(sh.l);
+    sh_pure.push_back(sh.pure ? 1 : 0);
+    sh_nf.push_back(sh.nfunc());
+  }
+  std::vector<double> sph;
+  for (int l = 0; l <= 3; ++l) {
+    auto T = cqx::solid_harmonics(l);
+    sph.insert(sph.end(), T.begin(), T.end());
+  }
+  c->d_cart_shell.upload(cart_shell, c->stream);
+  c->d_cart_comp.upload(cart_comp, c->stream);
+  c->d_bf_shell.upload(bf_shell, c->stream);
+  c->d_bf_comp.upload(bf_comp, c->stream);
+  c->d_sh_bf.upload(sh_bf, c->stream);
+  c->d_sh_bfc.upload(sh_bfc, c->stream);
+  c->d_sh_l.upload(sh_l, c->stream);
+  c->d_sh_pure.upload(sh_pure, c->stream);
+  c->d_sh_nf.upload(sh_nf, c->stream);
+  c->d_sph.upload(sph, c->stream);
+  c->cmap.cart_shell = c->d_cart_shell.p;
+  c->cmap.cart_comp = c->d_cart_comp.p;
+  c->cmap.sh_bf = c->d_sh_bf.p;
+  c->cmap.sh_bfc = c->d_sh_bfc.p;
+  c->cmap.sh_l = c->d_sh_l.p;
+  c->cmap.sh_pure = c->d_sh_pure.p;
+  c->cmap.sph = c->d_sph.p;
+  c->cmap.bf_shell = c->d_bf_shell.p;
+  c->cmap.bf_comp = c->d_bf_comp.p;
+  c->cmap.nb = c->nb;
+  c->cmap.nbc = c->nbc;
+  c->cmap.ns = c->ns;
+}
+
+// Significant pairs, primitive pairs and GPU Schwarz bounds.
+void build_pairs(cqx_ctx *c) {
+  const cqx::Basis &B = c->basis;
+  c->ref_pairs = B.significant_pairs();
+  // primitive-pair bound scale
+  std::vector<PrimPair> tmp;
+  std::vector<double> bnd;
+  double umax = 0.0;
+  for (auto &pr : c->ref_pairs) {
+    tmp.clear();
+    bnd.clear();
+    make_prims(B.shells[pr.first], B.shells[pr.second], tmp, &bnd);
+    for (double u : bnd) umax = std::max(umax, u);
+  }
+  c->cls.assign(kNumClass, {});
+  c->prims.clear();
+  for (auto &pr : c->ref_pairs) {
+    int s1 = pr.first, s2 = pr.second;
+    int a = s1, b = s2;
+    if (B.shells[s1].l < B.shells[s2].l) std::swap(a, b);
+    tmp.clear();
+    bnd.clear();
+    make_prims(B.shells[a], B.shells[b], tmp, &bnd);
+    int off = (int)c->prims.size(), n = 0;
+    for (size_t k = 0; k < tmp.size(); ++k)
+      if (bnd[k] * umax >= c->prim_thresh) {
+        c->prims.push_back(tmp[k]);
+        ++n;
+      }
+    c->cls[class_of(B.shells[a].l, B.shells[b].l)].push_back(make_pair_rec(B, a, b, s1, s2, off, n));
+  }
+  // Schwarz: (ab|ab) for every significant pair, with all primitives
+  // (the reference evaluates these at precision 0, aointegrals_utilities.cxx:70)
+  c->schwarz.assign((size_t)c->ns * c->ns, 0.0);
+  for (int k = 0; k < kNumClass; ++k) {
+    auto &v = c->cls[k];
+    if (v.empty()) continue;
+    int la, lb;
+    class_ls(k, la, lb);
+    std::vector<PairRec> full;
+    std::vector<PrimPair> fp;
+    for (auto &r : v) {
+      int off = (int)fp.size();
+      make_prims(B.shells[r.a], B.shells[r.b], fp, nullptr);
+      full.push_back(make_pair_rec(B, r.a, r.b, r.s1, r.s2, off, (int)fp.size() - off));
+    }
+    auto ints = eri_store(c, full, full, la, lb, la, lb, fp);
+    const int ncq = cqx::ncart(la) * cqx::ncart(lb) * cqx::ncart(la) * cqx::ncart(lb);
+    for (size_t q = 0; q < v.size(); ++q) {
+      auto &r = v[q];
+      int sh[4] = {r.a, r.b, r.a, r.b};
+      auto pure = cart_to_pure(B, sh, ints.data() + q * ncq);
+      const int na = B.shells[r.a].nfunc(), nbf = B.shells[r.b].nfunc();
+      // rows of the infinity norm run over the functions of s1
+      const bool a_is_s1 = (r.a == r.s1);
+      const int n1 = a_is_s1 ? na : nbf, n2 = a_is_s1 ? nbf : na;
+      double nrm = 0.0;
+      for (int i = 0; i < n1; ++i) {
+        double row = 0.0;
+        for (int j = 0; j < n2; ++j) {
+          const int ia = a_is_s1 ? i : j, ib = a_is_s1 ? j : i;
+          const size_t ab = (size_t)ia * nbf + ib;
+          row += std::fabs(pure[ab * na * nbf + ab]);
+        }
+        nrm = std::max(nrm, row);
+      }
+      r.Q = std::sqrt(nrm);
+      c->schwarz[r.s1 + (size_t)r.s2 * c->ns] = r.Q;
+      c->schwarz[r.s2 + (size_t)r.s1 * c->ns] = r.Q;
+    }
+    std::stable_sort(v.begin(), v.end(), [](const PairRec &x, const PairRec &y) { return x.Q > y.Q; });
+  }
+  c->d_prims.upload(c->prims, c->stream);
+  for (int k = 0; k < kNumClass; ++k) c->d_pairs[k].upload(c->cls[k], c->stream);
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *cqx_last_error(const cqx_ctx *ctx) { return ctx ? ctx->err.c_str() : g_error.c_str(); }
+
+int cqx_basis_create(int nshell, const int *l, const int *pure, const int *nprim, const double *alpha,
+                     const double *coef, const double *center, const int *atom, cqx_basis **out) {
+  if (!out || nshell <= 0 || !l || !pure || !nprim || !alpha || !coef || !center || !atom)
+    return fail(nullptr, CQX_ERR_ARG, "cqx_basis_create: bad arguments");
+  auto b = std::make_unique<cqx_basis>();
+  size_t off = 0;
+  for (int s = 0; s < nshell; ++s) {
+    cqx::Shell sh;
+    sh.l = l[s];
+    sh.pure = pure[s] != 0;
+    sh.atom = atom[s];
+    if (sh.l < 0 || sh.l > cqx::kMaxL || nprim[s] <= 0)
+      return fail(nullptr, CQX_ERR_ARG, "cqx_basis_create: bad shell " + std::to_string(s));
+    sh.alpha.assign(alpha + off, alpha + off + nprim[s]);
+    sh.coef.assign(coef + off, coef + off + nprim[s]);
+    off += nprim[s];
+    for (int d = 0; d < 3; ++d) sh.origin[d] = center[3 * s + d];
+    b->b.shells.push_back(sh);
+  }
+  b->b.finalise();
+  *out = b.release();
+  return CQX_OK;
+}
+
+int cqx_basis_from_g94(const char *text, int natom, const int *Z, const double *xyz, int force_cart,
+                       cqx_basis **out) {
+  if (!text || !out || natom <= 0 || !Z || !xyz) return fail(nullptr, CQX_ERR_ARG, "cqx_basis_from_g94: bad arguments");
+  try {
+    auto b = std::make_unique<cqx_basis>();
+    b->b = cqx::Basis::from_g94(text, std::vector<int>(Z, Z + natom),
+                                std::vector<double>(xyz, xyz + 3 * natom), force_cart != 0);
+    *out = b.release();
+    return CQX_OK;
+  } catch (const std::exception &e) {
+    return fail(nullptr, CQX_ERR_BASIS, e.what());
+  }
+}
+
+int cqx_basis_destroy(cqx_basis *b) {
+  delete b;
+  return CQX_OK;
+}
+
+int cqx_basis_dims(const cqx_basis *b, int *nshell, int *nbasis, int *max_prim, int *max_l) {
+  if (!b) return fail(nullptr, CQX_ERR_ARG, "null basis");
+  if (nshell) *nshell = (int)b->b.shells.size();
+  if (nbasis) *nbasis = b->b.nbasis;
+  if (max_prim) *max_prim = b->b.max_prim;
+  if (max_l) *max_l = b->b.max_l;
+  return CQX_OK;
+}
+
+int cqx_basis_shell(const cqx_basis *b, int s, int *l, int *pure, int *nprim, int *bf, int *atom) {
+  if (!b || s < 0 || s >= (int)b->b.shells.size()) return fail(nullptr, CQX_ERR_ARG, "bad shell index");
+  const auto &sh = b->b.shells[s];
+  if (l) *l = sh.l;
+  if (pure) *pure = sh.pure;
+  if (nprim) *nprim = sh.nprim();
+  if (bf) *bf = b->b.bf[s];
+  if (atom) *atom = sh.atom;
+  return CQX_OK;
+}
+
+int cqx_basis_shell_prims(const cqx_basis *b, int s, double *alpha, double *coef, double *origin) {
+  if (!b || s < 0 || s >= (int)b->b.shells.size()) return fail(nullptr, CQX_ERR_ARG, "bad shell index");
+  const auto &sh = b->b.shells[s];
+  if (alpha) std::copy(sh.alpha.begin(), sh.alpha.end(), alpha);
+  if (coef) std::copy(sh.coef.begin(), sh.coef.end(), coef);
+  if (origin) std::copy(sh.origin, sh.origin + 3, origin);
+  return CQX_OK;
+}
+
+int cqx_ctx_create(const cqx_basis *b, double thresh, int device, cqx_ctx **out) {
+  if (!b || !out) return fail(nullptr, CQX_ERR_ARG, "cqx_ctx_create: bad arguments");
+  if (b->b.max_l > cqx::kMaxLDev)
+    return fail(nullptr, CQX_ERR_UNSUPPORTED, "device kernels cover shells up to l = 3");
+  auto c = std::make_unique<cqx_ctx>();
+  try {
+    c->basis = b->b;
+    c->thresh = thresh;
+    if (device >= 0) HIPCHK(hipSetDevice(device));
+    HIPCHK(hipGetDevice(&c->device));
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&c->ev0));
+    HIPCHK(hipEventCreate(&c->ev1));
+    c->ns = (int)c->basis.shells.size();
+    c->nb = c->basis.nbasis;
+    c->nbc = c->basis.ncart_total;
+    std::vector<double> rys(kRysTable, kRysTable + sizeof(kRysTable) / sizeof(double));
+    c->d_rys.upload(rys, c->stream);
+    setup_device_maps(c.get());
+    build_pairs(c.get());
+  } catch (const std::exception &e) {
+    return fail(nullptr, CQX_ERR_HIP, e.what());
+  }
+  *out = c.release();
+  return CQX_OK;
+}
+
+int cqx_ctx_destroy(cqx_ctx *c) {
+  if (!c) return CQX_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  hipStream_t s = c->stream;
+  delete c;
+  if (s) (void)hipStreamDestroy(s);
+  return CQX_OK;
+}
+
+int cqx_ctx_schwarz(cqx_ctx *c, double *Q) {
+  if (!c || !Q) return fail(c, CQX_ERR_ARG, "cqx_ctx_schwarz: bad arguments");
+  std::copy(c->schwarz.begin(), c->schwarz.end(), Q);
+  return CQX_OK;
+}
+
+int cqx_ctx_shell_pairs(cqx_ctx *c, int *npairs, int *s1, int *s2) {
+  if (!c || !npairs) return fail(c, CQX_ERR_ARG, "cqx_ctx_shell_pairs: bad arguments");
+  *npairs = (int)c->ref_pairs.size();
+  if (s1 && s2)
+    for (size_t k = 0; k < c->ref_pairs.size(); ++k) {
+      s1[k] = c->ref_pairs[k].first;
+      s2[k] = c->ref_pairs[k].second;
+    }
+  return CQX_OK;
+}
+
+int cqx_set_option(cqx_ctx *c, const char *name, double value) {
+  if (!c || !name) return fail(c, CQX_ERR_ARG, "cqx_set_option: bad arguments");
+  if (!std::strcmp(name, "prim_screen")) {
+    c->prim_thresh = value;
+    try {
+      build_pairs(c);
+    } catch (const std::exception &e) {
+      return fail(c, CQX_ERR_HIP, e.what());
+    }
+    return CQX_OK;
+  }
+  if (!std::strcmp(name, "fast_max_nc")) {
+    c->fast_max_nc = (int)value;
+    return CQX_OK;
+  }
+  return fail(c, CQX_ERR_ARG, std::string("unknown option ") + name);
+}
+
+int cqx_last_stats(const cqx_ctx *c, cqx_stats *st) {
+  if (!c || !st) return fail(nullptr, CQX_ERR_ARG, "cqx_last_stats: bad arguments");
+  *st = c->stats;
+  return CQX_OK;
+}
+
+int cqx_eri_quartets(cqx_ctx *c, int nq, const int *s1, const int *s2, const int *s3, const int *s4,
+                     double *out, size_t *offsets) {
+  if (!c || nq < 0 || (nq && (!s1 || !s2 || !s3 || !s4 || !out || !offsets)))
+    return fail(c, CQX_ERR_ARG, "cqx_eri_quartets: bad arguments");
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    const cqx::Basis &B = c->basis;
+    size_t pos = 0;
+    for (int q = 0; q < nq; ++q) {
+      int sh[4] = {s1[q], s2[q], s3[q], s4[q]};
+      for (int k = 0; k < 4; ++k)
+        if (sh[k] < 0 || sh[k] >= c->ns) return fail(c, CQX_ERR_ARG, "shell index out of range");
+      std::vector<PrimPair> fp;
+      make_prims(B.shells[sh[0]], B.shells[sh[1]], fp, nullptr);
+      const int n12 = (int)fp.size();
+      make_prims(B.shells[sh[2]], B.shells[sh[3]], fp, nullptr);
+      std::vector<PairRec> bra{make_pair_rec(B, sh[0], sh[1], sh[0], sh[1], 0, n12)};
+      std::vector<PairRec> ket{make_pair_rec(B, sh[2], sh[3], sh[2], sh[3], n12, (int)fp.size() - n12)};
+      auto cart = eri_store(c, bra, ket, B.shells[sh[0]].l, B.shells[sh[1]].l, B.shells[sh[2]].l,
+                            B.shells[sh[3]].l, fp);
+      auto pure = cart_to_pure(B, sh, cart.data());
+      offsets[q] = pos;
+      std::copy(pure.begin(), pure.end(), out + pos);
+      pos += pure.size();
+    }
+    offsets[nq] = pos;
+  } catch (const std::exception &e) {
+    return fail(c, CQX_ERR_HIP, e.what());
+  }
+  return CQX_OK;
+}
+
+int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int is_complex,
+                          int mem) {
+  if (!c || ncon < 0 || (ncon && !list)) return fail(c, CQX_ERR_ARG, "cqx_two_body_contract: bad arguments");
+  if (ncon == 0) return fail(c, CQX_ERR_ARG, "Nothing specified to compute");
+  for (int i = 0; i < ncon; ++i)
+    if (!list[i].X || !list[i].AX || (list[i].type != CQX_COULOMB && list[i].type != CQX_EXCHANGE))
+      return fail(c, CQX_ERR_ARG, "invalid contraction " + std::to_string(i));
+  const auto t_start = std::chrono::steady_clock::now();
+  c->stats = cqx_stats{};
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int NB = c->nb, NBC = c->nbc, NS = c->ns;
+    const size_t nb2 = (size_t)NB * NB, nbc2 = (size_t)NBC * NBC;
+    const size_t ew = is_complex ? 2 : 1;  // doubles per element
+    float eri_ms_total = 0.0f;
+    // groups of at most kMaxMat contractions / kMaxPlanes planes
+    int start = 0;
+    while (start < ncon) {
+      int nmat = 0, nplane = 0;
+      while (start + nmat < ncon && nmat < cqx::kMaxMat) {
+        const auto &cc = list[start + nmat];
+        int np = (!is_complex || (cc.her && cc.type == CQX_COULOMB)) ? 1 : 2;
+        if (nplane + np > cqx::kMaxPlanes) break;
+        nplane += np;
+        ++nmat;
+      }
+      const cqx_contraction *L = list + start;
+      // densities and outputs on the device
+      c->d_X.ensure(nmat * nb2 * ew);
+      c->d_AX.ensure(nmat * nb2 * ew);
+      const hipMemcpyKind hk = mem == CQX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+      for (int m = 0; m < nmat; ++m) {
+        HIPCHK(hipMemcpyAsync(c->d_X.p + m * nb2 * ew, L[m].X, sizeof(double) * nb2 * ew, hk, s));
+        HIPCHK(hipMemcpyAsync(c->d_AX.p + m * nb2 * ew, L[m].AX, sizeof(double) * nb2 * ew, hk, s));
+      }
+      std::vector<int> her(nmat);
+      for (int m = 0; m < nmat; ++m) her[m] = L[m].her ? 1 : 0;
+      c->d_her.upload(her, s);
+      // shell-block norms
+      c->d_Bm.ensure((size_t)nmat * NS * NS);
+      c->d_Bg.ensure((size_t)NS * NS);
+      HIPCHK(cqx::launch_block_norms(c->d_X.p, nmat, is_complex, c->d_her.p, NB, NS, c->d_sh_bf.p,
+                                     c->d_sh_nf.p, c->d_Bm.p, c->d_Bg.p, s));
+      std::vector<double> Bg((size_t)NS * NS);
+      HIPCHK(hipMemcpyAsync(Bg.data(), c->d_Bg.p, sizeof(double) * Bg.size(), hipMemcpyDeviceToHost, s));
+      // Cartesian density planes and accumulators
+      c->d_planes.ensure(nplane * nbc2);
+      c->d_acc.ensure(nplane * nbc2);
+      HIPCHK(cqx::launch_zero(c->d_acc.p, nplane * nbc2, s));
+      cqx::PlaneSet ps{};
+      ps.nbc = NBC;
+      std::vector<int> first_plane(nmat);
+      for (int m = 0, p = 0; m < nmat; ++m) {
+        first_plane[m] = p;
+        const bool two = is_complex && !(L[m].her && L[m].type == CQX_COULOMB);
+        double *re = c->d_planes.p + p * nbc2;
+        double *im = two ? c->d_planes.p + (p + 1) * nbc2 : nullptr;
+        HIPCHK(cqx::launch_to_cart(c->cmap, c->d_X.p + m * nb2 * ew, is_complex, re, im, s));
+        int kind = L[m].type == CQX_COULOMB ? (L[m].her ? cqx::kJHer : cqx::kJNonHer)
+                                            : (L[m].her ? cqx::kKHer : cqx::kKNonHer);
+        for (int k = 0; k < (two ? 2 : 1); ++k, ++p) {
+          cqx::Plane &P = ps.p[p];
+          P.X = c->d_planes.p + p * nbc2;
+          P.A = c->d_acc.p + p * nbc2;
+          P.kind = kind;
+          P.sign = (k == 1 && L[m].her) ? -1.0 : 1.0;  // conj of the imaginary part
+          P.mat = m;
+        }
+      }
+      ps.n = nplane;
+      int nJ = 0, nK = 0;
+      for (int m = 0; m < nmat; ++m) (L[m].type == CQX_COULOMB ? nJ : nK)++;
+      bool pure_d = false;
+      for (auto &sh : c->basis.shells)
+        if (sh.l > 1 && sh.pure) pure_d = true;
+      HIPCHK(hipStreamSynchronize(s));
+      double bmax = 0.0;
+      for (double v : Bg) bmax = std::max(bmax, v);
+      // candidate lists per class pair (bra class >= ket class)
+      struct CP { int cb, ck; long long seg, ncand; std::vector<long long> off; };
+      std::vector<CP> cps;
+      long long total = 0;
+      for (int cb = 0; cb < kNumClass; ++cb)
+        for (int ck = 0; ck <= cb; ++ck) {
+          const auto &vb = c->cls[cb], &vk = c->cls[ck];
+          if (vb.empty() || vk.empty()) continue;
+          CP cp;
+          cp.cb = cb;
+          cp.ck = ck;
+          cp.off.resize(vb.size() + 1);
+          cp.off[0] = 0;
+          for (size_t i = 0; i < vb.size(); ++i) {
+            long long jmax = (long long)vk.size();
+            if (screen) {
+              if (bmax * vb[i].Q * vk[0].Q < c->thresh) {
+                jmax = 0;
+              } else {
+                // largest prefix with Q_i Q_j bmax >= thresh (Q sorted descending)
+                const double need = c->thresh / (bmax * vb[i].Q);
+                long long lo = 0, hi = (long long)vk.size();
+                while (lo < hi) {
+                  long long mid = (lo + hi) / 2;
+                  if (vk[mid].Q >= need) lo = mid + 1; else hi = mid;
+                }
+                jmax = lo;
+              }
+            }
+            if (cb == ck) jmax = std::min<long long>(jmax, (long long)i + 1);
+            cp.off[i + 1] = cp.off[i] + jmax;
+          }
+          cp.ncand = cp.off.back();
+          if (cp.ncand == 0) continue;
+          cp.seg = total;
+          total += cp.ncand;
+          cps.push_back(std::move(cp));
+        }
+      c->stats.n_candidates += total;
+      c->d_quart.ensure((size_t)std::max<long long>(total, 1));
+      c->d_count.ensure(std::max<size_t>(cps.size(), 1));
+      // counters cleared by a fill kernel (no memset nodes if the caller captures a graph)
+      HIPCHK(cqx::launch_zero(reinterpret_cast<double *>(c->d_count.p), std::max<size_t>(cps.size(), 1), s));
+      c->d_psum.ensure(std::max<size_t>(cps.size(), 1));
+      HIPCHK(cqx::launch_zero(c->d_psum.p, std::max<size_t>(cps.size(), 1), s));
+      size_t cand_total = 0;
+      for (auto &cp : cps) cand_total += cp.off.size();
+      c->d_cand.ensure(std::max<size_t>(cand_total, 1));
+      std::vector<long long> cand_all;
+      cand_all.reserve(cand_total);
+      for (auto &cp : cps) cand_all.insert(cand_all.end(), cp.off.begin(), cp.off.end());
+      if (!cand_all.empty())
+        HIPCHK(hipMemcpyAsync(c->d_cand.p, cand_all.data(), sizeof(long long) * cand_all.size(),
+                              hipMemcpyHostToDevice, s));
+      size_t coff = 0;
+      for (size_t k = 0; k < cps.size(); ++k) {
+        auto &cp = cps[k];
+        cqx::ListLaunch LL{};
+        LL.bra = c->d_pairs[cp.cb].p;
+        LL.ket = c->d_pairs[cp.ck].p;
+        LL.same_class = cp.cb == cp.ck;
+        LL.cand_off = c->d_cand.p + coff;
+        LL.nbra = (int)c->cls[cp.cb].size();
+        LL.ncand = cp.ncand;
+        LL.Bg = c->d_Bg.p;
+        LL.Bm = c->d_Bm.p;
+        LL.nmat = nmat;
+        LL.ns = NS;
+        LL.thresh = c->thresh;
+        LL.screen = screen;
+        LL.out = c->d_quart.p + cp.seg;
+        LL.count = c->d_count.p + k;
+        LL.prim_sum = c->d_psum.p + k;
+        LL.cap = cp.ncand;
+        HIPCHK(cqx::launch_quartet_list(LL, s));
+        coff += cp.off.size();
+      }
+      std::vector<unsigned long long> counts(cps.size());
+      std::vector<double> psum(cps.size());
+      if (!cps.empty()) {
+        HIPCHK(hipMemcpyAsync(counts.data(), c->d_count.p, sizeof(unsigned long long) * cps.size(),
+                              hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(psum.data(), c->d_psum.p, sizeof(double) * cps.size(),
+                              hipMemcpyDeviceToHost, s));
+      }
+      HIPCHK(hipStreamSynchronize(s));
+      // ERI + digestion
+      HIPCHK(hipEventRecord(c->ev0, s));
+      for (size_t k = 0; k < cps.size(); ++k) {
+        if (!counts[k]) continue;
+        auto &cp = cps[k];
+        cqx::EriLaunch E{};
+        class_ls(cp.cb, E.la, E.lb);
+        class_ls(cp.ck, E.lc, E.ld);
+        E.quartets = c->d_quart.p + cp.seg;
+        E.nq = (long long)counts[k];
+        E.bra = c->d_pairs[cp.cb].p;
+        E.ket = c->d_pairs[cp.ck].p;
+        E.prims = c->d_prims.p;
+        E.rys = c->d_rys.p;
+        E.planes = ps;
+        E.store_mode = 0;
+        const int ncq = cqx::ncart(E.la) * cqx::ncart(E.lb) * cqx::ncart(E.lc) * cqx::ncart(E.ld);
+        const bool fast = ncq <= c->fast_max_nc && cqx::eri_fast_available(E.la, E.lb, E.lc, E.ld);
+        hipError_t e = fast ? cqx::launch_eri_fast(E, s) : cqx::launch_eri_team(E, s);
+        HIPCHK(e);
+        c->stats.n_quartets += (long long)counts[k];
+        c->stats.n_prim_quartets_max += (long long)psum[k];
+        c->stats.flops_model += class_pair_flops(E.la, E.lb, E.lc, E.ld, pure_d, psum[k],
+                                                 (double)counts[k], is_complex ? 2 : 1, nJ, nK);
+        c->stats.n_launch_eri++;
+      }
+      HIPCHK(hipEventRecord(c->ev1, s));
+      // back to pure functions, symmetrise, accumulate into AX
+      for (int m = 0; m < nmat; ++m) {
+        const bool two = is_complex && !(L[m].her && L[m].type == CQX_COULOMB);
+        const double *are = c->d_acc.p + first_plane[m] * nbc2;
+        const double *aim = two ? c->d_acc.p + (first_plane[m] + 1) * nbc2 : nullptr;
+        HIPCHK(cqx::launch_from_cart(c->cmap, are, aim, L[m].her, is_complex, c->d_AX.p + m * nb2 * ew, s));
+      }
+      const hipMemcpyKind dk = mem == CQX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+      for (int m = 0; m < nmat; ++m)
+        HIPCHK(hipMemcpyAsync(L[m].AX, c->d_AX.p + m * nb2 * ew, sizeof(double) * nb2 * ew, dk, s));
+      HIPCHK(hipStreamSynchronize(s));
+      float ms = 0.0f;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      eri_ms_total += ms;
+      start += nmat;
+    }
+    c->stats.ms_eri = eri_ms_total;
+  } catch (const std::exception &e) {
+    return fail(c, CQX_ERR_HIP, e.what());
+  }
+  c->stats.ms_total =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  return CQX_OK;
+}
+
+int cqx_device_alloc(cqx_ctx *c, size_t bytes, void **ptr) {
+  if (!c || !ptr) return fail(c, CQX_ERR_ARG, "cqx_device_alloc: bad arguments");
+  if (hipSetDevice(c->device) != hipSuccess || hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess)
+    return fail(c, CQX_ERR_MEM, "hipMalloc failed");
+  return CQX_OK;
+}
+
+int cqx_device_free(cqx_ctx *c, void *ptr) {
+  if (!c) return fail(c, CQX_ERR_ARG, "cqx_device_free: null context");
+  if (ptr && hipFree(ptr) != hipSuccess) return fail(c, CQX_ERR_HIP, "hipFree failed");
+  return CQX_OK;
+}
+
+int cqx_memcpy(cqx_ctx *c, void *dst, const void *src, size_t bytes, int kind) {
+  if (!c || (bytes && (!dst || !src)) || kind < 0 || kind > 2)
+    return fail(c, CQX_ERR_ARG, "cqx_memcpy: bad arguments");
+  const hipMemcpyKind k[3] = {hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice};
+  if (hipSetDevice(c->device) != hipSuccess || hipMemcpyAsync(dst, src, bytes, k[kind], c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return fail(c, CQX_ERR_HIP, "hipMemcpy failed");
+  return CQX_OK;
+}
+
+int cqx_device_zero(cqx_ctx *c, void *ptr, size_t bytes) {
+  if (!c || (bytes && !ptr) || bytes % sizeof(double))
+    return fail(c, CQX_ERR_ARG, "cqx_device_zero: bad arguments (size must be a multiple of 8)");
+  if (hipSetDevice(c->device) != hipSuccess ||
+      cqx::launch_zero(static_cast<double *>(ptr), bytes / sizeof(double), c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return fail(c, CQX_ERR_HIP, "device zero failed");
+  return CQX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,59 @@
+// Data shared between host orchestration and the HIP kernels.
+#pragma once
+
+#include <cstdint>
+
+namespace cqx {
+
+constexpr int kMaxLDev = 3;    // device kernels cover s..f shells
+constexpr int kMaxPlanes = 16; // real accumulation planes per launch
+constexpr int kMaxMat = 8;     // contractions sharing one ERI evaluation
+
+// A significant shell pair in kernel orientation: shell a carries the higher
+// angular momentum (la >= lb).  s1 >= s2 is the same pair in the order the
+// reference loop visits it (direct.hpp:1659-1664), used for screening.
+struct PairRec {
+  int a, b;        // kernel-orientation shells
+  int s1, s2;      // reference-orientation shells
+  int pp_off, npp; // primitive pairs [pp_off, pp_off + npp)
+  int bfa, bfb;    // first Cartesian function of a and b
+  double Q;        // Schwarz bound of the pair
+  double AB[3];    // A - B
+  int kk;          // Ka * Kb (contraction lengths, before primitive screening)
+  int pad;
+};
+
+// One surviving primitive pair of a shell pair (kernel orientation).
+struct PrimPair {
+  double p;        // alpha_a + alpha_b
+  double P[3];     // Gaussian product centre
+  double PA[3];    // P - A
+  double K;        // c_a c_b exp(-alpha_a alpha_b / p |AB|^2)
+};
+
+// Digestion target: one real plane of a (possibly complex) contraction.
+enum PlaneKind : int { kJHer = 0, kKHer = 1, kJNonHer = 2, kKNonHer = 3 };
+
+struct Plane {
+  const double *X;  // Cartesian density plane, NBc x NBc column-major
+  double *A;        // Cartesian accumulator plane
+  int kind;         // PlaneKind
+  double sign;      // +1 real part; -1 imaginary part of a conjugated X
+  int mat;          // contraction index (screening mask bit)
+  int pad;
+};
+
+struct PlaneSet {
+  Plane p[kMaxPlanes];
+  int n;
+  int nbc;  // Cartesian dimension NBc
+};
+
+// A quartet: bra / ket pair indices within their class arrays plus the
+// mask of contractions whose separated screen it passed.
+struct Quartet {
+  uint32_t bra;
+  uint32_t ket_mask;  // ket index (low 24 bits) | mask << 24
+};
+
+}  // namespace cqx

@@ -1,0 +1,534 @@
+// HIP kernels of the direct J/K build for gfx950 (MI355X): screening and
+// quartet-list construction, the generic Rys ERI + digestion kernel, and the
+// matrix transforms around them.
+//
+// Reference algorithm: GTODirectTPIContraction::directScaffoldNew
+// (include/particleintegrals/contract/direct.hpp:1412-2123).  The quartet set,
+// its screening and the degeneracy factors follow the reference loop; ERIs
+// are evaluated by Rys quadrature in Cartesian form and the density /
+// result matrices are carried in the Cartesian AO basis (T X T^T and
+// T^T A T around the build), which is the same algebra as transforming every
+// integral batch to pure functions.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "rys.hpp"
+
+namespace cqx {
+
+namespace {
+
+constexpr double kTwoPi52 = 34.98683665524972;  // 2 pi^(5/2)
+
+__device__ __forceinline__ int ncart_d(int l) { return (l + 1) * (l + 2) / 2; }
+
+// Cartesian exponents of component c of an l shell (lx descending, then ly).
+__device__ __forceinline__ void cart_exp(int l, int c, int &lx, int &ly, int &lz) {
+  int k = c;
+  lx = l;
+  while (k >= l - lx + 1) {
+    k -= l - lx + 1;
+    --lx;
+  }
+  lz = k;
+  ly = l - lx - lz;
+}
+
+__device__ __forceinline__ void atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
+
+// ---------------------------------------------------------------------------
+// Generic Rys ERI kernel: one 64-lane wave per shell quartet.
+// Per primitive quartet, lanes r < NR fetch the Rys nodes/weights, lanes
+// (r, dir) < 3 NR build the 2D integrals I_dir(ia, ib, ic, id) (vertical
+// recursion on the Rys variables, then horizontal transfer to b and d) into
+// LDS, and every lane accumulates its share of the Cartesian quartet in LDS.
+// ---------------------------------------------------------------------------
+struct TeamDims {
+  int la, lb, lc, ld, nca, ncb, ncc, ncd, nc, ni, nk, nr, nh;
+};
+
+__device__ TeamDims team_dims(int la, int lb, int lc, int ld) {
+  TeamDims d;
+  d.la = la; d.lb = lb; d.lc = lc; d.ld = ld;
+  d.nca = ncart_d(la); d.ncb = ncart_d(lb); d.ncc = ncart_d(lc); d.ncd = ncart_d(ld);
+  d.nc = d.nca * d.ncb * d.ncc * d.ncd;
+  d.ni = la + lb + 1;
+  d.nk = lc + ld + 1;
+  d.nr = (la + lb + lc + ld) / 2 + 1;
+  d.nh = (la + 1) * (lb + 1) * (lc + 1) * (ld + 1);
+  return d;
+}
+
+// 2D integrals of one Rys root along one direction, written to H[nh].
+__device__ void rys_2d(const TeamDims &D, double t2, double p, double q, double PAd,
+                       double QCd, double PQd, double ABd, double CDd, double scale,
+                       double *__restrict__ H) {
+  double G[7][7];   // [i <= la+lb][k <= lc+ld]
+  double E[7][4][7];  // bra transfer: [i][j][k]
+  const double pq = p + q, rho = p * q / pq;
+  const double B00 = 0.5 * t2 / pq;
+  const double B10 = 0.5 / p * (1.0 - rho / p * t2);
+  const double B01 = 0.5 / q * (1.0 - rho / q * t2);
+  const double C00 = PAd - (q / pq) * t2 * PQd;
+  const double D00 = QCd + (p / pq) * t2 * PQd;
+  G[0][0] = scale;
+  if (D.ni > 1) G[1][0] = C00 * scale;
+  for (int i = 1; i + 1 < D.ni; ++i) G[i + 1][0] = C00 * G[i][0] + i * B10 * G[i - 1][0];
+  for (int k = 0; k + 1 < D.nk; ++k) {
+    G[0][k + 1] = D00 * G[0][k] + (k > 0 ? k * B01 * G[0][k - 1] : 0.0);
+    for (int i = 1; i < D.ni; ++i)
+      G[i][k + 1] = D00 * G[i][k] + (k > 0 ? k * B01 * G[i][k - 1] : 0.0) + i * B00 * G[i - 1][k];
+  }
+  // horizontal transfer on the bra: (i, j+1) = (i+1, j) + AB (i, j)
+  for (int i = 0; i < D.ni; ++i)
+    for (int k = 0; k < D.nk; ++k) E[i][0][k] = G[i][k];
+  for (int j = 1; j <= D.lb; ++j)
+    for (int i = 0; i + j < D.ni; ++i)
+      for (int k = 0; k < D.nk; ++k) E[i][j][k] = E[i + 1][j - 1][k] + ABd * E[i][j - 1][k];
+  // horizontal transfer on the ket for every (ia, ib)
+  for (int ia = 0; ia <= D.la; ++ia)
+    for (int ib = 0; ib <= D.lb; ++ib) {
+      double F[7][4];
+      for (int k = 0; k < D.nk; ++k) F[k][0] = E[ia][ib][k];
+      for (int l = 1; l <= D.ld; ++l)
+        for (int k = 0; k + l < D.nk; ++k) F[k][l] = F[k + 1][l - 1] + CDd * F[k][l - 1];
+      for (int ic = 0; ic <= D.lc; ++ic)
+        for (int id = 0; id <= D.ld; ++id)
+          H[((ia * (D.lb + 1) + ib) * (D.lc + 1) + ic) * (D.ld + 1) + id] = F[ic][id];
+    }
+}
+
+// One digestion term over LDS quartet values sI (canonical labels a,b,c,d =
+// roles 0..3): A(ar, ac) += coef * sum_{xr, xc} I * X(xr, xc), where the A
+// roles and the X roles together are {a, b, c, d}.
+__device__ void digest_term(const double *__restrict__ sI, const int n[4], const int bf[4],
+                            int ar, int ac, int xr, int xc, double coef,
+                            const double *__restrict__ X, double *__restrict__ A, int N,
+                            int lane) {
+  const int nt = n[ar] * n[ac];
+  const int ns = n[xr] * n[xc];
+  for (int t = lane; t < nt; t += 64) {
+    int idx[4];
+    idx[ar] = t / n[ac];
+    idx[ac] = t % n[ac];
+    double sum = 0.0;
+    for (int u = 0; u < ns; ++u) {
+      idx[xr] = u / n[xc];
+      idx[xc] = u % n[xc];
+      const double I = sI[((idx[0] * n[1] + idx[1]) * n[2] + idx[2]) * n[3] + idx[3]];
+      sum += I * X[(bf[xr] + idx[xr]) + (size_t)(bf[xc] + idx[xc]) * N];
+    }
+    atomic_add(&A[(bf[ar] + idx[ar]) + (size_t)(bf[ac] + idx[ac]) * N], coef * sum);
+  }
+}
+
+// The reference digestion formulas (direct.hpp:1858-2002) in canonical
+// labels 0..3 = shells 1..4, for one real plane.
+__device__ void digest_plane(const double *sI, const int n[4], const int bf[4],
+                             const Plane &P, int N, int lane) {
+  const double *X = P.X;
+  double *A = P.A;
+  enum { a = 0, b = 1, c = 2, d = 3 };
+  switch (P.kind) {
+    case kJHer:  // AX(1,2) += X(4,3) I ; AX(4,3) += X(1,2) I
+      digest_term(sI, n, bf, a, b, d, c, 1.0, X, A, N, lane);
+      digest_term(sI, n, bf, d, c, a, b, 1.0, X, A, N, lane);
+      break;
+    case kKHer: {  // four updates with 1/2 conj(X)
+      const double h = 0.5 * P.sign;
+      digest_term(sI, n, bf, a, c, d, b, h, X, A, N, lane);
+      digest_term(sI, n, bf, d, b, a, c, h, X, A, N, lane);
+      digest_term(sI, n, bf, d, a, b, c, h, X, A, N, lane);
+      digest_term(sI, n, bf, b, c, d, a, h, X, A, N, lane);
+    } break;
+    case kJNonHer:  // (1,2),(2,1) += 1/2 (X43 + X34) I ; (3,4),(4,3) += 1/2 (X21 + X12) I
+      digest_term(sI, n, bf, a, b, d, c, 0.5, X, A, N, lane);
+      digest_term(sI, n, bf, a, b, c, d, 0.5, X, A, N, lane);
+      digest_term(sI, n, bf, b, a, d, c, 0.5, X, A, N, lane);
+      digest_term(sI, n, bf, b, a, c, d, 0.5, X, A, N, lane);
+      digest_term(sI, n, bf, c, d, b, a, 0.5, X, A, N, lane);
+      digest_term(sI, n, bf, c, d, a, b, 0.5, X, A, N, lane);
+      digest_term(sI, n, bf, d, c, b, a, 0.5, X, A, N, lane);
+      digest_term(sI, n, bf, d, c, a, b, 0.5, X, A, N, lane);
+      break;
+    case kKNonHer:  // eight updates with 1/2 X
+      digest_term(sI, n, bf, c, a, d, b, 0.5, X, A, N, lane);
+      digest_term(sI, n, bf, d, b, c, a, 0.5, X, A, N, lane);
+      digest_term(sI, n, bf, d, a, c, b, 0.5, X, A, N, lane);
+      digest_term(sI, n, bf, c, b, d, a, 0.5, X, A, N, lane);
+      digest_term(sI, n, bf, a, c, b, d, 0.5, X, A, N, lane);
+      digest_term(sI, n, bf, b, d, a, c, 0.5, X, A, N, lane);
+      digest_term(sI, n, bf, a, d, b, c, 0.5, X, A, N, lane);
+      digest_term(sI, n, bf, b, c, a, d, 0.5, X, A, N, lane);
+      break;
+  }
+}
+
+__global__ __launch_bounds__(64) void eri_team_kernel(EriLaunch a) {
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x;
+  const TeamDims D = team_dims(a.la, a.lb, a.lc, a.ld);
+  double *s_x = lds;
+  double *s_w = lds + 8;
+  double *s_H = lds + 16;
+  double *s_I = s_H + D.nr * 3 * D.nh;
+  for (long long q = blockIdx.x; q < a.nq; q += gridDim.x) {
+    const Quartet qt = a.quartets[q];
+    const PairRec bp = a.bra[qt.bra];
+    const PairRec kp = a.ket[qt.ket_mask & 0xFFFFFFu];
+    for (int o = lane; o < D.nc; o += 64) s_I[o] = 0.0;
+    for (int i = 0; i < bp.npp; ++i) {
+      const PrimPair A = a.prims[bp.pp_off + i];
+      for (int j = 0; j < kp.npp; ++j) {
+        const PrimPair B = a.prims[kp.pp_off + j];
+        const double pq = A.p + B.p, rho = A.p * B.p / pq;
+        const double PQ[3] = {A.P[0] - B.P[0], A.P[1] - B.P[1], A.P[2] - B.P[2]};
+        const double T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+        const double pref = A.K * B.K * kTwoPi52 / (A.p * B.p * sqrt(pq));
+        __syncthreads();
+        if (lane < D.nr) {
+          double x, w;
+          rys_root_rt(D.nr, lane, T, a.rys, x, w);
+          s_x[lane] = x;
+          s_w[lane] = w * pref;
+        }
+        __syncthreads();
+        if (lane < 3 * D.nr) {
+          const int r = lane / 3, dir = lane % 3;
+          rys_2d(D, s_x[r], A.p, B.p, A.PA[dir], B.PA[dir], PQ[dir], bp.AB[dir], kp.AB[dir],
+                 dir == 2 ? s_w[r] : 1.0, s_H + (r * 3 + dir) * D.nh);
+        }
+        __syncthreads();
+        for (int o = lane; o < D.nc; o += 64) {
+          int rem = o;
+          const int id = rem % D.ncd; rem /= D.ncd;
+          const int ic = rem % D.ncc; rem /= D.ncc;
+          const int ib = rem % D.ncb; rem /= D.ncb;
+          const int ia = rem;
+          int e[4][3];
+          cart_exp(D.la, ia, e[0][0], e[0][1], e[0][2]);
+          cart_exp(D.lb, ib, e[1][0], e[1][1], e[1][2]);
+          cart_exp(D.lc, ic, e[2][0], e[2][1], e[2][2]);
+          cart_exp(D.ld, id, e[3][0], e[3][1], e[3][2]);
+          int h[3];
+          for (int dir = 0; dir < 3; ++dir)
+            h[dir] = ((e[0][dir] * (D.lb + 1) + e[1][dir]) * (D.lc + 1) + e[2][dir]) * (D.ld + 1) +
+                     e[3][dir];
+          double s = 0.0;
+          for (int r = 0; r < D.nr; ++r) {
+            const double *Hr = s_H + r * 3 * D.nh;
+            s += Hr[h[0]] * Hr[D.nh + h[1]] * Hr[2 * D.nh + h[2]];
+          }
+          s_I[o] += s;
+        }
+      }
+    }
+    __syncthreads();
+    if (a.store_mode) {
+      double *out = a.store + a.store_off[q];
+      for (int o = lane; o < D.nc; o += 64) out[o] = s_I[o];
+    } else {
+      // degeneracy of the quartet in the 8-fold loop (direct.hpp:1664-1667,
+      // 1793-1802), times the 1/2 of direct.hpp:1849-1850
+      const bool same_pair = (bp.a == kp.a && bp.b == kp.b);
+      const double deg = (bp.a == bp.b ? 1.0 : 2.0) * (kp.a == kp.b ? 1.0 : 2.0) * (same_pair ? 1.0 : 2.0);
+      for (int o = lane; o < D.nc; o += 64) s_I[o] *= 0.5 * deg;
+      __syncthreads();
+      const int n[4] = {D.nca, D.ncb, D.ncc, D.ncd};
+      const int bf[4] = {bp.bfa, bp.bfb, kp.bfa, kp.bfb};
+      const unsigned mask = qt.ket_mask >> 24;
+      for (int p = 0; p < a.planes.n; ++p) {
+        const Plane &P = a.planes.p[p];
+        if (!((mask >> P.mat) & 1u)) continue;
+        digest_plane(s_I, n, bf, P, a.planes.nbc, lane);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Screening: quartet candidates (bra i, ket j) after the Schwarz pre-cut,
+// tested with the reference's Schwarz x shell-block-norm rule in reference
+// labels (direct.hpp:1689-1789); survivors appended to a list with the mask
+// of contractions that pass their own separated screen.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void quartet_list_kernel(ListLaunch a) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const int lane = threadIdx.x & 63;
+  for (long long base = (long long)blockIdx.x * blockDim.x; base < a.ncand; base += stride) {
+    const long long t = base + threadIdx.x;
+    bool pass = false;
+    unsigned mask = 0;
+    int bi = 0, kj = 0;
+    if (t < a.ncand) {
+      // bra index: last i with cand_off[i] <= t
+      int lo = 0, hi = a.nbra - 1;
+      while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (a.cand_off[mid] <= t) lo = mid; else hi = mid - 1;
+      }
+      bi = lo;
+      kj = (int)(t - a.cand_off[lo]);
+      const PairRec P = a.bra[bi], Q = a.ket[kj];
+      const bool p_first = (P.s1 > Q.s1) || (P.s1 == Q.s1 && P.s2 >= Q.s2);
+      const int s1 = p_first ? P.s1 : Q.s1, s2 = p_first ? P.s2 : Q.s2;
+      const int s3 = p_first ? Q.s1 : P.s1, s4 = p_first ? Q.s2 : P.s2;
+      const int ns = a.ns;
+      if (!a.screen) {
+        pass = true;
+        mask = (1u << a.nmat) - 1u;
+      } else {
+        const double qq = P.Q * Q.Q;
+        auto bmax = [&](const double *B) {
+          double m = fmax(B[s1 + (size_t)s2 * ns], fmax(B[s1 + (size_t)s3 * ns], B[s2 + (size_t)s3 * ns]));
+          m = fmax(m, B[s3 + (size_t)s4 * ns]);
+          m = fmax(m, fmax(B[s1 + (size_t)s4 * ns], B[s2 + (size_t)s4 * ns]));
+          return m;
+        };
+        if (!(bmax(a.Bg) * qq < a.thresh)) {
+          if (a.nmat == 1) {
+            mask = 1u;
+          } else {
+            for (int m = 0; m < a.nmat; ++m)
+              if (!(bmax(a.Bm + (size_t)m * ns * ns) * qq < a.thresh)) mask |= 1u << m;
+          }
+          pass = mask != 0u;
+        }
+      }
+    }
+    // wave-aggregated append
+    const unsigned long long bal = __ballot(pass);
+    if (bal) {
+      double pk = pass ? (double)a.bra[bi].kk * (double)a.ket[kj].kk : 0.0;
+      for (int off = 32; off > 0; off >>= 1) pk += __shfl_xor(pk, off);
+      const int leader = __ffsll((long long)bal) - 1;
+      unsigned long long start = 0;
+      if (lane == leader) {
+        start = atomicAdd(a.count, (unsigned long long)__popcll(bal));
+        atomic_add(a.prim_sum, pk);
+      }
+      start = __shfl(start, leader);
+      if (pass) {
+        const unsigned long long below = bal & ((1ull << lane) - 1ull);
+        const unsigned long long slot = start + __popcll(below);
+        if ((long long)slot < a.cap) {
+          Quartet qv;
+          qv.bra = (uint32_t)bi;
+          qv.ket_mask = (uint32_t)kj | (mask << 24);
+          a.out[slot] = qv;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ShellBlockNorm (direct.hpp:73-105): B(s1, s2) = max_i sum_j |X(i, j)| over
+// the (s1, s2) block.
+// ---------------------------------------------------------------------------
+__global__ void block_norm_kernel(const double *X, int nmat, int cplx, int nb, int ns,
+                                  const int *sh_bf, const int *sh_nf, double *Bm) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long tot = (long long)nmat * ns * ns;
+  if (t >= tot) return;
+  const int m = (int)(t / ((long long)ns * ns));
+  const int rem = (int)(t % ((long long)ns * ns));
+  const int s1 = rem % ns, s2 = rem / ns;
+  const size_t nb2 = (size_t)nb * nb;
+  const double *Xm = X + (size_t)m * nb2 * (cplx ? 2 : 1);
+  double mx = 0.0;
+  for (int i = 0; i < sh_nf[s1]; ++i) {
+    double row = 0.0;
+    for (int j = 0; j < sh_nf[s2]; ++j) {
+      const size_t idx = (sh_bf[s1] + i) + (size_t)(sh_bf[s2] + j) * nb;
+      row += cplx ? hypot(Xm[2 * idx], Xm[2 * idx + 1]) : fabs(Xm[idx]);
+    }
+    mx = fmax(mx, row);
+  }
+  Bm[(size_t)m * ns * ns + s1 + (size_t)s2 * ns] = mx;
+}
+
+__global__ void block_norm_finish_kernel(int nmat, int ns, const int *her, double *Bm, double *Bg) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)ns * ns) return;
+  const int k = (int)(t % ns), l = (int)(t / ns);
+  double g = 0.0;
+  for (int m = 0; m < nmat; ++m) {
+    double *B = Bm + (size_t)m * ns * ns;
+    double v = B[k + (size_t)l * ns];
+    if (!her[m]) v = fmax(v, B[l + (size_t)k * ns]);  // symmetrised, direct.hpp:1537-1545
+    g = fmax(g, v);
+  }
+  Bg[k + (size_t)l * ns] = g;
+}
+
+__global__ void block_norm_symm_kernel(int nmat, int ns, const int *her, double *Bm) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nmat * ns * ns) return;
+  const int m = (int)(t / ((long long)ns * ns));
+  if (her[m]) return;
+  const int rem = (int)(t % ((long long)ns * ns));
+  const int k = rem % ns, l = rem / ns;
+  if (k <= l) return;
+  double *B = Bm + (size_t)m * ns * ns;
+  const double mx = fmax(B[k + (size_t)l * ns], B[l + (size_t)k * ns]);
+  B[k + (size_t)l * ns] = mx;
+  B[l + (size_t)k * ns] = mx;
+}
+
+__device__ __forceinline__ int sph_off(int l) {
+  // sum_{k<l} (2k+1) ncart(k): 0, 1, 10, 40, 110
+  const int off[5] = {0, 1, 10, 40, 110};
+  return off[l];
+}
+
+// coefficient T(cart comp c, pure function m) of a shell
+__device__ __forceinline__ double tcoef(const CartMap &M, int s, int c, int m) {
+  if (!M.sh_pure[s]) return c == m ? 1.0 : 0.0;
+  const int l = M.sh_l[s];
+  return M.sph[sph_off(l) + m * ncart_d(l) + c];
+}
+
+__device__ __forceinline__ int nfunc(const CartMap &M, int s) {
+  return M.sh_pure[s] ? 2 * M.sh_l[s] + 1 : ncart_d(M.sh_l[s]);
+}
+
+__global__ void to_cart_kernel(CartMap M, const double *X, int cplx, double *Pre, double *Pim) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)M.nbc * M.nbc) return;
+  const int i = (int)(t % M.nbc), j = (int)(t / M.nbc);
+  const int si = M.cart_shell[i], ci = M.cart_comp[i];
+  const int sj = M.cart_shell[j], cj = M.cart_comp[j];
+  double re = 0.0, im = 0.0;
+  for (int mu = 0; mu < nfunc(M, si); ++mu) {
+    const double ti = tcoef(M, si, ci, mu);
+    if (ti == 0.0) continue;
+    for (int nu = 0; nu < nfunc(M, sj); ++nu) {
+      const double tj = tcoef(M, sj, cj, nu);
+      if (tj == 0.0) continue;
+      const size_t idx = (M.sh_bf[si] + mu) + (size_t)(M.sh_bf[sj] + nu) * M.nb;
+      if (cplx) {
+        re += ti * tj * X[2 * idx];
+        im += ti * tj * X[2 * idx + 1];
+      } else {
+        re += ti * tj * X[idx];
+      }
+    }
+  }
+  Pre[t] = re;
+  if (Pim) Pim[t] = im;
+}
+
+__global__ void from_cart_kernel(CartMap M, const double *Are, const double *Aim, int her,
+                                 int cplx, double *AX) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)M.nb * M.nb) return;
+  const int mu = (int)(t % M.nb), nu = (int)(t / M.nb);
+  const int sm = M.bf_shell[mu], cm = M.bf_comp[mu];
+  const int sn = M.bf_shell[nu], cn = M.bf_comp[nu];
+  const int ncm = ncart_d(M.sh_l[sm]), ncn = ncart_d(M.sh_l[sn]);
+  double a_re = 0.0, a_im = 0.0, b_re = 0.0, b_im = 0.0;
+  for (int i = 0; i < ncm; ++i) {
+    const double ti = tcoef(M, sm, i, cm);
+    if (ti == 0.0) continue;
+    for (int j = 0; j < ncn; ++j) {
+      const double tj = tcoef(M, sn, j, cn);
+      if (tj == 0.0) continue;
+      const size_t ij = (M.sh_bfc[sm] + i) + (size_t)(M.sh_bfc[sn] + j) * M.nbc;
+      const size_t ji = (M.sh_bfc[sn] + j) + (size_t)(M.sh_bfc[sm] + i) * M.nbc;
+      a_re += ti * tj * Are[ij];
+      b_re += ti * tj * Are[ji];
+      if (Aim) {
+        a_im += ti * tj * Aim[ij];
+        b_im += ti * tj * Aim[ji];
+      }
+    }
+  }
+  double re, im;
+  if (her) {  // 1/2 (A + A^H), direct.hpp:2040-2049
+    re = 0.5 * (a_re + b_re);
+    im = 0.5 * (a_im - b_im);
+  } else {    // 1/2 A, direct.hpp:2051-2062
+    re = 0.5 * a_re;
+    im = 0.5 * a_im;
+  }
+  if (cplx) {
+    AX[2 * t] += re;
+    AX[2 * t + 1] += im;
+  } else {
+    AX[t] += re;
+  }
+}
+
+__global__ void zero_kernel(double *p, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0.0;
+}
+
+inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+hipError_t launch_eri_team(const EriLaunch &a, hipStream_t s) {
+  if (a.nq <= 0) return hipSuccess;
+  const int nr = (a.la + a.lb + a.lc + a.ld) / 2 + 1;
+  const int nh = (a.la + 1) * (a.lb + 1) * (a.lc + 1) * (a.ld + 1);
+  auto nc = [](int l) { return (l + 1) * (l + 2) / 2; };
+  const int ncq = nc(a.la) * nc(a.lb) * nc(a.lc) * nc(a.ld);
+  const size_t lds = sizeof(double) * (16 + (size_t)nr * 3 * nh + ncq);
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute((const void *)eri_team_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+    attr_set = true;
+  }
+  const long long grid = a.nq < 65536 ? a.nq : 65536;
+  hipLaunchKernelGGL(eri_team_kernel, dim3((unsigned)grid), dim3(64), lds, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_quartet_list(const ListLaunch &a, hipStream_t s) {
+  if (a.ncand <= 0) return hipSuccess;
+  long long blocks = (a.ncand + 255) / 256;
+  if (blocks > 262144) blocks = 262144;
+  hipLaunchKernelGGL(quartet_list_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_block_norms(const double *X, int nmat, int is_complex, const int *her, int nb,
+                              int ns, const int *sh_bf, const int *sh_nf, double *Bm, double *Bg,
+                              hipStream_t s) {
+  const long long n1 = (long long)nmat * ns * ns;
+  hipLaunchKernelGGL(block_norm_kernel, dim3(nblocks(n1, 256)), dim3(256), 0, s, X, nmat,
+                     is_complex, nb, ns, sh_bf, sh_nf, Bm);
+  hipLaunchKernelGGL(block_norm_symm_kernel, dim3(nblocks(n1, 256)), dim3(256), 0, s, nmat, ns,
+                     her, Bm);
+  hipLaunchKernelGGL(block_norm_finish_kernel, dim3(nblocks((long long)ns * ns, 256)), dim3(256),
+                     0, s, nmat, ns, her, Bm, Bg);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_cart(const CartMap &m, const double *X, int is_complex, double *pre,
+                          double *pim, hipStream_t s) {
+  hipLaunchKernelGGL(to_cart_kernel, dim3(nblocks((long long)m.nbc * m.nbc, 256)), dim3(256), 0,
+                     s, m, X, is_complex, pre, pim);
+  return hipGetLastError();
+}
+
+hipError_t launch_from_cart(const CartMap &m, const double *are, const double *aim, int her,
+                            int is_complex, double *AX, hipStream_t s) {
+  hipLaunchKernelGGL(from_cart_kernel, dim3(nblocks((long long)m.nb * m.nb, 256)), dim3(256), 0,
+                     s, m, are, aim, her, is_complex, AX);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero(double *p, size_t n, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(zero_kernel, dim3(nblocks((long long)n, 256)), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace cqx

@@ -1,0 +1,81 @@
+// Host-callable launchers of the HIP kernels (implemented in *.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "device_types.hpp"
+
+namespace cqx {
+
+struct EriLaunch {
+  int la, lb, lc, ld;
+  const Quartet *quartets;
+  long long nq;
+  const PairRec *bra, *ket;
+  const PrimPair *prims;
+  const double *rys;
+  PlaneSet planes;
+  double *store;                 // STORE mode: contracted Cartesian quartets
+  const long long *store_off;    // per quartet offset into store
+  int store_mode;
+};
+
+// Generic one-wave-per-quartet kernel (any class up to (ff|ff)).
+hipError_t launch_eri_team(const EriLaunch &a, hipStream_t s);
+// Fast thread-per-quartet kernels for the low classes; returns
+// hipErrorNotSupported when the class has no specialised kernel.
+hipError_t launch_eri_fast(const EriLaunch &a, hipStream_t s);
+bool eri_fast_available(int la, int lb, int lc, int ld);
+
+struct ListLaunch {
+  const PairRec *bra, *ket;
+  int same_class;
+  const long long *cand_off;  // nbra + 1 prefix of candidate counts
+  int nbra;
+  long long ncand;
+  const double *Bg;           // NS x NS global block norms
+  const double *Bm;           // nmat x NS x NS per-matrix norms
+  int nmat, ns;
+  double thresh;
+  int screen;
+  Quartet *out;
+  unsigned long long *count;
+  double *prim_sum;           // sum over survivors of Ka Kb Kc Kd
+  long long cap;
+};
+hipError_t launch_quartet_list(const ListLaunch &a, hipStream_t s);
+
+// Shell-block infinity norms (max absolute row sum) of nmat spherical
+// matrices; symmetrised for non-Hermitian ones; global max over matrices.
+hipError_t launch_block_norms(const double *X, int nmat, int is_complex,
+                              const int *her, int nb, int ns,
+                              const int *sh_bf, const int *sh_nf,
+                              double *Bm, double *Bg, hipStream_t s);
+
+// Cartesian <-> pure transforms of whole matrices.
+struct CartMap {
+  const int *cart_shell;  // Cartesian function -> shell
+  const int *cart_comp;   // Cartesian function -> component within shell
+  const int *sh_bf;       // shell -> first pure-or-Cartesian function
+  const int *sh_bfc;      // shell -> first Cartesian function
+  const int *sh_l;
+  const int *sh_pure;
+  const double *sph;      // packed solid harmonics: l-blocks of (2l+1) x ncart(l)
+  const int *bf_shell;    // pure-or-Cartesian function -> shell
+  const int *bf_comp;     // pure-or-Cartesian function -> index within shell
+  int nb, nbc, ns;
+};
+// X (NB x NB, real or interleaved complex) -> real planes of T X T^T
+hipError_t launch_to_cart(const CartMap &m, const double *X, int is_complex,
+                          double *plane_re, double *plane_im, hipStream_t s);
+// AX += finalize(T^T A T): mode 0 Hermitian (1/2 (A + A^H)), 1 non-Hermitian
+// (1/2 A); A_im may be null (real accumulator only).
+hipError_t launch_from_cart(const CartMap &m, const double *A_re,
+                            const double *A_im, int her, int is_complex,
+                            double *AX, hipStream_t s);
+
+hipError_t launch_zero(double *p, size_t n, hipStream_t s);
+
+}  // namespace cqx

@@ -1,0 +1,163 @@
+/*
+ * cqx -- MI355X-native direct Fock-build (J/K) engine, C ABI.
+ *
+ * This is the drop-in boundary for ChronusQ's direct two-body contraction:
+ *   GTODirectTPIContraction<MatsT,IntsT>::twoBodyContract
+ *     (include/particleintegrals/twopints/gtodirecttpi.hpp:199-206)
+ *   -> directScaffoldNew
+ *     (include/particleintegrals/contract/direct.hpp:1412-2123)
+ * and the DirectTPI state it reads (gtodirecttpi.hpp:30-141).  A C++
+ * adapter deriving from TPIContractions<MatsT,IntsT> (twopints.hpp:172-260)
+ * calls these entry points; see include/cqx/cqx.hpp for the mirror of the
+ * ChronusQ-side classes.
+ *
+ * All entry points return 0 on success and a non-zero cqx_status on error;
+ * the message is available from cqx_last_error().  ChronusQ's CErr throws
+ * std::runtime_error("FATAL") (include/cerr.hpp:33-51); the C++ adapter
+ * converts a non-zero status into that exception.
+ *
+ * Matrices are column-major, M(i,j) = M[i + j*NB], as cqmatrix::Matrix
+ * stores them (include/matrix/matrix.hpp:189).  Complex matrices are
+ * interleaved (re, im) pairs, the layout of std::complex<double>.
+ */
+#ifndef CQX_H
+#define CQX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum cqx_status {
+  CQX_OK = 0,
+  CQX_ERR_ARG = 1,     /* invalid argument */
+  CQX_ERR_HIP = 2,     /* HIP runtime error */
+  CQX_ERR_MEM = 3,     /* device or host allocation failed */
+  CQX_ERR_BASIS = 4,   /* basis-set parse or lookup failure */
+  CQX_ERR_UNSUPPORTED = 5
+};
+
+enum cqx_contraction_type { /* TWOBODY_CONTRACTION_TYPE, twopints.hpp:31-41 */
+  CQX_COULOMB = 0,
+  CQX_EXCHANGE = 1
+};
+
+enum cqx_memory { CQX_HOST = 0, CQX_DEVICE = 1 };
+
+typedef struct cqx_basis cqx_basis; /* host-side shell set */
+typedef struct cqx_ctx cqx_ctx;     /* device-resident DirectTPI state */
+
+/* ---------------------------------------------------------------------
+ * Shell sets.  A basis is a list of contracted shells, each with its
+ * angular momentum, pure/Cartesian flag, origin (bohr) and primitives whose
+ * coefficients are already normalised in the libint2 convention that
+ * ChronusQ's BasisSet holds (libint2::Shell after renorm()).  Replaces the
+ * BasisSet the reference builds in src/basisset/basisset.cxx:98-196.
+ * ------------------------------------------------------------------- */
+
+/* Build from per-shell arrays.  alpha/coef are concatenated over shells
+ * (nprim[s] entries each); center is 3*nshell doubles; atom[s] labels the
+ * nuclear centre (shells with equal labels count as "same centre"). */
+int cqx_basis_create(int nshell, const int *l, const int *pure,
+                     const int *nprim, const double *alpha,
+                     const double *coef, const double *center,
+                     const int *atom, cqx_basis **out);
+
+/* Build from G94 text for a molecule, with the parsing and normalisation
+ * rules of ReferenceBasisSet (src/basisset/reference.cxx:127-357): SP
+ * shells split into S and P, pure functions iff l > 1 unless force_cart,
+ * shells ordered by atom then file order.  Z[natom], xyz[3*natom] bohr. */
+int cqx_basis_from_g94(const char *g94_text, int natom, const int *Z,
+                       const double *xyz, int force_cart, cqx_basis **out);
+
+int cqx_basis_destroy(cqx_basis *b);
+int cqx_basis_dims(const cqx_basis *b, int *nshell, int *nbasis,
+                   int *max_prim, int *max_l);
+/* per-shell l, pure, nprim, first basis function, atom label */
+int cqx_basis_shell(const cqx_basis *b, int s, int *l, int *pure,
+                    int *nprim, int *bf, int *atom);
+/* normalised exponents/coefficients of shell s and its origin */
+int cqx_basis_shell_prims(const cqx_basis *b, int s, double *alpha,
+                          double *coef, double *origin);
+
+/* ---------------------------------------------------------------------
+ * Context: DirectTPI<double>(basis, basis, mol, threshSchwarz) plus the
+ * device-resident shell-pair data.  Creating it computes the significant
+ * shell-pair list (ShellPairData::computeShellPairs, basisset.cxx:652-705)
+ * and the Schwarz bounds on the GPU (DirectTPI::computeSchwarz,
+ * aointegrals_utilities.cxx:53-162).  device < 0 uses the current device.
+ * ------------------------------------------------------------------- */
+int cqx_ctx_create(const cqx_basis *b, double thresh_schwarz, int device,
+                   cqx_ctx **out);
+int cqx_ctx_destroy(cqx_ctx *ctx);
+const char *cqx_last_error(const cqx_ctx *ctx); /* NULL ctx: global error */
+
+/* Schwarz matrix (nshell x nshell, column-major, symmetric), copied to a
+ * host buffer. */
+int cqx_ctx_schwarz(cqx_ctx *ctx, double *Q);
+/* number of significant shell pairs and the pairs themselves (s2 <= s1,
+ * in the reference loop order); s1/s2 may be NULL to query the count. */
+int cqx_ctx_shell_pairs(cqx_ctx *ctx, int *npairs, int *s1, int *s2);
+
+/* One contraction: TwoBodyContraction<MatsT> (twopints.hpp:59-75) without
+ * the ERI4/intTrans fields that only the in-core path reads. */
+typedef struct {
+  const void *X; /* NB x NB, double or interleaved complex */
+  void *AX;      /* NB x NB, same type; accumulated into (callers zero it) */
+  int her;       /* X is Hermitian */
+  int type;      /* cqx_contraction_type */
+} cqx_contraction;
+
+/* GTODirectTPIContraction::twoBodyContract(comm, screen, list, pert):
+ * AX += J[X] or K[X] for every entry, with the reference's Schwarz x
+ * shell-block screening, 8-fold symmetry, digestion and Hermitian
+ * symmetrisation.  is_complex selects double vs complex<double>; mem says
+ * whether X/AX are host or device pointers.  Any number of contractions;
+ * they are processed in groups of at most 8 that share the ERIs. */
+int cqx_two_body_contract(cqx_ctx *ctx, int screen, int ncontract,
+                          const cqx_contraction *list, int is_complex,
+                          int mem);
+
+/* Contracted shell-quartet ERIs for nq quartets (s1,s2,s3,s4 arrays), in
+ * the libint2 layout the reference digests (row-major ijkl, pure
+ * functions where the shell is pure).  out receives the quartets back to
+ * back; offsets[q] (nq+1 entries, host) gives where each starts. */
+int cqx_eri_quartets(cqx_ctx *ctx, int nq, const int *s1, const int *s2,
+                     const int *s3, const int *s4, double *out,
+                     size_t *offsets);
+
+/* Statistics of the last cqx_two_body_contract call. */
+typedef struct {
+  long long n_candidates; /* quartets tested after the Schwarz pre-cut */
+  long long n_quartets;   /* quartets evaluated */
+  long long n_prim_quartets_max; /* sum over evaluated quartets of Ka Kb Kc Kd */
+  double flops_model;     /* algorithmic flop model summed over evaluated quartets */
+  double ms_total;        /* wall time of the call */
+  double ms_eri;          /* device time of the ERI/digestion kernels */
+  int n_launch_eri;       /* ERI kernel launches */
+} cqx_stats;
+int cqx_last_stats(const cqx_ctx *ctx, cqx_stats *st);
+
+/* Tuning knobs: "prim_screen" (primitive-pair drop threshold, default
+ * 1e-18; rebuilds the pair data), "fast_max_nc" (largest Cartesian quartet
+ * size routed to the thread-per-quartet kernels; 0 = generic kernel only). */
+int cqx_set_option(cqx_ctx *ctx, const char *name, double value);
+
+/* ---------------------------------------------------------------------
+ * Device memory for callers that keep densities and results resident in
+ * HBM (mem = CQX_DEVICE).  No counterpart in the reference, whose matrices
+ * live in host memory (CQMemManager); these let a caller stage X / AX on
+ * the GPU of a context without a second HIP runtime in the process.
+ * ------------------------------------------------------------------- */
+int cqx_device_alloc(cqx_ctx *ctx, size_t bytes, void **ptr);
+int cqx_device_free(cqx_ctx *ctx, void *ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device; synchronous */
+int cqx_memcpy(cqx_ctx *ctx, void *dst, const void *src, size_t bytes, int kind);
+/* zero device memory (fill kernel on the context stream) and wait */
+int cqx_device_zero(cqx_ctx *ctx, void *ptr, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CQX_H */

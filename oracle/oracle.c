@@ -1024,6 +1024,14 @@ int oracle_two_body_contract(oracle_basis *b, double thresh, int screen,
   return 0;
 }
 
+void oracle_prepare(oracle_basis *b) {
+  build_tables();
+  if (!b->schwarz) {
+    b->schwarz = malloc(sizeof(double) * (size_t)b->nshell * b->nshell);
+    oracle_schwarz(b, b->schwarz);
+  }
+}
+
 void oracle_last_stats(const oracle_basis *b, long long *nc, long long *nsk) {
   *nc = b->st_computed;
   *nsk = b->st_skipped;

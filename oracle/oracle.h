@@ -100,6 +100,10 @@ int oracle_two_body_contract(oracle_basis *b, double thresh_schwarz,
                              const oracle_contraction *c, int is_complex,
                              int nthreads);
 
+/* compute and cache the Schwarz bounds (otherwise done by the first
+ * contract call, as DirectTPI::computeSchwarz is in the reference) */
+void oracle_prepare(oracle_basis *b);
+
 /* quartet accounting of the last contract call on this basis */
 void oracle_last_stats(const oracle_basis *b, long long *n_computed,
                        long long *n_skipped);

@@ -51,6 +51,7 @@ def lib():
         L.oracle_two_body_contract.argtypes = [P, D, I, I, P, I, I]
         L.oracle_two_body_contract.restype = I
         L.oracle_last_stats.argtypes = [P, P, P]
+        L.oracle_prepare.argtypes = [P]
         _lib = L
     return _lib
 
@@ -150,6 +151,9 @@ class OracleBasis:
         if rc:
             raise RuntimeError(lib().oracle_last_error().decode())
         return AX
+
+    def prepare(self):
+        lib().oracle_prepare(self._h)
 
     def stats(self):
         a, b = C.c_longlong(), C.c_longlong()
