@@ -14,7 +14,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/cqx.h"
+#include "cqx.h"
 #include "basis.hpp"
 #include "device_types.hpp"
 #include "kernels.hpp"
@@ -86,7 +86,8 @@ struct cqx_ctx {
   cqx::Basis basis;
   double thresh = 1e-12;
   double prim_thresh = 1e-18;
-  int fast_max_nc = 108;  // classes up to this Cartesian size use the fast kernels
+  int fast_max_nc = 27;   // classes up to this Cartesian size use the lane-group kernels
+  int use_wave = 1;       // templated wave-per-quartet kernels for the other s/p/d classes
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
@@ -520,6 +521,10 @@ int cqx_set_option(cqx_ctx *c, const char *name, double value) {
     c->fast_max_nc = (int)value;
     return CQX_OK;
   }
+  if (!std::strcmp(name, "use_wave")) {
+    c->use_wave = value != 0.0;
+    return CQX_OK;
+  }
   return fail(c, CQX_ERR_ARG, std::string("unknown option ") + name);
 }
 
@@ -607,30 +612,42 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
                                      c->d_sh_nf.p, c->d_Bm.p, c->d_Bg.p, s));
       std::vector<double> Bg((size_t)NS * NS);
       HIPCHK(hipMemcpyAsync(Bg.data(), c->d_Bg.p, sizeof(double) * Bg.size(), hipMemcpyDeviceToHost, s));
-      // Cartesian density planes and accumulators
-      c->d_planes.ensure(nplane * nbc2);
-      c->d_acc.ensure(nplane * nbc2);
-      HIPCHK(cqx::launch_zero(c->d_acc.p, nplane * nbc2, s));
+      // Cartesian density planes and accumulators, interleaved with stride PS
+      int PS = 1;
+      while (PS < nplane) PS *= 2;
+      c->d_planes.ensure(PS * nbc2);
+      c->d_acc.ensure(PS * nbc2);
+      HIPCHK(cqx::launch_zero(c->d_acc.p, PS * nbc2, s));
       cqx::PlaneSet ps{};
       ps.nbc = NBC;
-      std::vector<int> first_plane(nmat);
-      for (int m = 0, p = 0; m < nmat; ++m) {
-        first_plane[m] = p;
+      ps.ps = PS;
+      // planes (one per real part), ordered by kind; slot = position, so the
+      // planes that one digestion term visits sit in consecutive slots
+      struct PlaneDesc { int m, part, kind; double sign; };
+      std::vector<PlaneDesc> pd;
+      for (int m = 0; m < nmat; ++m) {
         const bool two = is_complex && !(L[m].her && L[m].type == CQX_COULOMB);
-        double *re = c->d_planes.p + p * nbc2;
-        double *im = two ? c->d_planes.p + (p + 1) * nbc2 : nullptr;
-        HIPCHK(cqx::launch_to_cart(c->cmap, c->d_X.p + m * nb2 * ew, is_complex, re, im, s));
         int kind = L[m].type == CQX_COULOMB ? (L[m].her ? cqx::kJHer : cqx::kJNonHer)
                                             : (L[m].her ? cqx::kKHer : cqx::kKNonHer);
-        for (int k = 0; k < (two ? 2 : 1); ++k, ++p) {
-          cqx::Plane &P = ps.p[p];
-          P.X = c->d_planes.p + p * nbc2;
-          P.A = c->d_acc.p + p * nbc2;
-          P.kind = kind;
-          P.sign = (k == 1 && L[m].her) ? -1.0 : 1.0;  // conj of the imaginary part
-          P.mat = m;
-        }
+        for (int k = 0; k < (two ? 2 : 1); ++k)
+          pd.push_back({m, k, kind, (k == 1 && L[m].her) ? -1.0 : 1.0});  // conj of Im X
       }
+      std::stable_sort(pd.begin(), pd.end(), [](const PlaneDesc &x, const PlaneDesc &y) { return x.kind < y.kind; });
+      std::vector<int> slot_re(nmat, -1), slot_im(nmat, -1);
+      for (int k = 0; k < 4; ++k) { ps.kstart[k] = 0; ps.kcount[k] = 0; }
+      for (int p = 0; p < (int)pd.size(); ++p) {
+        cqx::Plane &P = ps.p[p];
+        P.X = c->d_planes.p + p;
+        P.A = c->d_acc.p + p;
+        P.kind = pd[p].kind;
+        P.sign = pd[p].sign;
+        P.mat = pd[p].m;
+        (pd[p].part ? slot_im : slot_re)[pd[p].m] = p;
+        if (ps.kcount[P.kind]++ == 0) ps.kstart[P.kind] = p;
+      }
+      for (int m = 0; m < nmat; ++m)
+        HIPCHK(cqx::launch_to_cart(c->cmap, c->d_X.p + m * nb2 * ew, is_complex, c->d_planes.p + slot_re[m],
+                                   slot_im[m] >= 0 ? c->d_planes.p + slot_im[m] : nullptr, PS, s));
       ps.n = nplane;
       int nJ = 0, nK = 0;
       for (int m = 0; m < nmat; ++m) (L[m].type == CQX_COULOMB ? nJ : nK)++;
@@ -743,8 +760,13 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         E.planes = ps;
         E.store_mode = 0;
         const int ncq = cqx::ncart(E.la) * cqx::ncart(E.lb) * cqx::ncart(E.lc) * cqx::ncart(E.ld);
-        const bool fast = ncq <= c->fast_max_nc && cqx::eri_fast_available(E.la, E.lb, E.lc, E.ld);
-        hipError_t e = fast ? cqx::launch_eri_fast(E, s) : cqx::launch_eri_team(E, s);
+        hipError_t e;
+        if (ncq <= c->fast_max_nc && cqx::eri_fast_available(E.la, E.lb, E.lc, E.ld))
+          e = cqx::launch_eri_fast(E, s);
+        else if (c->use_wave && cqx::eri_wave_available(E.la, E.lb, E.lc, E.ld))
+          e = cqx::launch_eri_wave(E, s);
+        else
+          e = cqx::launch_eri_team(E, s);
         HIPCHK(e);
         c->stats.n_quartets += (long long)counts[k];
         c->stats.n_prim_quartets_max += (long long)psum[k];
@@ -756,9 +778,10 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
       // back to pure functions, symmetrise, accumulate into AX
       for (int m = 0; m < nmat; ++m) {
         const bool two = is_complex && !(L[m].her && L[m].type == CQX_COULOMB);
-        const double *are = c->d_acc.p + first_plane[m] * nbc2;
-        const double *aim = two ? c->d_acc.p + (first_plane[m] + 1) * nbc2 : nullptr;
-        HIPCHK(cqx::launch_from_cart(c->cmap, are, aim, L[m].her, is_complex, c->d_AX.p + m * nb2 * ew, s));
+        const double *are = c->d_acc.p + slot_re[m];
+        const double *aim = two ? c->d_acc.p + slot_im[m] : nullptr;
+        HIPCHK(cqx::launch_from_cart(c->cmap, are, aim, PS, L[m].her, is_complex,
+                                     c->d_AX.p + m * nb2 * ew, s));
       }
       const hipMemcpyKind dk = mem == CQX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
       for (int m = 0; m < nmat; ++m)

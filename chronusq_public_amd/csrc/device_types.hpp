@@ -34,6 +34,8 @@ struct PrimPair {
 // Digestion target: one real plane of a (possibly complex) contraction.
 enum PlaneKind : int { kJHer = 0, kKHer = 1, kJNonHer = 2, kKNonHer = 3 };
 
+// Planes are interleaved: element (i, j) of plane p lives at
+// base[(i + j * NBc) * PlaneSet::ps + p]; X / A below already include + p.
 struct Plane {
   const double *X;  // Cartesian density plane, NBc x NBc column-major
   double *A;        // Cartesian accumulator plane
@@ -47,6 +49,8 @@ struct PlaneSet {
   Plane p[kMaxPlanes];
   int n;
   int nbc;  // Cartesian dimension NBc
+  int ps;   // interleave stride (number of plane slots)
+  int kstart[4], kcount[4];  // planes are ordered by kind: [kstart, kstart + kcount)
 };
 
 // A quartet: bra / ket pair indices within their class arrays plus the

@@ -1,0 +1,6 @@
+// Lane-group ERI kernels instantiated for G = 1 lanes per shell quartet.
+#include "eri_fast_impl.hpp"
+
+namespace cqx {
+template hipError_t launch_fast_g<1>(const EriLaunch &a, hipStream_t s);
+}  // namespace cqx

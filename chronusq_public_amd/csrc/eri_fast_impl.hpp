@@ -1,0 +1,307 @@
+// Lane-group Rys ERI + J/K digestion kernels for the low angular-momentum
+// classes (Cartesian quartet size <= 27), templated on the class and on the
+// number G of lanes that share one shell quartet.
+//
+// Every size is a compile-time constant, so the Rys nodes/weights, the 2D
+// recursions and the contracted Cartesian quartet stay in registers.  The
+// digestion follows the reference formulas (direct.hpp:1858-2002) in the
+// kernel's canonical labels a,b,c,d; each update is reduced in registers
+// over the summed indices before one atomic add per target element.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "rys.hpp"
+
+namespace cqx {
+
+namespace {
+
+constexpr double kTwoPi52 = 34.98683665524972;  // 2 pi^(5/2)
+
+constexpr int ncart_c(int l) { return (l + 1) * (l + 2) / 2; }
+
+// Cartesian exponents of component c of an l shell (lx descending, then ly)
+constexpr int cexp(int l, int c, int dir) {
+  int k = c, lx = l;
+  while (k >= l - lx + 1) {
+    k -= l - lx + 1;
+    --lx;
+  }
+  const int lz = k, ly = l - lx - lz;
+  return dir == 0 ? lx : (dir == 1 ? ly : lz);
+}
+
+__device__ __forceinline__ void atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
+
+template <int LA, int LB, int LC, int LD>
+struct Cls {
+  static constexpr int NCA = ncart_c(LA), NCB = ncart_c(LB), NCC = ncart_c(LC), NCD = ncart_c(LD);
+  static constexpr int NC = NCA * NCB * NCC * NCD;
+  static constexpr int NI = LA + LB + 1, NK = LC + LD + 1;
+  static constexpr int NR = (LA + LB + LC + LD) / 2 + 1;
+  static constexpr int NH = (LA + 1) * (LB + 1) * (LC + 1) * (LD + 1);
+  static constexpr int n(int role) { return role == 0 ? NCA : role == 1 ? NCB : role == 2 ? NCC : NCD; }
+  static constexpr int oidx(int ia, int ib, int ic, int id) { return ((ia * NCB + ib) * NCC + ic) * NCD + id; }
+  static constexpr int hidx(int a, int b, int c, int d) {
+    return ((a * (LB + 1) + b) * (LC + 1) + c) * (LD + 1) + d;
+  }
+};
+
+// 2D integrals of one root along one direction: VRR on (i, k), then HRR to
+// (ia, ib, ic, id); H has Cls::NH entries.
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void rys_2d_full(double C00, double D00, double B10, double B01,
+                                            double B00, double ABd, double CDd, double scale,
+                                            double (&H)[Cls<LA, LB, LC, LD>::NH]) {
+  using K = Cls<LA, LB, LC, LD>;
+  double G[K::NI][K::NK];
+  G[0][0] = scale;
+  if constexpr (K::NI > 1) G[1][0] = C00 * scale;
+#pragma unroll
+  for (int i = 1; i + 1 < K::NI; ++i) G[i + 1][0] = C00 * G[i][0] + i * B10 * G[i - 1][0];
+#pragma unroll
+  for (int k = 0; k + 1 < K::NK; ++k) {
+    if (k == 0) {
+      G[0][1] = D00 * G[0][0];
+#pragma unroll
+      for (int i = 1; i < K::NI; ++i) G[i][1] = D00 * G[i][0] + i * B00 * G[i - 1][0];
+    } else {
+      G[0][k + 1] = D00 * G[0][k] + k * B01 * G[0][k - 1];
+#pragma unroll
+      for (int i = 1; i < K::NI; ++i)
+        G[i][k + 1] = D00 * G[i][k] + k * B01 * G[i][k - 1] + i * B00 * G[i - 1][k];
+    }
+  }
+  // bra transfer: E[i][j][k], (i, j+1) = (i+1, j) + AB (i, j)
+  double E[K::NI][LB + 1][K::NK];
+#pragma unroll
+  for (int i = 0; i < K::NI; ++i)
+#pragma unroll
+    for (int k = 0; k < K::NK; ++k) E[i][0][k] = G[i][k];
+#pragma unroll
+  for (int j = 1; j <= LB; ++j)
+#pragma unroll
+    for (int i = 0; i + j < K::NI; ++i)
+#pragma unroll
+      for (int k = 0; k < K::NK; ++k) E[i][j][k] = E[i + 1][j - 1][k] + ABd * E[i][j - 1][k];
+  // ket transfer for each (ia, ib)
+#pragma unroll
+  for (int ia = 0; ia <= LA; ++ia)
+#pragma unroll
+    for (int ib = 0; ib <= LB; ++ib) {
+      double F[K::NK][LD + 1];
+#pragma unroll
+      for (int k = 0; k < K::NK; ++k) F[k][0] = E[ia][ib][k];
+#pragma unroll
+      for (int l = 1; l <= LD; ++l)
+#pragma unroll
+        for (int k = 0; k + l < K::NK; ++k) F[k][l] = F[k + 1][l - 1] + CDd * F[k][l - 1];
+#pragma unroll
+      for (int ic = 0; ic <= LC; ++ic)
+#pragma unroll
+        for (int id = 0; id <= LD; ++id) H[K::hidx(ia, ib, ic, id)] = F[ic][id];
+    }
+}
+
+// A(AR, AC) += coef * sum_{XR, XC} I * X(XR, XC), roles 0..3 = a, b, c, d.
+// Planes are interleaved with stride PS, so the lanes of a group (one plane
+// each) touch consecutive doubles: their loads and atomics coalesce.
+template <class K, int AR, int AC, int XR, int XC>
+__device__ __forceinline__ void digest_term(const double (&I)[K::NC], const int (&bf)[4],
+                                            const double *__restrict__ X, double *__restrict__ A,
+                                            int N, int PS, double coef) {
+#pragma unroll
+  for (int t0 = 0; t0 < K::n(AR); ++t0)
+#pragma unroll
+    for (int t1 = 0; t1 < K::n(AC); ++t1) {
+      double s = 0.0;
+#pragma unroll
+      for (int u0 = 0; u0 < K::n(XR); ++u0)
+#pragma unroll
+        for (int u1 = 0; u1 < K::n(XC); ++u1) {
+          int idx[4] = {0, 0, 0, 0};
+          idx[AR] = t0;
+          idx[AC] = t1;
+          idx[XR] = u0;
+          idx[XC] = u1;
+          const double x = X[((bf[XR] + u0) + (size_t)(bf[XC] + u1) * N) * PS];
+          s = fma(I[K::oidx(idx[0], idx[1], idx[2], idx[3])], x, s);
+        }
+      atomic_add(&A[((bf[AR] + t0) + (size_t)(bf[AC] + t1) * N) * PS], coef * s);
+    }
+}
+
+template <class K>
+__device__ __forceinline__ void digest_plane(const double (&I)[K::NC], const int (&bf)[4],
+                                             const Plane &P, int N, int PS) {
+  const double *X = P.X;
+  double *A = P.A;
+  switch (P.kind) {
+    case kJHer:
+      digest_term<K, 0, 1, 3, 2>(I, bf, X, A, N, PS, 1.0);
+      digest_term<K, 3, 2, 0, 1>(I, bf, X, A, N, PS, 1.0);
+      break;
+    case kKHer: {
+      const double h = 0.5 * P.sign;
+      digest_term<K, 0, 2, 3, 1>(I, bf, X, A, N, PS, h);
+      digest_term<K, 3, 1, 0, 2>(I, bf, X, A, N, PS, h);
+      digest_term<K, 3, 0, 1, 2>(I, bf, X, A, N, PS, h);
+      digest_term<K, 1, 2, 3, 0>(I, bf, X, A, N, PS, h);
+    } break;
+    case kJNonHer:
+      digest_term<K, 0, 1, 3, 2>(I, bf, X, A, N, PS, 0.5);
+      digest_term<K, 0, 1, 2, 3>(I, bf, X, A, N, PS, 0.5);
+      digest_term<K, 1, 0, 3, 2>(I, bf, X, A, N, PS, 0.5);
+      digest_term<K, 1, 0, 2, 3>(I, bf, X, A, N, PS, 0.5);
+      digest_term<K, 2, 3, 1, 0>(I, bf, X, A, N, PS, 0.5);
+      digest_term<K, 2, 3, 0, 1>(I, bf, X, A, N, PS, 0.5);
+      digest_term<K, 3, 2, 1, 0>(I, bf, X, A, N, PS, 0.5);
+      digest_term<K, 3, 2, 0, 1>(I, bf, X, A, N, PS, 0.5);
+      break;
+    case kKNonHer:
+      digest_term<K, 2, 0, 3, 1>(I, bf, X, A, N, PS, 0.5);
+      digest_term<K, 3, 1, 2, 0>(I, bf, X, A, N, PS, 0.5);
+      digest_term<K, 3, 0, 2, 1>(I, bf, X, A, N, PS, 0.5);
+      digest_term<K, 2, 1, 3, 0>(I, bf, X, A, N, PS, 0.5);
+      digest_term<K, 0, 2, 1, 3>(I, bf, X, A, N, PS, 0.5);
+      digest_term<K, 1, 3, 0, 2>(I, bf, X, A, N, PS, 0.5);
+      digest_term<K, 0, 3, 1, 2>(I, bf, X, A, N, PS, 0.5);
+      digest_term<K, 1, 2, 0, 3>(I, bf, X, A, N, PS, 0.5);
+      break;
+  }
+}
+
+// G lanes per quartet: the lanes split the primitive quartets of the shell
+// quartet, combine the contracted Cartesian integrals with a butterfly of
+// lane shuffles, and then lane g digests planes g, g + G, ...
+template <int LA, int LB, int LC, int LD, int G>
+__global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
+  using K = Cls<LA, LB, LC, LD>;
+  const int g = threadIdx.x & (G - 1);
+  const long long q = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const bool valid = q < a.nq;
+  Quartet qt{0u, 0u};
+  PairRec bp{}, kp{};
+  if (valid) {
+    qt = a.quartets[q];
+    bp = a.bra[qt.bra];
+    kp = a.ket[qt.ket_mask & 0xFFFFFFu];
+  }
+  double I[K::NC];
+#pragma unroll
+  for (int o = 0; o < K::NC; ++o) I[o] = 0.0;
+  const int nk = kp.npp, ntot = bp.npp * kp.npp;
+  int i = 0, j = g;
+  while (nk > 0 && j >= nk) {
+    j -= nk;
+    ++i;
+  }
+  for (int k = g; k < ntot; k += G) {
+    const PrimPair A = a.prims[bp.pp_off + i];
+    const PrimPair B = a.prims[kp.pp_off + j];
+    j += G;
+    while (j >= nk) {
+      j -= nk;
+      ++i;
+    }
+    const double pq = A.p + B.p, rpq = 1.0 / pq;
+    const double rho = A.p * B.p * rpq;
+    const double PQ[3] = {A.P[0] - B.P[0], A.P[1] - B.P[1], A.P[2] - B.P[2]};
+    const double T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+    const double pref = A.K * B.K * kTwoPi52 / (A.p * B.p * sqrt(pq));
+    double x[K::NR], w[K::NR];
+    rys_roots<K::NR>(T, a.rys, x, w);
+    if constexpr (K::NC == 1) {
+      double s = 0.0;
+#pragma unroll
+      for (int r = 0; r < K::NR; ++r) s += w[r];
+      I[0] = fma(pref, s, I[0]);
+    } else {
+      const double rp = 1.0 / A.p, rq = 1.0 / B.p;
+#pragma unroll
+      for (int r = 0; r < K::NR; ++r) {
+        const double t2 = x[r];
+        const double B00 = 0.5 * t2 * rpq;
+        const double B10 = 0.5 * rp * (1.0 - rho * rp * t2);
+        const double B01 = 0.5 * rq * (1.0 - rho * rq * t2);
+        double H[3][K::NH];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double C00 = A.PA[d] - B.p * rpq * t2 * PQ[d];
+          const double D00 = B.PA[d] + A.p * rpq * t2 * PQ[d];
+          rys_2d_full<LA, LB, LC, LD>(C00, D00, B10, B01, B00, bp.AB[d], kp.AB[d],
+                                      d == 2 ? pref * w[r] : 1.0, H[d]);
+        }
+#pragma unroll
+        for (int ia = 0; ia < K::NCA; ++ia)
+#pragma unroll
+          for (int ib = 0; ib < K::NCB; ++ib)
+#pragma unroll
+            for (int ic = 0; ic < K::NCC; ++ic)
+#pragma unroll
+              for (int id = 0; id < K::NCD; ++id) {
+                const double hx = H[0][K::hidx(cexp(LA, ia, 0), cexp(LB, ib, 0), cexp(LC, ic, 0), cexp(LD, id, 0))];
+                const double hy = H[1][K::hidx(cexp(LA, ia, 1), cexp(LB, ib, 1), cexp(LC, ic, 1), cexp(LD, id, 1))];
+                const double hz = H[2][K::hidx(cexp(LA, ia, 2), cexp(LB, ib, 2), cexp(LC, ic, 2), cexp(LD, id, 2))];
+                double &o = I[K::oidx(ia, ib, ic, id)];
+                o = fma(hx * hy, hz, o);
+              }
+      }
+    }
+  }
+  if constexpr (G > 1) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+      for (int o = 0; o < K::NC; ++o) I[o] += __shfl_xor(I[o], off);
+  }
+  if (!valid) return;
+  // degeneracy (direct.hpp:1664-1667, 1793-1802) and the 1/2 of :1849-1850
+  const bool same_pair = (bp.a == kp.a && bp.b == kp.b);
+  const double scale = 0.5 * (bp.a == bp.b ? 1.0 : 2.0) * (kp.a == kp.b ? 1.0 : 2.0) * (same_pair ? 1.0 : 2.0);
+#pragma unroll
+  for (int o = 0; o < K::NC; ++o) I[o] *= scale;
+  const int bf[4] = {bp.bfa, bp.bfb, kp.bfa, kp.bfb};
+  const unsigned mask = qt.ket_mask >> 24;
+  for (int p = g; p < a.planes.n; p += G) {
+    const Plane &P = a.planes.p[p];
+    if (!((mask >> P.mat) & 1u)) continue;
+    digest_plane<K>(I, bf, P, a.planes.nbc, a.planes.ps);
+  }
+}
+
+template <int LA, int LB, int LC, int LD, int G>
+hipError_t launch_cls(const EriLaunch &a, hipStream_t s) {
+  const long long threads = a.nq * G;
+  hipLaunchKernelGGL((eri_fast_kernel<LA, LB, LC, LD, G>), dim3((unsigned)((threads + 255) / 256)),
+                     dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// classes (bra class >= ket class in (la,lb) order ss, ps, pp, ds, dp, dd)
+// with at most 27 Cartesian components
+#define CQX_FAST_CLASSES(X) \
+  X(0, 0, 0, 0)             \
+  X(1, 0, 0, 0)             \
+  X(1, 0, 1, 0)             \
+  X(1, 1, 0, 0)             \
+  X(1, 1, 1, 0)             \
+  X(2, 0, 0, 0)             \
+  X(2, 0, 1, 0)             \
+  X(2, 1, 0, 0)
+
+// all classes of the list for one lane-group size G
+template <int G>
+hipError_t launch_fast_g(const EriLaunch &a, hipStream_t s) {
+#define CQX_CASE(A, B, C, D) \
+  if (a.la == A && a.lb == B && a.lc == C && a.ld == D) return launch_cls<A, B, C, D, G>(a, s);
+  CQX_FAST_CLASSES(CQX_CASE)
+#undef CQX_CASE
+  return hipErrorNotSupported;
+}
+
+}  // namespace cqx

@@ -1,0 +1,161 @@
+// One-wave-per-quartet Rys ERI + J/K digestion kernels for the larger
+// s/p/d classes (Cartesian quartet size > 27), templated on the class.
+//
+// Per primitive quartet every lane evaluates the Rys nodes/weights (the
+// same T across the wave, so the table reads broadcast), lanes (root, dir)
+// < 3 NR build the 2D integrals of their root and direction in registers
+// and store them to the wave's LDS slice, and every lane accumulates its
+// own share of the contracted Cartesian quartet.  After the primitive loop
+// the quartet goes to LDS and the lanes digest it (digest.hpp).
+#include <hip/hip_runtime.h>
+
+#include "digest.hpp"
+#include "eri_fast_impl.hpp"
+
+namespace cqx {
+
+namespace {
+
+constexpr int kWavesPerBlock = 4;
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch a) {
+  using K = Cls<LA, LB, LC, LD>;
+  constexpr int NOWN = (K::NC + 63) / 64;
+  constexpr int HSZ = 3 * K::NR * K::NH;
+  __shared__ double s_H[kWavesPerBlock][HSZ];
+  __shared__ double s_I[kWavesPerBlock][K::NC];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long q = (long long)blockIdx.x * kWavesPerBlock + wv;
+  if (q >= a.nq) return;  // uniform across the wave; no block barriers below
+  const Quartet qt = a.quartets[q];
+  const PairRec bp = a.bra[qt.bra];
+  const PairRec kp = a.ket[qt.ket_mask & 0xFFFFFFu];
+  double *H = s_H[wv];
+  double *sI = s_I[wv];
+
+  // the lane's outputs o = lane + 64 k and their 2D-table offsets
+  int ox[NOWN], oy[NOWN], oz[NOWN];
+  double acc[NOWN];
+#pragma unroll
+  for (int k = 0; k < NOWN; ++k) {
+    acc[k] = 0.0;
+    int o = lane + 64 * k;
+    if (o >= K::NC) o = 0;
+    int rem = o;
+    const int id = rem % K::NCD; rem /= K::NCD;
+    const int ic = rem % K::NCC; rem /= K::NCC;
+    const int ib = rem % K::NCB;
+    const int ia = rem / K::NCB;
+    ox[k] = K::hidx(cexp(LA, ia, 0), cexp(LB, ib, 0), cexp(LC, ic, 0), cexp(LD, id, 0));
+    oy[k] = K::NH + K::hidx(cexp(LA, ia, 1), cexp(LB, ib, 1), cexp(LC, ic, 1), cexp(LD, id, 1));
+    oz[k] = 2 * K::NH + K::hidx(cexp(LA, ia, 2), cexp(LB, ib, 2), cexp(LC, ic, 2), cexp(LD, id, 2));
+  }
+  const int my_r = lane / 3, my_d = lane % 3;
+  for (int i = 0; i < bp.npp; ++i) {
+    const PrimPair A = a.prims[bp.pp_off + i];
+    for (int j = 0; j < kp.npp; ++j) {
+      const PrimPair B = a.prims[kp.pp_off + j];
+      const double pq = A.p + B.p, rpq = 1.0 / pq;
+      const double rho = A.p * B.p * rpq;
+      const double PQ[3] = {A.P[0] - B.P[0], A.P[1] - B.P[1], A.P[2] - B.P[2]};
+      const double T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+      const double pref = A.K * B.K * kTwoPi52 / (A.p * B.p * sqrt(pq));
+      if (lane < 3 * K::NR) {
+        double x[K::NR], w[K::NR];
+        rys_roots<K::NR>(T, a.rys, x, w);
+        double t2 = x[0], wr = w[0];
+#pragma unroll
+        for (int r = 1; r < K::NR; ++r)
+          if (my_r == r) {
+            t2 = x[r];
+            wr = w[r];
+          }
+        const double rp = 1.0 / A.p, rq = 1.0 / B.p;
+        const double B00 = 0.5 * t2 * rpq;
+        const double B10 = 0.5 * rp * (1.0 - rho * rp * t2);
+        const double B01 = 0.5 * rq * (1.0 - rho * rq * t2);
+        const double PQd = my_d == 0 ? PQ[0] : (my_d == 1 ? PQ[1] : PQ[2]);
+        const double PAd = my_d == 0 ? A.PA[0] : (my_d == 1 ? A.PA[1] : A.PA[2]);
+        const double QCd = my_d == 0 ? B.PA[0] : (my_d == 1 ? B.PA[1] : B.PA[2]);
+        const double ABd = my_d == 0 ? bp.AB[0] : (my_d == 1 ? bp.AB[1] : bp.AB[2]);
+        const double CDd = my_d == 0 ? kp.AB[0] : (my_d == 1 ? kp.AB[1] : kp.AB[2]);
+        const double C00 = PAd - B.p * rpq * t2 * PQd;
+        const double D00 = QCd + A.p * rpq * t2 * PQd;
+        double h[K::NH];
+        rys_2d_full<LA, LB, LC, LD>(C00, D00, B10, B01, B00, ABd, CDd, my_d == 2 ? pref * wr : 1.0, h);
+        double *dst = H + (my_r * 3 + my_d) * K::NH;
+#pragma unroll
+        for (int e = 0; e < K::NH; ++e) dst[e] = h[e];
+      }
+      wave_sync();
+#pragma unroll
+      for (int k = 0; k < NOWN; ++k) {
+        if (lane + 64 * k >= K::NC) continue;
+        double s = 0.0;
+#pragma unroll
+        for (int r = 0; r < K::NR; ++r) {
+          const double *Hr = H + r * 3 * K::NH;
+          s = fma(Hr[ox[k]] * Hr[oy[k]], Hr[oz[k]], s);
+        }
+        acc[k] += s;
+      }
+      wave_sync();
+    }
+  }
+  // degeneracy (direct.hpp:1664-1667, 1793-1802) and the 1/2 of :1849-1850
+  const bool same_pair = (bp.a == kp.a && bp.b == kp.b);
+  const double scale = 0.5 * (bp.a == bp.b ? 1.0 : 2.0) * (kp.a == kp.b ? 1.0 : 2.0) * (same_pair ? 1.0 : 2.0);
+#pragma unroll
+  for (int k = 0; k < NOWN; ++k)
+    if (lane + 64 * k < K::NC) sI[lane + 64 * k] = acc[k] * scale;
+  wave_sync();
+  const int n[4] = {K::NCA, K::NCB, K::NCC, K::NCD};
+  const int bf[4] = {bp.bfa, bp.bfb, kp.bfa, kp.bfb};
+  digest_wave(sI, n, bf, a.planes, qt.ket_mask >> 24, lane, 64);
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_wave_cls(const EriLaunch &a, hipStream_t s) {
+  const long long blocks = (a.nq + kWavesPerBlock - 1) / kWavesPerBlock;
+  hipLaunchKernelGGL((eri_wave_kernel<LA, LB, LC, LD>), dim3((unsigned)blocks),
+                     dim3(64 * kWavesPerBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// s/p/d classes with more than 27 Cartesian components
+#define CQX_WAVE_CLASSES(X) \
+  X(1, 1, 1, 1)             \
+  X(2, 0, 1, 1)             \
+  X(2, 0, 2, 0)             \
+  X(2, 1, 1, 1)             \
+  X(2, 1, 2, 0)             \
+  X(2, 1, 2, 1)             \
+  X(2, 2, 0, 0)             \
+  X(2, 2, 1, 0)             \
+  X(2, 2, 1, 1)             \
+  X(2, 2, 2, 0)             \
+  X(2, 2, 2, 1)             \
+  X(2, 2, 2, 2)
+
+bool eri_wave_available(int la, int lb, int lc, int ld) {
+#define CQX_CASE(A, B, C, D) \
+  if (la == A && lb == B && lc == C && ld == D) return true;
+  CQX_WAVE_CLASSES(CQX_CASE)
+#undef CQX_CASE
+  return false;
+}
+
+hipError_t launch_eri_wave(const EriLaunch &a, hipStream_t s) {
+  if (a.nq <= 0) return hipSuccess;
+  if (a.store_mode) return hipErrorNotSupported;
+#define CQX_CASE(A, B, C, D) \
+  if (a.la == A && a.lb == B && a.lc == C && a.ld == D) return launch_wave_cls<A, B, C, D>(a, s);
+  CQX_WAVE_CLASSES(CQX_CASE)
+#undef CQX_CASE
+  return hipErrorNotSupported;
+}
+
+}  // namespace cqx

@@ -11,6 +11,7 @@
 // integral batch to pure functions.
 #include <hip/hip_runtime.h>
 
+#include "digest.hpp"
 #include "kernels.hpp"
 #include "rys.hpp"
 
@@ -98,72 +99,6 @@ __device__ void rys_2d(const TeamDims &D, double t2, double p, double q, double 
     }
 }
 
-// One digestion term over LDS quartet values sI (canonical labels a,b,c,d =
-// roles 0..3): A(ar, ac) += coef * sum_{xr, xc} I * X(xr, xc), where the A
-// roles and the X roles together are {a, b, c, d}.
-__device__ void digest_term(const double *__restrict__ sI, const int n[4], const int bf[4],
-                            int ar, int ac, int xr, int xc, double coef,
-                            const double *__restrict__ X, double *__restrict__ A, int N,
-                            int lane) {
-  const int nt = n[ar] * n[ac];
-  const int ns = n[xr] * n[xc];
-  for (int t = lane; t < nt; t += 64) {
-    int idx[4];
-    idx[ar] = t / n[ac];
-    idx[ac] = t % n[ac];
-    double sum = 0.0;
-    for (int u = 0; u < ns; ++u) {
-      idx[xr] = u / n[xc];
-      idx[xc] = u % n[xc];
-      const double I = sI[((idx[0] * n[1] + idx[1]) * n[2] + idx[2]) * n[3] + idx[3]];
-      sum += I * X[(bf[xr] + idx[xr]) + (size_t)(bf[xc] + idx[xc]) * N];
-    }
-    atomic_add(&A[(bf[ar] + idx[ar]) + (size_t)(bf[ac] + idx[ac]) * N], coef * sum);
-  }
-}
-
-// The reference digestion formulas (direct.hpp:1858-2002) in canonical
-// labels 0..3 = shells 1..4, for one real plane.
-__device__ void digest_plane(const double *sI, const int n[4], const int bf[4],
-                             const Plane &P, int N, int lane) {
-  const double *X = P.X;
-  double *A = P.A;
-  enum { a = 0, b = 1, c = 2, d = 3 };
-  switch (P.kind) {
-    case kJHer:  // AX(1,2) += X(4,3) I ; AX(4,3) += X(1,2) I
-      digest_term(sI, n, bf, a, b, d, c, 1.0, X, A, N, lane);
-      digest_term(sI, n, bf, d, c, a, b, 1.0, X, A, N, lane);
-      break;
-    case kKHer: {  // four updates with 1/2 conj(X)
-      const double h = 0.5 * P.sign;
-      digest_term(sI, n, bf, a, c, d, b, h, X, A, N, lane);
-      digest_term(sI, n, bf, d, b, a, c, h, X, A, N, lane);
-      digest_term(sI, n, bf, d, a, b, c, h, X, A, N, lane);
-      digest_term(sI, n, bf, b, c, d, a, h, X, A, N, lane);
-    } break;
-    case kJNonHer:  // (1,2),(2,1) += 1/2 (X43 + X34) I ; (3,4),(4,3) += 1/2 (X21 + X12) I
-      digest_term(sI, n, bf, a, b, d, c, 0.5, X, A, N, lane);
-      digest_term(sI, n, bf, a, b, c, d, 0.5, X, A, N, lane);
-      digest_term(sI, n, bf, b, a, d, c, 0.5, X, A, N, lane);
-      digest_term(sI, n, bf, b, a, c, d, 0.5, X, A, N, lane);
-      digest_term(sI, n, bf, c, d, b, a, 0.5, X, A, N, lane);
-      digest_term(sI, n, bf, c, d, a, b, 0.5, X, A, N, lane);
-      digest_term(sI, n, bf, d, c, b, a, 0.5, X, A, N, lane);
-      digest_term(sI, n, bf, d, c, a, b, 0.5, X, A, N, lane);
-      break;
-    case kKNonHer:  // eight updates with 1/2 X
-      digest_term(sI, n, bf, c, a, d, b, 0.5, X, A, N, lane);
-      digest_term(sI, n, bf, d, b, c, a, 0.5, X, A, N, lane);
-      digest_term(sI, n, bf, d, a, c, b, 0.5, X, A, N, lane);
-      digest_term(sI, n, bf, c, b, d, a, 0.5, X, A, N, lane);
-      digest_term(sI, n, bf, a, c, b, d, 0.5, X, A, N, lane);
-      digest_term(sI, n, bf, b, d, a, c, 0.5, X, A, N, lane);
-      digest_term(sI, n, bf, a, d, b, c, 0.5, X, A, N, lane);
-      digest_term(sI, n, bf, b, c, a, d, 0.5, X, A, N, lane);
-      break;
-  }
-}
-
 __global__ __launch_bounds__(64) void eri_team_kernel(EriLaunch a) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x;
@@ -236,12 +171,7 @@ __global__ __launch_bounds__(64) void eri_team_kernel(EriLaunch a) {
       __syncthreads();
       const int n[4] = {D.nca, D.ncb, D.ncc, D.ncd};
       const int bf[4] = {bp.bfa, bp.bfb, kp.bfa, kp.bfb};
-      const unsigned mask = qt.ket_mask >> 24;
-      for (int p = 0; p < a.planes.n; ++p) {
-        const Plane &P = a.planes.p[p];
-        if (!((mask >> P.mat) & 1u)) continue;
-        digest_plane(s_I, n, bf, P, a.planes.nbc, lane);
-      }
+      digest_wave(s_I, n, bf, a.planes, qt.ket_mask >> 24, lane, 64);
     }
     __syncthreads();
   }
@@ -394,7 +324,8 @@ __device__ __forceinline__ int nfunc(const CartMap &M, int s) {
   return M.sh_pure[s] ? 2 * M.sh_l[s] + 1 : ncart_d(M.sh_l[s]);
 }
 
-__global__ void to_cart_kernel(CartMap M, const double *X, int cplx, double *Pre, double *Pim) {
+__global__ void to_cart_kernel(CartMap M, const double *X, int cplx, double *Pre, double *Pim,
+                               int ps) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)M.nbc * M.nbc) return;
   const int i = (int)(t % M.nbc), j = (int)(t / M.nbc);
@@ -416,12 +347,12 @@ __global__ void to_cart_kernel(CartMap M, const double *X, int cplx, double *Pre
       }
     }
   }
-  Pre[t] = re;
-  if (Pim) Pim[t] = im;
+  Pre[t * ps] = re;
+  if (Pim) Pim[t * ps] = im;
 }
 
-__global__ void from_cart_kernel(CartMap M, const double *Are, const double *Aim, int her,
-                                 int cplx, double *AX) {
+__global__ void from_cart_kernel(CartMap M, const double *Are, const double *Aim, int ps,
+                                 int her, int cplx, double *AX) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)M.nb * M.nb) return;
   const int mu = (int)(t % M.nb), nu = (int)(t / M.nb);
@@ -437,11 +368,11 @@ __global__ void from_cart_kernel(CartMap M, const double *Are, const double *Aim
       if (tj == 0.0) continue;
       const size_t ij = (M.sh_bfc[sm] + i) + (size_t)(M.sh_bfc[sn] + j) * M.nbc;
       const size_t ji = (M.sh_bfc[sn] + j) + (size_t)(M.sh_bfc[sm] + i) * M.nbc;
-      a_re += ti * tj * Are[ij];
-      b_re += ti * tj * Are[ji];
+      a_re += ti * tj * Are[ij * ps];
+      b_re += ti * tj * Are[ji * ps];
       if (Aim) {
-        a_im += ti * tj * Aim[ij];
-        b_im += ti * tj * Aim[ji];
+        a_im += ti * tj * Aim[ij * ps];
+        b_im += ti * tj * Aim[ji * ps];
       }
     }
   }
@@ -512,16 +443,16 @@ hipError_t launch_block_norms(const double *X, int nmat, int is_complex, const i
 }
 
 hipError_t launch_to_cart(const CartMap &m, const double *X, int is_complex, double *pre,
-                          double *pim, hipStream_t s) {
+                          double *pim, int ps, hipStream_t s) {
   hipLaunchKernelGGL(to_cart_kernel, dim3(nblocks((long long)m.nbc * m.nbc, 256)), dim3(256), 0,
-                     s, m, X, is_complex, pre, pim);
+                     s, m, X, is_complex, pre, pim, ps);
   return hipGetLastError();
 }
 
-hipError_t launch_from_cart(const CartMap &m, const double *are, const double *aim, int her,
+hipError_t launch_from_cart(const CartMap &m, const double *are, const double *aim, int ps, int her,
                             int is_complex, double *AX, hipStream_t s) {
   hipLaunchKernelGGL(from_cart_kernel, dim3(nblocks((long long)m.nb * m.nb, 256)), dim3(256), 0,
-                     s, m, are, aim, her, is_complex, AX);
+                     s, m, are, aim, ps, her, is_complex, AX);
   return hipGetLastError();
 }
 

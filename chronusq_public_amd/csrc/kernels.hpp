@@ -28,6 +28,9 @@ hipError_t launch_eri_team(const EriLaunch &a, hipStream_t s);
 // hipErrorNotSupported when the class has no specialised kernel.
 hipError_t launch_eri_fast(const EriLaunch &a, hipStream_t s);
 bool eri_fast_available(int la, int lb, int lc, int ld);
+// One-wave-per-quartet templated kernels for the larger s/p/d classes.
+hipError_t launch_eri_wave(const EriLaunch &a, hipStream_t s);
+bool eri_wave_available(int la, int lb, int lc, int ld);
 
 struct ListLaunch {
   const PairRec *bra, *ket;
@@ -67,14 +70,14 @@ struct CartMap {
   const int *bf_comp;     // pure-or-Cartesian function -> index within shell
   int nb, nbc, ns;
 };
-// X (NB x NB, real or interleaved complex) -> real planes of T X T^T
+// X (NB x NB, real or interleaved complex) -> real planes of T X T^T, written
+// with interleave stride ps (plane pointers already offset to their slot)
 hipError_t launch_to_cart(const CartMap &m, const double *X, int is_complex,
-                          double *plane_re, double *plane_im, hipStream_t s);
-// AX += finalize(T^T A T): mode 0 Hermitian (1/2 (A + A^H)), 1 non-Hermitian
-// (1/2 A); A_im may be null (real accumulator only).
-hipError_t launch_from_cart(const CartMap &m, const double *A_re,
-                            const double *A_im, int her, int is_complex,
-                            double *AX, hipStream_t s);
+                          double *plane_re, double *plane_im, int ps, hipStream_t s);
+// AX += finalize(T^T A T): her = 1 gives 1/2 (A + A^H), her = 0 gives 1/2 A;
+// A_im may be null (real accumulator only); planes interleaved with stride ps.
+hipError_t launch_from_cart(const CartMap &m, const double *A_re, const double *A_im, int ps,
+                            int her, int is_complex, double *AX, hipStream_t s);
 
 hipError_t launch_zero(double *p, size_t n, hipStream_t s);
 

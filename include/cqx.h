@@ -141,7 +141,9 @@ int cqx_last_stats(const cqx_ctx *ctx, cqx_stats *st);
 
 /* Tuning knobs: "prim_screen" (primitive-pair drop threshold, default
  * 1e-18; rebuilds the pair data), "fast_max_nc" (largest Cartesian quartet
- * size routed to the thread-per-quartet kernels; 0 = generic kernel only). */
+ * size routed to the lane-group kernels, default 27), "use_wave" (0 routes
+ * the larger s/p/d classes to the generic kernel instead of the templated
+ * wave-per-quartet ones). */
 int cqx_set_option(cqx_ctx *ctx, const char *name, double value);
 
 /* ---------------------------------------------------------------------

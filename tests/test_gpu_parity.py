@@ -177,9 +177,9 @@ def test_device_pointer_path():
 
 
 def test_fast_and_generic_kernels_agree():
-    """The thread-per-quartet kernels and the generic wave-per-quartet
-    kernel give the same J/K (independent implementations of the same
-    quadrature): 1e-12 max abs on (H2O)_2 / 6-31G(d), 2C complex."""
+    """The lane-group, templated wave and generic kernels give the same J/K
+    (three implementations of the same quadrature and digestion): 1e-12 max
+    abs on (H2O)_2 / 6-31G(d), complex, Hermitian and non-Hermitian."""
     mol, gbs = _cluster(2, 4.0, 5, "6-31G(d)")
     say("case: fast vs generic")
     _, _, b, tpi, gto = make(mol, gbs)
@@ -188,8 +188,11 @@ def test_fast_and_generic_kernels_agree():
     C_ = rng.normal(size=(nb, 5)) + 1j * rng.normal(size=(nb, 5))
     D = C_ @ C_.conj().T
     Xs, types, hers = [D, D, D], [COULOMB, EXCHANGE, EXCHANGE], [True, True, False]
-    fast = contract(gto, Xs, types, hers)
+    default = contract(gto, Xs, types, hers)      # lane-group + wave kernels
     tpi.set_option("fast_max_nc", 0)
-    gen = contract(gto, Xs, types, hers)
-    for f, g in zip(fast, gen):
+    wave = contract(gto, Xs, types, hers)         # wave kernels for every s/p/d class
+    tpi.set_option("use_wave", 0)
+    gen = contract(gto, Xs, types, hers)          # generic kernel only
+    for f, w, g in zip(default, wave, gen):
         assert np.abs(f - g).max() < 1e-12
+        assert np.abs(w - g).max() < 1e-12
