@@ -86,6 +86,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="ghf16")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (HBM "
+                         "counter calibration for rocprofv3 --pmc runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         raise SystemExit("bench.py measures one GPU: the direct J/K engine runs one process "
@@ -107,6 +110,15 @@ def main():
             a.zero()
         gto.twoBodyContract(cont)
 
+    if args.copy_mb:
+        import ctypes as C
+        nbytes = args.copy_mb << 20
+        src, dst = C.c_void_p(), C.c_void_p()
+        cqx._check(cqx.lib().cqx_device_alloc(tpi._h, nbytes, C.byref(src)), tpi._h)
+        cqx._check(cqx.lib().cqx_device_alloc(tpi._h, nbytes, C.byref(dst)), tpi._h)
+        cqx._check(cqx.lib().cqx_memcpy(tpi._h, dst, src, nbytes, 2), tpi._h)
+        cqx.lib().cqx_device_free(tpi._h, src)
+        cqx.lib().cqx_device_free(tpi._h, dst)
     for _ in range(args.warmup):
         step()
     eri_ms, flops = [], []

@@ -63,5 +63,18 @@ def build(verbose=False, jobs=None):
     return LIB
 
 
+def build_cpp_driver():
+    """The C++ adapter example/test driver (tests/cpp/formgd), linked
+    against the in-tree libcqx.so."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "formgd.cpp")
+    out = os.path.join(root, "tests", "cpp", "formgd")
+    cmd = ["g++", "-std=c++17", "-O2", "-I", INCLUDE, src, "-o", out, "-L", HERE, "-lcqx",
+           "-Wl,-rpath,$ORIGIN/../../chronusq_public_amd"]
+    subprocess.run(cmd, check=True)
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_cpp_driver()

@@ -40,6 +40,8 @@ __device__ __constant__ const DigestTerms kDigestTerms = {
 __device__ inline void digest_wave(const double *__restrict__ sI, const int n[4], const int bf[4],
                                    const PlaneSet &ps, unsigned mask, int lane, int W) {
   const int N = ps.nbc, S = ps.ps;
+  // stride of each role in the quartet layout I[((a n_b + b) n_c + c) n_d + d]
+  const int st[4] = {n[1] * n[2] * n[3], n[2] * n[3], n[3], 1};
   for (int kind = 0; kind < 4; ++kind) {
     const int p0 = ps.kstart[kind], np = ps.kcount[kind];
     if (np == 0) continue;
@@ -47,23 +49,30 @@ __device__ inline void digest_wave(const double *__restrict__ sI, const int n[4]
     for (int term = 0; term < kDigestTerms.n[kind]; ++term) {
       const int ar = kDigestTerms.t[kind][term][0], ac = kDigestTerms.t[kind][term][1];
       const int xr = kDigestTerms.t[kind][term][2], xc = kDigestTerms.t[kind][term][3];
-      const int nt = n[ar] * n[ac], nsum = n[xr] * n[xc];
+      const int n_ac = n[ac], n_xr = n[xr], n_xc = n[xc];
+      const int s_ar = st[ar], s_ac = st[ac], s_xr = st[xr], s_xc = st[xc];
+      const int bf_ar = bf[ar], bf_ac = bf[ac], bf_xr = bf[xr], bf_xc = bf[xc];
+      const int nt = n[ar] * n_ac;
       for (int t = lane; t < nt * np; t += W) {
         const int pl = p0 + t % np, tgt = t / np;
         const Plane &P = ps.p[pl];
         if (!((mask >> P.mat) & 1u)) continue;
-        int idx[4];
-        idx[ar] = tgt / n[ac];
-        idx[ac] = tgt % n[ac];
-        double sum = 0.0;
-        for (int u = 0; u < nsum; ++u) {
-          idx[xr] = u / n[xc];
-          idx[xc] = u % n[xc];
-          const double I = sI[((idx[0] * n[1] + idx[1]) * n[2] + idx[2]) * n[3] + idx[3]];
-          sum = fma(I, P.X[((bf[xr] + idx[xr]) + (size_t)(bf[xc] + idx[xc]) * N) * S], sum);
+        const int t0 = tgt / n_ac, t1 = tgt - (tgt / n_ac) * n_ac;
+        const double *__restrict__ Ib = sI + t0 * s_ar + t1 * s_ac;
+        const double *__restrict__ Xb = P.X + ((size_t)bf_xr + (size_t)bf_xc * N) * S;
+        double sum0 = 0.0, sum1 = 0.0;
+        for (int u0 = 0; u0 < n_xr; ++u0) {
+          const double *Iu = Ib + u0 * s_xr;
+          const double *Xu = Xb + (size_t)u0 * S;
+          int u1 = 0;
+          for (; u1 + 1 < n_xc; u1 += 2) {
+            sum0 = fma(Iu[u1 * s_xc], Xu[(size_t)u1 * N * S], sum0);
+            sum1 = fma(Iu[(u1 + 1) * s_xc], Xu[(size_t)(u1 + 1) * N * S], sum1);
+          }
+          if (u1 < n_xc) sum0 = fma(Iu[u1 * s_xc], Xu[(size_t)u1 * N * S], sum0);
         }
         const double coef = kind == kKHer ? base_coef * P.sign : base_coef;
-        unsafeAtomicAdd(&P.A[((bf[ar] + idx[ar]) + (size_t)(bf[ac] + idx[ac]) * N) * S], coef * sum);
+        unsafeAtomicAdd(&P.A[((bf_ar + t0) + (size_t)(bf_ac + t1) * N) * S], coef * (sum0 + sum1));
       }
     }
   }

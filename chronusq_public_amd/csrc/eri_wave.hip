@@ -130,6 +130,7 @@ hipError_t launch_wave_cls(const EriLaunch &a, hipStream_t s) {
   X(1, 1, 1, 1)             \
   X(2, 0, 1, 1)             \
   X(2, 0, 2, 0)             \
+  X(2, 1, 1, 0)             \
   X(2, 1, 1, 1)             \
   X(2, 1, 2, 0)             \
   X(2, 1, 2, 1)             \
