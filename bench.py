@@ -15,6 +15,13 @@ FP64 flops (the Rys flop model over the quartets evaluated) divided by
 their device time from HIP events on the launch stream.  CPU baseline: the
 FP64 oracle (a C/OpenMP restatement of the reference scaffold) timed on one
 full build of the same workload on the host cores.
+
+The "mixed" object of the line is the mixed-precision comparison on the
+linear alkane C40H82 (all-trans, C-C 1.53 A, C-H 1.09 A, seed 0 +-0.05 A
+jitter), 6-31G* (724 functions), RHF density D = C C^T with C 724 x 161,
+N(0, 1/sqrt(724)): builds/s of the FP64 build and of the mixed build
+(quartets with Schwarz product < 1e-4 in FP32), max|dF| (F - H = G = 2J - K)
+and |dE| = 1/4 |Tr(D dG)| of the mixed result against the FP64 one.
 """
 import argparse
 import json
@@ -29,6 +36,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Fock builds/sec, mixed vs FP64 at |ΔE|≤1µEh; SCF p50/p99 latency"
 FP64_VECTOR_PEAK = 78.6  # TFLOP/s, MI355X
+FP32_VECTOR_PEAK = 157.3  # TFLOP/s, MI355X
 
 
 def pauli_density(nb, nocc, seed):
@@ -42,7 +50,9 @@ def pauli_density(nb, nocc, seed):
 
 
 def workload(name):
-    from chronusq_public_amd.chem import basis_text, water_cluster
+    from chronusq_public_amd.chem import alkane_chain, basis_text, water_cluster
+    if name == "alkane40":
+        return alkane_chain(40, 0), basis_text("6-31G*"), "rhf"
     if name == "ghf16":
         mol = water_cluster(16, 12.0, seed=0)
         return mol, basis_text("cc-pVDZ"), "2c"
@@ -52,14 +62,14 @@ def workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def densities(kind, nb):
+def densities(kind, nb, nocc=20):
     if kind == "2c":
         S, Z, Y, X = pauli_density(nb, 160, 0)
         mats = [S, S, Z, Y, X]
         types = [0, 1, 1, 1, 1]
         return mats, types, True
     rng = np.random.Generator(np.random.PCG64(0))
-    C = rng.normal(0, 1 / np.sqrt(nb), size=(nb, 20))
+    C = rng.normal(0, 1 / np.sqrt(nb), size=(nb, nocc))
     D = C @ C.T
     D = 0.5 * (D + D.T)
     return [D, D], [0, 1], False
@@ -79,6 +89,81 @@ def cpu_baseline(gbs, mol, mats, types, cplx, gpu_out):
             "seconds": dt}, err
 
 
+class Build:
+    """One workload's engine, resident densities and outputs; step() = one
+    formGD-equivalent contraction list with the outputs zeroed first."""
+
+    def __init__(self, name, nocc=20):
+        from chronusq_public_amd import cqx
+        self.mol, self.gbs, kind = workload(name)
+        self.basis = cqx.BasisSet(self.gbs, self.mol)
+        self.tpi = cqx.DirectTPI(self.basis, 1e-12, device=0)
+        self.gto = cqx.GTODirectTPIContraction(self.tpi)
+        nb = self.basis.nBasis
+        self.mats, self.types, self.cplx = densities(kind, nb, nocc)
+        self.dX = [cqx.DeviceMatrix(self.tpi, nb, self.cplx).upload(m) for m in self.mats]
+        self.dA = [cqx.DeviceMatrix(self.tpi, nb, self.cplx) for _ in self.mats]
+        self.cont = [cqx.TwoBodyContraction(x, a, True, t)
+                     for x, a, t in zip(self.dX, self.dA, self.types)]
+
+    def step(self):
+        for a in self.dA:
+            a.zero()
+        self.gto.twoBodyContract(self.cont)
+
+    def timed(self, steps, warmup):
+        for _ in range(warmup):
+            self.step()
+        eri_ms, flops, f32 = [], [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            self.step()
+            st = self.tpi.last_stats()
+            eri_ms.append(st["ms_eri"])
+            flops.append(st["flops_model"])
+            f32.append(st["flops_model_f32"])
+        dt = time.perf_counter() - t0
+        return dt, float(np.mean(eri_ms)), float(np.mean(flops)), float(np.mean(f32))
+
+    def outputs(self):
+        return [a.download() for a in self.dA]
+
+
+def mixed_comparison(steps, warmup, cut=1e-4):
+    """FP64 and mixed builds of the alkane workload, timed alike; the mixed
+    result is compared with the FP64 one (the FP64 path itself is held to
+    1e-10 of the oracle by the GPU parity tests)."""
+    b = Build("alkane40", nocc=161)
+    dt64, eri64, fl64, _ = b.timed(steps, warmup)
+    J, K = b.outputs()
+    st64 = b.tpi.last_stats()
+    b.tpi.set_option("fp32_cut", cut)
+    dtm, erim, flm, flm32 = b.timed(steps, warmup)
+    Jm, Km = b.outputs()
+    stm = b.tpi.last_stats()
+    dG = (2 * Jm - Km) - (2 * J - K)
+    D = b.mats[0]
+    # roofline of the mixed build: FP64 part at the FP64 peak, FP32 part at the FP32 peak
+    t_roof = (flm - flm32) / (FP64_VECTOR_PEAK * 1e12) + flm32 / (FP32_VECTOR_PEAK * 1e12)
+    return {
+        "workload": ("RHF J/K, C40H82 all-trans seed 0, 6-31G* (NB %d), D = C C^T with C NB x 161, "
+                     "Schwarz 1e-12, FP32 when Q_ab Q_cd < %g" % (b.basis.nBasis, cut)),
+        "steps": steps,
+        "fp64_builds_per_s": steps / dt64,
+        "mixed_builds_per_s": steps / dtm,
+        "speedup": dt64 / dtm,
+        "fp64_eri_ms": eri64,
+        "mixed_eri_ms": erim,
+        "quartets": stm["n_quartets"],
+        "quartets_fp32": stm["n_quartets_f32"],
+        "max_abs_dF": float(np.abs(dG).max()),
+        "abs_dE": float(abs(0.25 * np.sum(D * dG))),
+        "fp64_roofline_frac": fl64 / (eri64 * 1e-3) / (FP64_VECTOR_PEAK * 1e12),
+        "mixed_roofline_frac": t_roof / (erim * 1e-3),
+        "fp64_quartets_check": st64["n_quartets"] == stm["n_quartets"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +171,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="ghf16")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mixed-steps", type=int, default=3,
+                    help="timed builds per precision of the mixed-precision comparison (0 skips it)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM "
                          "counter calibration for rocprofv3 --pmc runs)")
@@ -95,20 +182,10 @@ def main():
                          "on one MI355X (multi-GPU sharding is not part of this build)")
 
     from chronusq_public_amd import cqx
-    mol, gbs, kind = workload(args.workload)
-    basis = cqx.BasisSet(gbs, mol)
-    tpi = cqx.DirectTPI(basis, 1e-12, device=0)
-    gto = cqx.GTODirectTPIContraction(tpi)
-    nb = basis.nBasis
-    mats, types, cplx = densities(kind, nb)
-    dX = [cqx.DeviceMatrix(tpi, nb, cplx).upload(m) for m in mats]
-    dA = [cqx.DeviceMatrix(tpi, nb, cplx) for _ in mats]
-    cont = [cqx.TwoBodyContraction(x, a, True, t) for x, a, t in zip(dX, dA, types)]
-
-    def step():
-        for a in dA:
-            a.zero()
-        gto.twoBodyContract(cont)
+    bench = Build(args.workload)
+    mol, gbs, tpi = bench.mol, bench.gbs, bench.tpi
+    mats, types, cplx = bench.mats, bench.types, bench.cplx
+    nb = bench.basis.nBasis
 
     if args.copy_mb:
         import ctypes as C
@@ -119,21 +196,11 @@ def main():
         cqx._check(cqx.lib().cqx_memcpy(tpi._h, dst, src, nbytes, 2), tpi._h)
         cqx.lib().cqx_device_free(tpi._h, src)
         cqx.lib().cqx_device_free(tpi._h, dst)
-    for _ in range(args.warmup):
-        step()
-    eri_ms, flops = [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-        st = tpi.last_stats()
-        eri_ms.append(st["ms_eri"])
-        flops.append(st["flops_model"])
-    dt = time.perf_counter() - t0
+    dt, eri_avg, flops_avg, _ = bench.timed(args.steps, args.warmup)
     st = tpi.last_stats()
     ms_step = 1e3 * dt / args.steps
-    eri_avg = float(np.mean(eri_ms))
-    achieved = float(np.mean(flops)) / (eri_avg * 1e-3) / 1e12
-    out = [a.download() for a in dA]
+    achieved = flops_avg / (eri_avg * 1e-3) / 1e12
+    out = bench.outputs()
 
     line = {
         "metric": METRIC,
@@ -167,10 +234,12 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / FP64_VECTOR_PEAK,
             "traffic": None,
-            "flops_per_build": float(np.mean(flops)),
+            "flops_per_build": flops_avg,
             "eri_ms_per_build": eri_avg,
         },
     }
+    if args.mixed_steps > 0:
+        line["mixed"] = mixed_comparison(args.mixed_steps, 1)
     if not args.no_cpu:
         cb, err = cpu_baseline(gbs, mol, mats, types, cplx, out)
         line["cpu_baseline"] = cb

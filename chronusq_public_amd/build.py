@@ -1,7 +1,8 @@
 """Build libcqx.so (HIP for gfx950) in-tree.
 
-Sources: csrc/*.cpp (host orchestration and basis handling) and csrc/*.hip
-(device kernels).  The Rys table (csrc/rys_table.bin, written by
+Sources: csrc/*.cpp (host orchestration, basis handling, one-electron
+integrals, the SCF driver) and csrc/*.hip (device kernels); linked against
+rocBLAS and rocSOLVER from the ROCm install.  The Rys table (csrc/rys_table.bin, written by
 csrc/gen_rys_table.py) is embedded as a C array.
 """
 import os
@@ -57,7 +58,10 @@ def build(verbose=False, jobs=None):
     jobs = jobs or min(8, os.cpu_count() or 1)
     with ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(compile_one, srcs))
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(HIPCC))), "lib")
+    cmd = ([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs +
+           # rocBLAS / rocSOLVER: GEMM and eigensolves of the device-resident SCF
+           ["-L", rocm_lib, "-lrocsolver", "-lrocblas", "-Wl,-rpath," + rocm_lib])
     subprocess.run(cmd, check=True)
     os.replace(LIB + ".tmp", LIB)
     return LIB

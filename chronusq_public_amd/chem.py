@@ -125,18 +125,19 @@ def alkane_chain(ncarbon, seed, jitter=0.05, cc=1.53, ch=1.09):
     atoms = [("C", *c) for c in carbons]
     hs = []
     for i, c in enumerate(carbons):
-        # the two in-plane neighbours set the local frame
+        # both chain neighbours lie on the far side in y: the H-C-H bisector
+        # points away from them (+y for the upper carbons), the two H leave
+        # the chain plane symmetrically, H-C-H tetrahedral
         up = np.array([0.0, 1.0 if i % 2 else -1.0, 0.0])
-        # two H out of plane, tetrahedral about the carbon
         for s in (1.0, -1.0):
-            v = -up * np.cos(np.deg2rad(54.75)) + np.array([0.0, 0.0, s]) * np.sin(np.deg2rad(54.75))
-            hs.append(c + ch * v / np.linalg.norm(v))
-    ends = [(0, -1.0), (ncarbon - 1, 1.0)]
-    for i, sgn in ends:
+            v = up * np.cos(half) + np.array([0.0, 0.0, s]) * np.sin(half)
+            hs.append(c + ch * v)
+    # the terminal carbons take a third H where the next chain carbon would be
+    for i, sgn in ((0, -1.0), (ncarbon - 1, 1.0)):
         c = carbons[i]
-        up = np.array([0.0, 1.0 if i % 2 else -1.0, 0.0])
-        v = np.array([sgn * np.sin(half), up[1] * np.cos(half), 0.0])
-        hs.append(c + ch * v / np.linalg.norm(v))
+        yside = 1.0 if i % 2 else -1.0
+        v = np.array([sgn * np.sin(half), -yside * np.cos(half), 0.0])
+        hs.append(c + ch * v)
     atoms += [("H", *h) for h in hs]
     mol = Molecule.from_angstrom(atoms)
     mol.xyz = mol.xyz + rng.uniform(-jitter, jitter, size=mol.xyz.shape) / ANG_PER_BOHR

@@ -42,7 +42,8 @@ class Stats(C.Structure):
     _fields_ = [("n_candidates", C.c_longlong), ("n_quartets", C.c_longlong),
                 ("n_prim_quartets_max", C.c_longlong), ("flops_model", C.c_double),
                 ("ms_total", C.c_double), ("ms_eri", C.c_double),
-                ("n_launch_eri", C.c_int)]
+                ("n_launch_eri", C.c_int), ("n_quartets_f32", C.c_longlong),
+                ("flops_model_f32", C.c_double)]
 
 
 SYMBOLS = ["cqx_basis_create", "cqx_basis_from_g94", "cqx_basis_destroy",
@@ -50,7 +51,12 @@ SYMBOLS = ["cqx_basis_create", "cqx_basis_from_g94", "cqx_basis_destroy",
            "cqx_ctx_create", "cqx_ctx_destroy", "cqx_last_error", "cqx_ctx_schwarz",
            "cqx_ctx_shell_pairs", "cqx_two_body_contract", "cqx_eri_quartets",
            "cqx_last_stats", "cqx_set_option", "cqx_device_alloc", "cqx_device_free",
-           "cqx_memcpy", "cqx_device_zero"]
+           "cqx_memcpy", "cqx_device_zero", "cqx_one_body", "cqx_scf_default_opts",
+           "cqx_scf_create", "cqx_scf_iterate", "cqx_scf_run", "cqx_scf_result_get",
+           "cqx_scf_matrix", "cqx_scf_orbital_energies", "cqx_scf_destroy",
+           "cqx_scf_last_error"]
+
+OVERLAP, KINETIC, POTENTIAL = 0, 1, 2
 
 
 def lib():
@@ -81,6 +87,7 @@ def lib():
         L.cqx_device_free.argtypes = [P, P]
         L.cqx_memcpy.argtypes = [P, P, P, C.c_size_t, I]
         L.cqx_device_zero.argtypes = [P, P, C.c_size_t]
+        L.cqx_one_body.argtypes = [P, I, I, P, P, P]
         _lib = L
     return _lib
 
@@ -124,6 +131,18 @@ class BasisSet:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.cqx_basis_destroy(self._h)
             self._h = None
+
+    def one_body(self, kind, Z=None, xyz=None):
+        """S, T or V (point nuclei; default: this basis' molecule) as an
+        NB x NB array -- OnePInts::computeAOInts (aointegrals_onee_drivers.cxx:762-784)."""
+        out = np.zeros((self.nBasis, self.nBasis), order="F")
+        if kind == POTENTIAL:
+            Z = np.ascontiguousarray(self.mol.Z if Z is None else Z, dtype=np.float64)
+            xyz = np.ascontiguousarray(self.mol.xyz if xyz is None else xyz, dtype=np.float64)
+            _check(lib().cqx_one_body(self._h, kind, len(Z), _ptr(Z), _ptr(xyz), _ptr(out)))
+        else:
+            _check(lib().cqx_one_body(self._h, kind, 0, None, None, _ptr(out)))
+        return out
 
     def shell(self, s):
         v = [C.c_int() for _ in range(5)]
@@ -301,3 +320,106 @@ class FockBuilder:
             ss.exchangeMatrix[c][...] = K[c]
             ss.twoeH[c][...] = (2.0 * J if c == 0 else 0.0) - (xHFX * K[c] if doK else 0.0)
         return ss
+
+
+class ScfOpts(C.Structure):
+    _fields_ = [("nocc", C.c_int), ("max_iter", C.c_int), ("rmsd_tol", C.c_double),
+                ("maxd_tol", C.c_double), ("ene_tol", C.c_double), ("diis_keep", C.c_int),
+                ("guess", C.c_int), ("fp32_cut", C.c_double)]
+
+
+class ScfResult(C.Structure):
+    _fields_ = [("e_total", C.c_double), ("e_one", C.c_double), ("e_two", C.c_double),
+                ("e_nuc", C.c_double), ("iterations", C.c_int), ("converged", C.c_int),
+                ("rmsd", C.c_double), ("maxd", C.c_double), ("delta_e", C.c_double),
+                ("ms_setup", C.c_double), ("ms_fock", C.c_double), ("ms_total", C.c_double)]
+
+
+GUESS = {"CORE": 0, "SAD": 1}
+SCF_DENSITY, SCF_FOCK, SCF_HCORE, SCF_OVERLAP = 0, 1, 2, 3
+
+
+def _scf_lib():
+    L = lib()
+    if not getattr(L, "_scf_typed", False):
+        P, I = C.c_void_p, C.c_int
+        L.cqx_scf_default_opts.argtypes = [C.POINTER(ScfOpts)]
+        L.cqx_scf_default_opts.restype = None
+        L.cqx_scf_create.argtypes = [P, I, P, P, C.POINTER(ScfOpts), C.POINTER(P)]
+        L.cqx_scf_iterate.argtypes = [P, C.POINTER(I)]
+        L.cqx_scf_run.argtypes = [P, C.POINTER(ScfResult)]
+        L.cqx_scf_result_get.argtypes = [P, C.POINTER(ScfResult)]
+        L.cqx_scf_matrix.argtypes = [P, I, P]
+        L.cqx_scf_orbital_energies.argtypes = [P, P]
+        L.cqx_scf_destroy.argtypes = [P]
+        L.cqx_scf_last_error.argtypes = [P]
+        L.cqx_scf_last_error.restype = C.c_char_p
+        L._scf_typed = True
+    return L
+
+
+class RHF:
+    """Closed-shell real Hartree-Fock: SingleSlater<double,double> with the
+    conventional SCF (orbitaloptimizernew/impl.hpp:40-137), run on the GPU
+    of a DirectTPI -- one-electron integrals, Lowdin orthogonaliser, SAD or
+    core guess, CDIIS, eigensolve and density all device-resident; Fock
+    builds through the direct J/K engine.  Energies in Hartree, matrices in
+    the Pauli-scalar convention (D = 2 C C^T, H = 2 (T + V))."""
+
+    def __init__(self, tpi, nocc=None, guess="SAD", fp32_cut=0.0, max_iter=128,
+                 rmsd_tol=1e-8, maxd_tol=1e-5, ene_tol=1e-5, diis_keep=8):
+        L = _scf_lib()
+        mol = tpi.basis.mol
+        if nocc is None:
+            ne = mol.nelectron()
+            if ne % 2:
+                raise CErr("RHF needs an even electron count")
+            nocc = ne // 2
+        o = ScfOpts()
+        L.cqx_scf_default_opts(C.byref(o))
+        o.nocc, o.max_iter, o.rmsd_tol, o.maxd_tol = int(nocc), int(max_iter), rmsd_tol, maxd_tol
+        o.ene_tol, o.diis_keep, o.guess, o.fp32_cut = ene_tol, int(diis_keep), GUESS[guess.upper()], fp32_cut
+        self.tpi, self.nb = tpi, tpi.basis.nBasis
+        self._Z = np.ascontiguousarray(mol.Z, dtype=np.float64)
+        self._xyz = np.ascontiguousarray(mol.xyz, dtype=np.float64)
+        h = C.c_void_p()
+        rc = L.cqx_scf_create(tpi._h, len(self._Z), _ptr(self._Z), _ptr(self._xyz), C.byref(o),
+                              C.byref(h))
+        if rc:
+            raise CErr(f"FATAL (cqx status {rc}): {L.cqx_scf_last_error(None).decode()}")
+        self._h = h
+
+    def _chk(self, rc):
+        if rc:
+            raise CErr(f"FATAL (cqx status {rc}): {_scf_lib().cqx_scf_last_error(self._h).decode()}")
+
+    def iterate(self):
+        done = C.c_int()
+        self._chk(_scf_lib().cqx_scf_iterate(self._h, C.byref(done)))
+        return bool(done.value)
+
+    def result(self):
+        r = ScfResult()
+        self._chk(_scf_lib().cqx_scf_result_get(self._h, C.byref(r)))
+        return {k: getattr(r, k) for k, _ in ScfResult._fields_}
+
+    def SCF(self):
+        """Iterate to convergence (CErr after max_iter, like the reference)."""
+        r = ScfResult()
+        self._chk(_scf_lib().cqx_scf_run(self._h, C.byref(r)))
+        return {k: getattr(r, k) for k, _ in ScfResult._fields_}
+
+    def matrix(self, which):
+        out = np.zeros((self.nb, self.nb), order="F")
+        self._chk(_scf_lib().cqx_scf_matrix(self._h, which, _ptr(out)))
+        return out
+
+    def orbital_energies(self):
+        out = np.zeros(self.nb)
+        self._chk(_scf_lib().cqx_scf_orbital_energies(self._h, _ptr(out)))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.cqx_scf_destroy(self._h)
+            self._h = None

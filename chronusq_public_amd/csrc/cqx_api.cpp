@@ -18,6 +18,8 @@
 #include "basis.hpp"
 #include "device_types.hpp"
 #include "kernels.hpp"
+#include "onebody.hpp"
+#include "ctx_access.hpp"
 
 namespace {
 
@@ -25,7 +27,7 @@ const double kRysTable[] = {
 #include "rys_table_data.inc"
 };
 
-std::string g_error;
+thread_local std::string g_error;  // error of the last failing call of this thread
 
 }  // namespace
 
@@ -88,6 +90,7 @@ struct cqx_ctx {
   double prim_thresh = 1e-18;
   int fast_max_nc = 27;   // classes up to this Cartesian size use the lane-group kernels
   int use_wave = 1;       // templated wave-per-quartet kernels for the other s/p/d classes
+  double fp32_cut = 0.0;  // quartets with Q_ab Q_cd below this are evaluated in FP32
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
@@ -101,6 +104,7 @@ struct cqx_ctx {
   DevBuf<PairRec> d_pairs[kNumClass];
   DevBuf<PrimPair> d_prims;
   DevBuf<double> d_rys, d_sph;
+  DevBuf<float> d_rys32;
   DevBuf<int> d_cart_shell, d_cart_comp, d_sh_bf, d_sh_bfc, d_sh_l, d_sh_pure, d_sh_nf,
       d_bf_shell, d_bf_comp;
   cqx::CartMap cmap{};
@@ -378,6 +382,7 @@ extern "C" {
 
 const char *cqx_last_error(const cqx_ctx *ctx) { return ctx ? ctx->err.c_str() : g_error.c_str(); }
 
+
 int cqx_basis_create(int nshell, const int *l, const int *pure, const int *nprim, const double *alpha,
                      const double *coef, const double *center, const int *atom, cqx_basis **out) {
   if (!out || nshell <= 0 || !l || !pure || !nprim || !alpha || !coef || !center || !atom)
@@ -430,6 +435,25 @@ int cqx_basis_dims(const cqx_basis *b, int *nshell, int *nbasis, int *max_prim, 
   return CQX_OK;
 }
 
+int cqx_one_body(const cqx_basis *b, int kind, int ncharge, const double *Z, const double *xyz,
+                 double *out) {
+  if (!b || !out || kind < CQX_OVERLAP || kind > CQX_POTENTIAL ||
+      (kind == CQX_POTENTIAL && (ncharge < 0 || (ncharge && (!Z || !xyz)))))
+    return fail(nullptr, CQX_ERR_ARG, "cqx_one_body: bad arguments");
+  try {
+    std::vector<double> Zc, C;
+    if (kind == CQX_POTENTIAL) {
+      Zc.assign(Z, Z + ncharge);
+      C.assign(xyz, xyz + 3 * (size_t)ncharge);
+    }
+    const auto M = cqx::one_body_matrix(b->b, kind, Zc, C);
+    std::copy(M.begin(), M.end(), out);
+  } catch (const std::exception &e) {
+    return fail(nullptr, CQX_ERR_ARG, e.what());
+  }
+  return CQX_OK;
+}
+
 int cqx_basis_shell(const cqx_basis *b, int s, int *l, int *pure, int *nprim, int *bf, int *atom) {
   if (!b || s < 0 || s >= (int)b->b.shells.size()) return fail(nullptr, CQX_ERR_ARG, "bad shell index");
   const auto &sh = b->b.shells[s];
@@ -468,6 +492,9 @@ int cqx_ctx_create(const cqx_basis *b, double thresh, int device, cqx_ctx **out)
     c->nbc = c->basis.ncart_total;
     std::vector<double> rys(kRysTable, kRysTable + sizeof(kRysTable) / sizeof(double));
     c->d_rys.upload(rys, c->stream);
+    std::vector<float> rys32(rys.begin(), rys.end());
+    c->d_rys32.upload(rys32, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
     setup_device_maps(c.get());
     build_pairs(c.get());
   } catch (const std::exception &e) {
@@ -523,6 +550,11 @@ int cqx_set_option(cqx_ctx *c, const char *name, double value) {
   }
   if (!std::strcmp(name, "use_wave")) {
     c->use_wave = value != 0.0;
+    return CQX_OK;
+  }
+  if (!std::strcmp(name, "fp32_cut")) {
+    if (!(value >= 0.0)) return fail(c, CQX_ERR_ARG, "fp32_cut must be >= 0");
+    c->fp32_cut = value;
     return CQX_OK;
   }
   return fail(c, CQX_ERR_ARG, std::string("unknown option ") + name);
@@ -696,12 +728,23 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           cps.push_back(std::move(cp));
         }
       c->stats.n_candidates += total;
-      c->d_quart.ensure((size_t)std::max<long long>(total, 1));
-      c->d_count.ensure(std::max<size_t>(cps.size(), 1));
+      // kernel family of a class: 0 lane-group, 1 wave-per-quartet, 2 generic
+      auto route = [&](int la, int lb, int lc, int ld) {
+        const int ncq = cqx::ncart(la) * cqx::ncart(lb) * cqx::ncart(lc) * cqx::ncart(ld);
+        if (ncq <= c->fast_max_nc && cqx::eri_fast_available(la, lb, lc, ld)) return 0;
+        if (c->use_wave && cqx::eri_wave_available(la, lb, lc, ld)) return 1;
+        return 2;
+      };
+      // mixed precision: a second list per class pair (FP32 evaluation) for
+      // the classes whose kernel family has an FP32 instance
+      const bool mixed = c->fp32_cut > 0.0;
+      const size_t ncp = cps.size(), nslot = std::max<size_t>(2 * ncp, 1);
+      c->d_quart.ensure((size_t)std::max<long long>(total, 1) * (mixed ? 2 : 1));
+      c->d_count.ensure(nslot);
       // counters cleared by a fill kernel (no memset nodes if the caller captures a graph)
-      HIPCHK(cqx::launch_zero(reinterpret_cast<double *>(c->d_count.p), std::max<size_t>(cps.size(), 1), s));
-      c->d_psum.ensure(std::max<size_t>(cps.size(), 1));
-      HIPCHK(cqx::launch_zero(c->d_psum.p, std::max<size_t>(cps.size(), 1), s));
+      HIPCHK(cqx::launch_zero(reinterpret_cast<double *>(c->d_count.p), nslot, s));
+      c->d_psum.ensure(nslot);
+      HIPCHK(cqx::launch_zero(c->d_psum.p, nslot, s));
       size_t cand_total = 0;
       for (auto &cp : cps) cand_total += cp.off.size();
       c->d_cand.ensure(std::max<size_t>(cand_total, 1));
@@ -731,47 +774,67 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         LL.count = c->d_count.p + k;
         LL.prim_sum = c->d_psum.p + k;
         LL.cap = cp.ncand;
+        int lb_, la_, lc_, ld_;
+        class_ls(cp.cb, la_, lb_);
+        class_ls(cp.ck, lc_, ld_);
+        if (mixed && route(la_, lb_, lc_, ld_) < 2) {
+          LL.cut32 = c->fp32_cut;
+          LL.out32 = c->d_quart.p + total + cp.seg;
+          LL.count32 = c->d_count.p + ncp + k;
+          LL.prim_sum32 = c->d_psum.p + ncp + k;
+        }
         HIPCHK(cqx::launch_quartet_list(LL, s));
         coff += cp.off.size();
       }
-      std::vector<unsigned long long> counts(cps.size());
-      std::vector<double> psum(cps.size());
-      if (!cps.empty()) {
-        HIPCHK(hipMemcpyAsync(counts.data(), c->d_count.p, sizeof(unsigned long long) * cps.size(),
+      std::vector<unsigned long long> counts(2 * ncp);
+      std::vector<double> psum(2 * ncp);
+      if (ncp) {
+        HIPCHK(hipMemcpyAsync(counts.data(), c->d_count.p, sizeof(unsigned long long) * 2 * ncp,
                               hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(psum.data(), c->d_psum.p, sizeof(double) * cps.size(),
+        HIPCHK(hipMemcpyAsync(psum.data(), c->d_psum.p, sizeof(double) * 2 * ncp,
                               hipMemcpyDeviceToHost, s));
       }
       HIPCHK(hipStreamSynchronize(s));
       // ERI + digestion
       HIPCHK(hipEventRecord(c->ev0, s));
-      for (size_t k = 0; k < cps.size(); ++k) {
-        if (!counts[k]) continue;
+      for (size_t kk = 0; kk < 2 * ncp; ++kk) {
+        if (!counts[kk]) continue;
+        const size_t k = kk % ncp;
+        const bool f32 = kk >= ncp;
         auto &cp = cps[k];
         cqx::EriLaunch E{};
         class_ls(cp.cb, E.la, E.lb);
         class_ls(cp.ck, E.lc, E.ld);
-        E.quartets = c->d_quart.p + cp.seg;
-        E.nq = (long long)counts[k];
+        E.quartets = c->d_quart.p + (f32 ? total : 0) + cp.seg;
+        E.nq = (long long)counts[kk];
         E.bra = c->d_pairs[cp.cb].p;
         E.ket = c->d_pairs[cp.ck].p;
         E.prims = c->d_prims.p;
         E.rys = c->d_rys.p;
+        E.rys32 = c->d_rys32.p;
+        E.fp32 = f32 ? 1 : 0;
         E.planes = ps;
         E.store_mode = 0;
-        const int ncq = cqx::ncart(E.la) * cqx::ncart(E.lb) * cqx::ncart(E.lc) * cqx::ncart(E.ld);
+        const int r = route(E.la, E.lb, E.lc, E.ld);
         hipError_t e;
-        if (ncq <= c->fast_max_nc && cqx::eri_fast_available(E.la, E.lb, E.lc, E.ld))
+        if (r == 0)
           e = cqx::launch_eri_fast(E, s);
-        else if (c->use_wave && cqx::eri_wave_available(E.la, E.lb, E.lc, E.ld))
+        else if (r == 1)
           e = cqx::launch_eri_wave(E, s);
-        else
+        else if (!f32)
           e = cqx::launch_eri_team(E, s);
+        else
+          throw std::runtime_error("FP32 quartet list for a class without an FP32 kernel");
         HIPCHK(e);
-        c->stats.n_quartets += (long long)counts[k];
-        c->stats.n_prim_quartets_max += (long long)psum[k];
-        c->stats.flops_model += class_pair_flops(E.la, E.lb, E.lc, E.ld, pure_d, psum[k],
-                                                 (double)counts[k], is_complex ? 2 : 1, nJ, nK);
+        const double fl = class_pair_flops(E.la, E.lb, E.lc, E.ld, pure_d, psum[kk], (double)counts[kk],
+                                           is_complex ? 2 : 1, nJ, nK);
+        c->stats.n_quartets += (long long)counts[kk];
+        c->stats.n_prim_quartets_max += (long long)psum[kk];
+        c->stats.flops_model += fl;
+        if (f32) {
+          c->stats.n_quartets_f32 += (long long)counts[kk];
+          c->stats.flops_model_f32 += fl;
+        }
         c->stats.n_launch_eri++;
       }
       HIPCHK(hipEventRecord(c->ev1, s));
@@ -835,3 +898,10 @@ int cqx_device_zero(cqx_ctx *c, void *ptr, size_t bytes) {
 }
 
 }  // extern "C"
+namespace cqx {
+hipStream_t ctx_stream(const cqx_ctx *c) { return c->stream; }
+int ctx_device(const cqx_ctx *c) { return c->device; }
+double ctx_thresh(const cqx_ctx *c) { return c->thresh; }
+const Basis &ctx_basis(const cqx_ctx *c) { return c->basis; }
+int ctx_fail(cqx_ctx *c, int code, const std::string &msg) { return fail(c, code, msg); }
+}  // namespace cqx

@@ -51,12 +51,11 @@ struct Cls {
 
 // 2D integrals of one root along one direction: VRR on (i, k), then HRR to
 // (ia, ib, ic, id); H has Cls::NH entries.
-template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void rys_2d_full(double C00, double D00, double B10, double B01,
-                                            double B00, double ABd, double CDd, double scale,
-                                            double (&H)[Cls<LA, LB, LC, LD>::NH]) {
+template <int LA, int LB, int LC, int LD, class R>
+__device__ __forceinline__ void rys_2d_full(R C00, R D00, R B10, R B01, R B00, R ABd, R CDd, R scale,
+                                            R (&H)[Cls<LA, LB, LC, LD>::NH]) {
   using K = Cls<LA, LB, LC, LD>;
-  double G[K::NI][K::NK];
+  R G[K::NI][K::NK];
   G[0][0] = scale;
   if constexpr (K::NI > 1) G[1][0] = C00 * scale;
 #pragma unroll
@@ -75,7 +74,7 @@ __device__ __forceinline__ void rys_2d_full(double C00, double D00, double B10, 
     }
   }
   // bra transfer: E[i][j][k], (i, j+1) = (i+1, j) + AB (i, j)
-  double E[K::NI][LB + 1][K::NK];
+  R E[K::NI][LB + 1][K::NK];
 #pragma unroll
   for (int i = 0; i < K::NI; ++i)
 #pragma unroll
@@ -91,7 +90,7 @@ __device__ __forceinline__ void rys_2d_full(double C00, double D00, double B10, 
   for (int ia = 0; ia <= LA; ++ia)
 #pragma unroll
     for (int ib = 0; ib <= LB; ++ib) {
-      double F[K::NK][LD + 1];
+      R F[K::NK][LD + 1];
 #pragma unroll
       for (int k = 0; k < K::NK; ++k) F[k][0] = E[ia][ib][k];
 #pragma unroll
@@ -173,10 +172,17 @@ __device__ __forceinline__ void digest_plane(const double (&I)[K::NC], const int
   }
 }
 
+// Rys table of the evaluation precision
+template <class R> __device__ __forceinline__ const R *rys_table(const EriLaunch &a);
+template <> __device__ __forceinline__ const double *rys_table<double>(const EriLaunch &a) { return a.rys; }
+template <> __device__ __forceinline__ const float *rys_table<float>(const EriLaunch &a) { return a.rys32; }
+
 // G lanes per quartet: the lanes split the primitive quartets of the shell
 // quartet, combine the contracted Cartesian integrals with a butterfly of
 // lane shuffles, and then lane g digests planes g, g + G, ...
-template <int LA, int LB, int LC, int LD, int G>
+// R is the evaluation precision (Rys nodes, 2D integrals, the contracted
+// quartet); the digestion and the accumulators are FP64 either way.
+template <int LA, int LB, int LC, int LD, int G, class R>
 __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
   using K = Cls<LA, LB, LC, LD>;
   const int g = threadIdx.x & (G - 1);
@@ -189,9 +195,10 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
     bp = a.bra[qt.bra];
     kp = a.ket[qt.ket_mask & 0xFFFFFFu];
   }
-  double I[K::NC];
+  const R *__restrict__ rys = rys_table<R>(a);
+  R I[K::NC];
 #pragma unroll
-  for (int o = 0; o < K::NC; ++o) I[o] = 0.0;
+  for (int o = 0; o < K::NC; ++o) I[o] = 0;
   const int nk = kp.npp, ntot = bp.npp * kp.npp;
   int i = 0, j = g;
   while (nk > 0 && j >= nk) {
@@ -206,33 +213,35 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
       j -= nk;
       ++i;
     }
-    const double pq = A.p + B.p, rpq = 1.0 / pq;
-    const double rho = A.p * B.p * rpq;
-    const double PQ[3] = {A.P[0] - B.P[0], A.P[1] - B.P[1], A.P[2] - B.P[2]};
-    const double T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
-    const double pref = A.K * B.K * kTwoPi52 / (A.p * B.p * sqrt(pq));
-    double x[K::NR], w[K::NR];
-    rys_roots<K::NR>(T, a.rys, x, w);
+    const R ap = (R)A.p, bq = (R)B.p;
+    const R pq = ap + bq, rpq = (R)1 / pq;
+    const R rho = ap * bq * rpq;
+    // centre differences in FP64 before any rounding to R
+    const R PQ[3] = {(R)(A.P[0] - B.P[0]), (R)(A.P[1] - B.P[1]), (R)(A.P[2] - B.P[2])};
+    const R T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+    const R pref = (R)(A.K * B.K * kTwoPi52) / (ap * bq * sqrt(pq));
+    R x[K::NR], w[K::NR];
+    rys_roots<K::NR, R>(T, rys, x, w);
     if constexpr (K::NC == 1) {
-      double s = 0.0;
+      R s = 0;
 #pragma unroll
       for (int r = 0; r < K::NR; ++r) s += w[r];
       I[0] = fma(pref, s, I[0]);
     } else {
-      const double rp = 1.0 / A.p, rq = 1.0 / B.p;
+      const R rp = (R)1 / ap, rq = (R)1 / bq;
 #pragma unroll
       for (int r = 0; r < K::NR; ++r) {
-        const double t2 = x[r];
-        const double B00 = 0.5 * t2 * rpq;
-        const double B10 = 0.5 * rp * (1.0 - rho * rp * t2);
-        const double B01 = 0.5 * rq * (1.0 - rho * rq * t2);
-        double H[3][K::NH];
+        const R t2 = x[r];
+        const R B00 = (R)0.5 * t2 * rpq;
+        const R B10 = (R)0.5 * rp * ((R)1 - rho * rp * t2);
+        const R B01 = (R)0.5 * rq * ((R)1 - rho * rq * t2);
+        R H[3][K::NH];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          const double C00 = A.PA[d] - B.p * rpq * t2 * PQ[d];
-          const double D00 = B.PA[d] + A.p * rpq * t2 * PQ[d];
-          rys_2d_full<LA, LB, LC, LD>(C00, D00, B10, B01, B00, bp.AB[d], kp.AB[d],
-                                      d == 2 ? pref * w[r] : 1.0, H[d]);
+          const R C00 = (R)A.PA[d] - bq * rpq * t2 * PQ[d];
+          const R D00 = (R)B.PA[d] + ap * rpq * t2 * PQ[d];
+          rys_2d_full<LA, LB, LC, LD, R>(C00, D00, B10, B01, B00, (R)bp.AB[d], (R)kp.AB[d],
+                                         d == 2 ? pref * w[r] : (R)1, H[d]);
         }
 #pragma unroll
         for (int ia = 0; ia < K::NCA; ++ia)
@@ -242,10 +251,10 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
             for (int ic = 0; ic < K::NCC; ++ic)
 #pragma unroll
               for (int id = 0; id < K::NCD; ++id) {
-                const double hx = H[0][K::hidx(cexp(LA, ia, 0), cexp(LB, ib, 0), cexp(LC, ic, 0), cexp(LD, id, 0))];
-                const double hy = H[1][K::hidx(cexp(LA, ia, 1), cexp(LB, ib, 1), cexp(LC, ic, 1), cexp(LD, id, 1))];
-                const double hz = H[2][K::hidx(cexp(LA, ia, 2), cexp(LB, ib, 2), cexp(LC, ic, 2), cexp(LD, id, 2))];
-                double &o = I[K::oidx(ia, ib, ic, id)];
+                const R hx = H[0][K::hidx(cexp(LA, ia, 0), cexp(LB, ib, 0), cexp(LC, ic, 0), cexp(LD, id, 0))];
+                const R hy = H[1][K::hidx(cexp(LA, ia, 1), cexp(LB, ib, 1), cexp(LC, ic, 1), cexp(LD, id, 1))];
+                const R hz = H[2][K::hidx(cexp(LA, ia, 2), cexp(LB, ib, 2), cexp(LC, ic, 2), cexp(LD, id, 2))];
+                R &o = I[K::oidx(ia, ib, ic, id)];
                 o = fma(hx * hy, hz, o);
               }
       }
@@ -261,21 +270,22 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
   // degeneracy (direct.hpp:1664-1667, 1793-1802) and the 1/2 of :1849-1850
   const bool same_pair = (bp.a == kp.a && bp.b == kp.b);
   const double scale = 0.5 * (bp.a == bp.b ? 1.0 : 2.0) * (kp.a == kp.b ? 1.0 : 2.0) * (same_pair ? 1.0 : 2.0);
+  double Id[K::NC];
 #pragma unroll
-  for (int o = 0; o < K::NC; ++o) I[o] *= scale;
+  for (int o = 0; o < K::NC; ++o) Id[o] = (double)I[o] * scale;
   const int bf[4] = {bp.bfa, bp.bfb, kp.bfa, kp.bfb};
   const unsigned mask = qt.ket_mask >> 24;
   for (int p = g; p < a.planes.n; p += G) {
     const Plane &P = a.planes.p[p];
     if (!((mask >> P.mat) & 1u)) continue;
-    digest_plane<K>(I, bf, P, a.planes.nbc, a.planes.ps);
+    digest_plane<K>(Id, bf, P, a.planes.nbc, a.planes.ps);
   }
 }
 
-template <int LA, int LB, int LC, int LD, int G>
+template <int LA, int LB, int LC, int LD, int G, class R>
 hipError_t launch_cls(const EriLaunch &a, hipStream_t s) {
   const long long threads = a.nq * G;
-  hipLaunchKernelGGL((eri_fast_kernel<LA, LB, LC, LD, G>), dim3((unsigned)((threads + 255) / 256)),
+  hipLaunchKernelGGL((eri_fast_kernel<LA, LB, LC, LD, G, R>), dim3((unsigned)((threads + 255) / 256)),
                      dim3(256), 0, s, a);
   return hipGetLastError();
 }
@@ -294,11 +304,11 @@ hipError_t launch_cls(const EriLaunch &a, hipStream_t s) {
   X(2, 0, 1, 0)             \
   X(2, 1, 0, 0)
 
-// all classes of the list for one lane-group size G
-template <int G>
+// all classes of the list for one lane-group size G and precision R
+template <int G, class R>
 hipError_t launch_fast_g(const EriLaunch &a, hipStream_t s) {
 #define CQX_CASE(A, B, C, D) \
-  if (a.la == A && a.lb == B && a.lc == C && a.ld == D) return launch_cls<A, B, C, D, G>(a, s);
+  if (a.la == A && a.lb == B && a.lc == C && a.ld == D) return launch_cls<A, B, C, D, G, R>(a, s);
   CQX_FAST_CLASSES(CQX_CASE)
 #undef CQX_CASE
   return hipErrorNotSupported;

@@ -18,12 +18,14 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;
 
-template <int LA, int LB, int LC, int LD>
+// R is the evaluation precision; the quartet handed to the digestion and
+// the accumulators are FP64 either way.
+template <int LA, int LB, int LC, int LD, class R>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch a) {
   using K = Cls<LA, LB, LC, LD>;
   constexpr int NOWN = (K::NC + 63) / 64;
   constexpr int HSZ = 3 * K::NR * K::NH;
-  __shared__ double s_H[kWavesPerBlock][HSZ];
+  __shared__ R s_H[kWavesPerBlock][HSZ];
   __shared__ double s_I[kWavesPerBlock][K::NC];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long long q = (long long)blockIdx.x * kWavesPerBlock + wv;
@@ -31,15 +33,16 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
   const Quartet qt = a.quartets[q];
   const PairRec bp = a.bra[qt.bra];
   const PairRec kp = a.ket[qt.ket_mask & 0xFFFFFFu];
-  double *H = s_H[wv];
+  R *H = s_H[wv];
   double *sI = s_I[wv];
+  const R *__restrict__ rys = rys_table<R>(a);
 
   // the lane's outputs o = lane + 64 k and their 2D-table offsets
   int ox[NOWN], oy[NOWN], oz[NOWN];
-  double acc[NOWN];
+  R acc[NOWN];
 #pragma unroll
   for (int k = 0; k < NOWN; ++k) {
-    acc[k] = 0.0;
+    acc[k] = 0;
     int o = lane + 64 * k;
     if (o >= K::NC) o = 0;
     int rem = o;
@@ -56,35 +59,37 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
     const PrimPair A = a.prims[bp.pp_off + i];
     for (int j = 0; j < kp.npp; ++j) {
       const PrimPair B = a.prims[kp.pp_off + j];
-      const double pq = A.p + B.p, rpq = 1.0 / pq;
-      const double rho = A.p * B.p * rpq;
-      const double PQ[3] = {A.P[0] - B.P[0], A.P[1] - B.P[1], A.P[2] - B.P[2]};
-      const double T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
-      const double pref = A.K * B.K * kTwoPi52 / (A.p * B.p * sqrt(pq));
+      const R ap = (R)A.p, bq = (R)B.p;
+      const R pq = ap + bq, rpq = (R)1 / pq;
+      const R rho = ap * bq * rpq;
+      // centre differences in FP64 before any rounding to R
+      const R PQ[3] = {(R)(A.P[0] - B.P[0]), (R)(A.P[1] - B.P[1]), (R)(A.P[2] - B.P[2])};
+      const R T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+      const R pref = (R)(A.K * B.K * kTwoPi52) / (ap * bq * sqrt(pq));
       if (lane < 3 * K::NR) {
-        double x[K::NR], w[K::NR];
-        rys_roots<K::NR>(T, a.rys, x, w);
-        double t2 = x[0], wr = w[0];
+        R x[K::NR], w[K::NR];
+        rys_roots<K::NR, R>(T, rys, x, w);
+        R t2 = x[0], wr = w[0];
 #pragma unroll
         for (int r = 1; r < K::NR; ++r)
           if (my_r == r) {
             t2 = x[r];
             wr = w[r];
           }
-        const double rp = 1.0 / A.p, rq = 1.0 / B.p;
-        const double B00 = 0.5 * t2 * rpq;
-        const double B10 = 0.5 * rp * (1.0 - rho * rp * t2);
-        const double B01 = 0.5 * rq * (1.0 - rho * rq * t2);
-        const double PQd = my_d == 0 ? PQ[0] : (my_d == 1 ? PQ[1] : PQ[2]);
-        const double PAd = my_d == 0 ? A.PA[0] : (my_d == 1 ? A.PA[1] : A.PA[2]);
-        const double QCd = my_d == 0 ? B.PA[0] : (my_d == 1 ? B.PA[1] : B.PA[2]);
-        const double ABd = my_d == 0 ? bp.AB[0] : (my_d == 1 ? bp.AB[1] : bp.AB[2]);
-        const double CDd = my_d == 0 ? kp.AB[0] : (my_d == 1 ? kp.AB[1] : kp.AB[2]);
-        const double C00 = PAd - B.p * rpq * t2 * PQd;
-        const double D00 = QCd + A.p * rpq * t2 * PQd;
-        double h[K::NH];
-        rys_2d_full<LA, LB, LC, LD>(C00, D00, B10, B01, B00, ABd, CDd, my_d == 2 ? pref * wr : 1.0, h);
-        double *dst = H + (my_r * 3 + my_d) * K::NH;
+        const R rp = (R)1 / ap, rq = (R)1 / bq;
+        const R B00 = (R)0.5 * t2 * rpq;
+        const R B10 = (R)0.5 * rp * ((R)1 - rho * rp * t2);
+        const R B01 = (R)0.5 * rq * ((R)1 - rho * rq * t2);
+        const R PQd = my_d == 0 ? PQ[0] : (my_d == 1 ? PQ[1] : PQ[2]);
+        const R PAd = (R)(my_d == 0 ? A.PA[0] : (my_d == 1 ? A.PA[1] : A.PA[2]));
+        const R QCd = (R)(my_d == 0 ? B.PA[0] : (my_d == 1 ? B.PA[1] : B.PA[2]));
+        const R ABd = (R)(my_d == 0 ? bp.AB[0] : (my_d == 1 ? bp.AB[1] : bp.AB[2]));
+        const R CDd = (R)(my_d == 0 ? kp.AB[0] : (my_d == 1 ? kp.AB[1] : kp.AB[2]));
+        const R C00 = PAd - bq * rpq * t2 * PQd;
+        const R D00 = QCd + ap * rpq * t2 * PQd;
+        R h[K::NH];
+        rys_2d_full<LA, LB, LC, LD, R>(C00, D00, B10, B01, B00, ABd, CDd, my_d == 2 ? pref * wr : (R)1, h);
+        R *dst = H + (my_r * 3 + my_d) * K::NH;
 #pragma unroll
         for (int e = 0; e < K::NH; ++e) dst[e] = h[e];
       }
@@ -92,10 +97,10 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
 #pragma unroll
       for (int k = 0; k < NOWN; ++k) {
         if (lane + 64 * k >= K::NC) continue;
-        double s = 0.0;
+        R s = 0;
 #pragma unroll
         for (int r = 0; r < K::NR; ++r) {
-          const double *Hr = H + r * 3 * K::NH;
+          const R *Hr = H + r * 3 * K::NH;
           s = fma(Hr[ox[k]] * Hr[oy[k]], Hr[oz[k]], s);
         }
         acc[k] += s;
@@ -108,7 +113,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
   const double scale = 0.5 * (bp.a == bp.b ? 1.0 : 2.0) * (kp.a == kp.b ? 1.0 : 2.0) * (same_pair ? 1.0 : 2.0);
 #pragma unroll
   for (int k = 0; k < NOWN; ++k)
-    if (lane + 64 * k < K::NC) sI[lane + 64 * k] = acc[k] * scale;
+    if (lane + 64 * k < K::NC) sI[lane + 64 * k] = (double)acc[k] * scale;
   wave_sync();
   const int n[4] = {K::NCA, K::NCB, K::NCC, K::NCD};
   const int bf[4] = {bp.bfa, bp.bfb, kp.bfa, kp.bfb};
@@ -118,8 +123,12 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_wave_cls(const EriLaunch &a, hipStream_t s) {
   const long long blocks = (a.nq + kWavesPerBlock - 1) / kWavesPerBlock;
-  hipLaunchKernelGGL((eri_wave_kernel<LA, LB, LC, LD>), dim3((unsigned)blocks),
-                     dim3(64 * kWavesPerBlock), 0, s, a);
+  if (a.fp32)
+    hipLaunchKernelGGL((eri_wave_kernel<LA, LB, LC, LD, float>), dim3((unsigned)blocks),
+                       dim3(64 * kWavesPerBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL((eri_wave_kernel<LA, LB, LC, LD, double>), dim3((unsigned)blocks),
+                       dim3(64 * kWavesPerBlock), 0, s, a);
   return hipGetLastError();
 }
 

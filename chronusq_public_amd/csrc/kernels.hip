@@ -183,17 +183,61 @@ __global__ __launch_bounds__(64) void eri_team_kernel(EriLaunch a) {
 // labels (direct.hpp:1689-1789); survivors appended to a list with the mask
 // of contractions that pass their own separated screen.
 // ---------------------------------------------------------------------------
+// Wave-aggregated append of the lanes with take set: one atomic per wave for
+// the slots and one for the sum of Ka Kb Kc Kd (the primitive-quartet bound).
+__device__ __forceinline__ void append_wave(bool take, int bi, int kj, unsigned mask,
+                                            const PairRec *bra, const PairRec *ket, Quartet *out,
+                                            unsigned long long *count, double *prim_sum,
+                                            long long cap, int lane) {
+  const unsigned long long bal = __ballot(take);
+  if (!bal) return;
+  double pk = take ? (double)bra[bi].kk * (double)ket[kj].kk : 0.0;
+  for (int off = 32; off > 0; off >>= 1) pk += __shfl_xor(pk, off);
+  const int leader = __ffsll((long long)bal) - 1;
+  unsigned long long start = 0;
+  if (lane == leader) {
+    start = atomicAdd(count, (unsigned long long)__popcll(bal));
+    atomic_add(prim_sum, pk);
+  }
+  start = __shfl(start, leader);
+  if (take) {
+    const unsigned long long below = bal & ((1ull << lane) - 1ull);
+    const unsigned long long slot = start + __popcll(below);
+    if ((long long)slot < cap) {
+      Quartet qv;
+      qv.bra = (uint32_t)bi;
+      qv.ket_mask = (uint32_t)kj | (mask << 24);
+      out[slot] = qv;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void quartet_list_kernel(ListLaunch a) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   const int lane = threadIdx.x & 63;
+  __shared__ int s_lo, s_hi;
   for (long long base = (long long)blockIdx.x * blockDim.x; base < a.ncand; base += stride) {
     const long long t = base + threadIdx.x;
     bool pass = false;
     unsigned mask = 0;
     int bi = 0, kj = 0;
+    // the block's candidates span bra indices [s_lo, s_hi]: two searches per
+    // block, then each thread searches that (usually short) range
+    if (threadIdx.x == 0 || threadIdx.x == blockDim.x - 1) {
+      const long long tt = threadIdx.x == 0 ? base : min(base + (long long)blockDim.x, a.ncand) - 1;
+      int lo = 0, hi = a.nbra - 1;
+      while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (a.cand_off[mid] <= tt) lo = mid; else hi = mid - 1;
+      }
+      if (threadIdx.x == 0) s_lo = lo; else s_hi = lo;
+    }
+    __syncthreads();
+    const int blo = s_lo, bhi = s_hi;
+    __syncthreads();
     if (t < a.ncand) {
       // bra index: last i with cand_off[i] <= t
-      int lo = 0, hi = a.nbra - 1;
+      int lo = blo, hi = bhi;
       while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
         if (a.cand_off[mid] <= t) lo = mid; else hi = mid - 1;
@@ -210,46 +254,30 @@ __global__ __launch_bounds__(256) void quartet_list_kernel(ListLaunch a) {
         mask = (1u << a.nmat) - 1u;
       } else {
         const double qq = P.Q * Q.Q;
-        auto bmax = [&](const double *B) {
-          double m = fmax(B[s1 + (size_t)s2 * ns], fmax(B[s1 + (size_t)s3 * ns], B[s2 + (size_t)s3 * ns]));
-          m = fmax(m, B[s3 + (size_t)s4 * ns]);
-          m = fmax(m, fmax(B[s1 + (size_t)s4 * ns], B[s2 + (size_t)s4 * ns]));
-          return m;
+        auto bmax = [&](const double *B, int ld, int m) {
+          double v = fmax(B[(s1 + (size_t)s2 * ns) * ld + m], fmax(B[(s1 + (size_t)s3 * ns) * ld + m],
+                                                                  B[(s2 + (size_t)s3 * ns) * ld + m]));
+          v = fmax(v, B[(s3 + (size_t)s4 * ns) * ld + m]);
+          v = fmax(v, fmax(B[(s1 + (size_t)s4 * ns) * ld + m], B[(s2 + (size_t)s4 * ns) * ld + m]));
+          return v;
         };
-        if (!(bmax(a.Bg) * qq < a.thresh)) {
+        if (!(bmax(a.Bg, 1, 0) * qq < a.thresh)) {
           if (a.nmat == 1) {
             mask = 1u;
           } else {
+            // per-matrix norms interleaved: the nmat values of one block share a line
             for (int m = 0; m < a.nmat; ++m)
-              if (!(bmax(a.Bm + (size_t)m * ns * ns) * qq < a.thresh)) mask |= 1u << m;
+              if (!(bmax(a.Bm, a.nmat, m) * qq < a.thresh)) mask |= 1u << m;
           }
           pass = mask != 0u;
         }
       }
     }
-    // wave-aggregated append
-    const unsigned long long bal = __ballot(pass);
-    if (bal) {
-      double pk = pass ? (double)a.bra[bi].kk * (double)a.ket[kj].kk : 0.0;
-      for (int off = 32; off > 0; off >>= 1) pk += __shfl_xor(pk, off);
-      const int leader = __ffsll((long long)bal) - 1;
-      unsigned long long start = 0;
-      if (lane == leader) {
-        start = atomicAdd(a.count, (unsigned long long)__popcll(bal));
-        atomic_add(a.prim_sum, pk);
-      }
-      start = __shfl(start, leader);
-      if (pass) {
-        const unsigned long long below = bal & ((1ull << lane) - 1ull);
-        const unsigned long long slot = start + __popcll(below);
-        if ((long long)slot < a.cap) {
-          Quartet qv;
-          qv.bra = (uint32_t)bi;
-          qv.ket_mask = (uint32_t)kj | (mask << 24);
-          a.out[slot] = qv;
-        }
-      }
-    }
+    // precision split on the Schwarz product (the FP32 list when below cut32)
+    const bool lo32 = pass && a.cut32 > 0.0 && a.bra[bi].Q * a.ket[kj].Q < a.cut32;
+    append_wave(pass && !lo32, bi, kj, mask, a.bra, a.ket, a.out, a.count, a.prim_sum, a.cap, lane);
+    if (a.cut32 > 0.0)
+      append_wave(lo32, bi, kj, mask, a.bra, a.ket, a.out32, a.count32, a.prim_sum32, a.cap, lane);
   }
 }
 
@@ -262,8 +290,9 @@ __global__ void block_norm_kernel(const double *X, int nmat, int cplx, int nb, i
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long tot = (long long)nmat * ns * ns;
   if (t >= tot) return;
-  const int m = (int)(t / ((long long)ns * ns));
-  const int rem = (int)(t % ((long long)ns * ns));
+  // Bm[(s1 + s2 NS) nmat + m]
+  const int m = (int)(t % nmat);
+  const int rem = (int)(t / nmat);
   const int s1 = rem % ns, s2 = rem / ns;
   const size_t nb2 = (size_t)nb * nb;
   const double *Xm = X + (size_t)m * nb2 * (cplx ? 2 : 1);
@@ -276,35 +305,32 @@ __global__ void block_norm_kernel(const double *X, int nmat, int cplx, int nb, i
     }
     mx = fmax(mx, row);
   }
-  Bm[(size_t)m * ns * ns + s1 + (size_t)s2 * ns] = mx;
+  Bm[t] = mx;
 }
 
 __global__ void block_norm_finish_kernel(int nmat, int ns, const int *her, double *Bm, double *Bg) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)ns * ns) return;
   const int k = (int)(t % ns), l = (int)(t / ns);
+  // per-matrix norms are already symmetrised (block_norm_symm_kernel)
   double g = 0.0;
-  for (int m = 0; m < nmat; ++m) {
-    double *B = Bm + (size_t)m * ns * ns;
-    double v = B[k + (size_t)l * ns];
-    if (!her[m]) v = fmax(v, B[l + (size_t)k * ns]);  // symmetrised, direct.hpp:1537-1545
-    g = fmax(g, v);
-  }
+  for (int m = 0; m < nmat; ++m) g = fmax(g, Bm[(k + (size_t)l * ns) * nmat + m]);
   Bg[k + (size_t)l * ns] = g;
 }
 
 __global__ void block_norm_symm_kernel(int nmat, int ns, const int *her, double *Bm) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)nmat * ns * ns) return;
-  const int m = (int)(t / ((long long)ns * ns));
+  const int m = (int)(t % nmat);
   if (her[m]) return;
-  const int rem = (int)(t % ((long long)ns * ns));
+  const int rem = (int)(t / nmat);
   const int k = rem % ns, l = rem / ns;
   if (k <= l) return;
-  double *B = Bm + (size_t)m * ns * ns;
-  const double mx = fmax(B[k + (size_t)l * ns], B[l + (size_t)k * ns]);
-  B[k + (size_t)l * ns] = mx;
-  B[l + (size_t)k * ns] = mx;
+  // symmetrised for non-Hermitian X, direct.hpp:1537-1545
+  double *B1 = Bm + (k + (size_t)l * ns) * nmat + m, *B2 = Bm + (l + (size_t)k * ns) * nmat + m;
+  const double mx = fmax(*B1, *B2);
+  *B1 = mx;
+  *B2 = mx;
 }
 
 __device__ __forceinline__ int sph_off(int l) {

@@ -16,6 +16,8 @@ struct EriLaunch {
   const PairRec *bra, *ket;
   const PrimPair *prims;
   const double *rys;
+  const float *rys32;            // the same table rounded to FP32
+  int fp32;                      // evaluate the quartets in FP32 (digestion stays FP64)
   PlaneSet planes;
   double *store;                 // STORE mode: contracted Cartesian quartets
   const long long *store_off;    // per quartet offset into store
@@ -39,7 +41,7 @@ struct ListLaunch {
   int nbra;
   long long ncand;
   const double *Bg;           // NS x NS global block norms
-  const double *Bm;           // nmat x NS x NS per-matrix norms
+  const double *Bm;           // per-matrix norms, Bm[(s1 + s2 NS) nmat + m]
   int nmat, ns;
   double thresh;
   int screen;
@@ -47,6 +49,12 @@ struct ListLaunch {
   unsigned long long *count;
   double *prim_sum;           // sum over survivors of Ka Kb Kc Kd
   long long cap;
+  // survivors with Schwarz product Q_ab Q_cd < cut32 go to the FP32 list
+  // (out32 / count32 / prim_sum32) instead; cut32 = 0 keeps one list
+  double cut32;
+  Quartet *out32;
+  unsigned long long *count32;
+  double *prim_sum32;
 };
 hipError_t launch_quartet_list(const ListLaunch &a, hipStream_t s);
 

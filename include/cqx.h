@@ -35,7 +35,8 @@ enum cqx_status {
   CQX_ERR_HIP = 2,     /* HIP runtime error */
   CQX_ERR_MEM = 3,     /* device or host allocation failed */
   CQX_ERR_BASIS = 4,   /* basis-set parse or lookup failure */
-  CQX_ERR_UNSUPPORTED = 5
+  CQX_ERR_UNSUPPORTED = 5,
+  CQX_ERR_NOT_CONVERGED = 6 /* SCF did not converge within max_iter */
 };
 
 enum cqx_contraction_type { /* TWOBODY_CONTRACTION_TYPE, twopints.hpp:31-41 */
@@ -80,6 +81,16 @@ int cqx_basis_shell(const cqx_basis *b, int s, int *l, int *pure,
 /* normalised exponents/coefficients of shell s and its origin */
 int cqx_basis_shell_prims(const cqx_basis *b, int s, double *alpha,
                           double *coef, double *origin);
+
+/* One-electron integrals of a basis into out (column-major NB x NB, pure
+ * functions where the shell is pure); host computation, no device needed.
+ * Stands in for OnePInts::computeAOInts with libint2's overlap, kinetic
+ * and point-nucleus operators (src/particleintegrals/aointegrals/
+ * aointegrals_onee_drivers.cxx:762-784).  For CQX_POTENTIAL the operator
+ * is sum_C -Z[C] / |r - C| over ncharge point charges Z at xyz (bohr). */
+enum { CQX_OVERLAP = 0, CQX_KINETIC = 1, CQX_POTENTIAL = 2 };
+int cqx_one_body(const cqx_basis *b, int kind, int ncharge, const double *Z,
+                 const double *xyz, double *out);
 
 /* ---------------------------------------------------------------------
  * Context: DirectTPI<double>(basis, basis, mol, threshSchwarz) plus the
@@ -136,6 +147,8 @@ typedef struct {
   double ms_total;        /* wall time of the call */
   double ms_eri;          /* device time of the ERI/digestion kernels */
   int n_launch_eri;       /* ERI kernel launches */
+  long long n_quartets_f32; /* of n_quartets, those evaluated in FP32 */
+  double flops_model_f32;   /* of flops_model, the part on FP32 quartets */
 } cqx_stats;
 int cqx_last_stats(const cqx_ctx *ctx, cqx_stats *st);
 
@@ -143,7 +156,10 @@ int cqx_last_stats(const cqx_ctx *ctx, cqx_stats *st);
  * 1e-18; rebuilds the pair data), "fast_max_nc" (largest Cartesian quartet
  * size routed to the lane-group kernels, default 27), "use_wave" (0 routes
  * the larger s/p/d classes to the generic kernel instead of the templated
- * wave-per-quartet ones). */
+ * wave-per-quartet ones), "fp32_cut" (mixed precision: quartets whose
+ * Schwarz product Q_ab Q_cd is below this value are evaluated in FP32 with
+ * an FP32 Rys table, digested and accumulated in FP64; default 0 = every
+ * quartet in FP64; classes with an f shell always stay FP64). */
 int cqx_set_option(cqx_ctx *ctx, const char *name, double value);
 
 /* ---------------------------------------------------------------------
@@ -158,6 +174,64 @@ int cqx_device_free(cqx_ctx *ctx, void *ptr);
 int cqx_memcpy(cqx_ctx *ctx, void *dst, const void *src, size_t bytes, int kind);
 /* zero device memory (fill kernel on the context stream) and wait */
 int cqx_device_zero(cqx_ctx *ctx, void *ptr, size_t bytes);
+
+/* ---------------------------------------------------------------------
+ * Device-resident RHF SCF on a context: the SingleSlater / conventional
+ * SCF loop of the reference for a closed-shell real reference
+ * (include/orbitalmodifiernew/orbitaloptimizernew/impl.hpp:40-137 and
+ * :266-330; Lowdin orthogonalisation include/orthogonalization/impl.hpp:
+ * 40-80; CDIIS include/orbitalmodifiernew/conventionalSCFnew/extrap.hpp:
+ * 36-123; formDensity / computeEnergy include/singleslater/quantum.hpp:
+ * 44-212).  Pauli-scalar conventions: D = D_S = 2 C_occ C_occ^T,
+ * H = 2 (T + V), G = 2 J[D] - K[D], F = H + G, E = 1/2 Tr(D H) +
+ * 1/4 Tr(D G) + V_NN.  Fock builds run through cqx_two_body_contract on the
+ * context's GPU; orthogonalisation, DIIS, eigensolve (rocSOLVER syevd) and
+ * density formation (rocBLAS GEMM on the matrix cores) stay on the device.
+ * ------------------------------------------------------------------- */
+enum cqx_guess { CQX_GUESS_CORE = 0, CQX_GUESS_SAD = 1 };
+
+typedef struct {
+  int nocc;          /* doubly occupied orbitals */
+  int max_iter;      /* default 128 */
+  double rmsd_tol;   /* ||D - D_prev||_2 / NB, default 1e-8 */
+  double maxd_tol;   /* max |D - D_prev|, default 1e-5 (SCFControls) */
+  double ene_tol;    /* |E - E_prev|, default 1e-5 (SCFControls) */
+  int diis_keep;     /* CDIIS subspace size, default 8 */
+  int guess;         /* cqx_guess, default CQX_GUESS_SAD */
+  double fp32_cut;   /* mixed-precision Fock builds (see "fp32_cut"); 0 = FP64 */
+} cqx_scf_opts;
+/* fill the defaults above (nocc = 0: the caller sets it) */
+void cqx_scf_default_opts(cqx_scf_opts *o);
+
+typedef struct {
+  double e_total, e_one, e_two, e_nuc;
+  int iterations;    /* Fock builds done */
+  int converged;
+  double rmsd, maxd, delta_e;
+  double ms_setup;   /* one-electron integrals, orthogonaliser, guess */
+  double ms_fock;    /* Fock builds */
+  double ms_total;
+} cqx_scf_result;
+
+typedef struct cqx_scf cqx_scf;
+/* natom nuclei Z (charges) at xyz (bohr); the context's basis must sit on
+ * them.  Computes S, T, V, the Lowdin orthogonaliser and the guess. */
+int cqx_scf_create(cqx_ctx *ctx, int natom, const double *Z, const double *xyz,
+                   const cqx_scf_opts *opts, cqx_scf **out);
+/* One SCF iteration (Fock build, energy, convergence test, CDIIS,
+ * eigensolve, new density); *done = 1 once converged. */
+int cqx_scf_iterate(cqx_scf *s, int *done);
+/* Iterate to convergence; CQX_ERR_NOT_CONVERGED after max_iter. */
+int cqx_scf_run(cqx_scf *s, cqx_scf_result *res);
+int cqx_scf_result_get(const cqx_scf *s, cqx_scf_result *res);
+/* host copies (column-major NB x NB) of the density D_S, the Fock matrix
+ * F_S built from it, the core Hamiltonian H_S and the overlap S */
+int cqx_scf_matrix(const cqx_scf *s, int which, double *out);
+enum { CQX_SCF_DENSITY = 0, CQX_SCF_FOCK = 1, CQX_SCF_HCORE = 2, CQX_SCF_OVERLAP = 3 };
+/* orbital energies (eigenvalues of F_S / 2) of the last diagonalisation */
+int cqx_scf_orbital_energies(const cqx_scf *s, double *eps);
+int cqx_scf_destroy(cqx_scf *s);
+const char *cqx_scf_last_error(const cqx_scf *s);
 
 #ifdef __cplusplus
 }

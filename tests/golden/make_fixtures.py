@@ -13,7 +13,8 @@ Sources (all data written by ChronusQ's DIRECT J/K path):
 * ``tests/scf/reference/*.bin.ref`` -- SCF restart files holding
   ``/SCF/FOCK_*``, ``/INTS/CORE_HAMILTONIAN_*`` and ``/SCF/1PDM_*``, so that
   G[D] = F - H is ChronusQ's own ``FockBuilder::formGD`` output
-  (``include/fockbuilder/impl.hpp:75-244``).  The molecule and basis of each
+  (``include/fockbuilder/impl.hpp:75-244``); also ``/INTS/KINETIC``,
+  ``/INTS/POTENTIAL`` and the Lowdin ``/SCF/ORTHO`` where the file has them.  The molecule and basis of each
   fixture are read from the input deck that produced it.
 
 Matrices are stored as mathematical matrices ``M[p, q]``.  ChronusQ writes
@@ -80,7 +81,8 @@ def parse_deck(path):
                 t = lines[i].split()
                 if len(t) != 4 or lines[i].strip().startswith("["):
                     break
-                geom.append([t[0], float(t[1]), float(t[2]), float(t[3])])
+                # isotope labels (Ne-22) name the element for the electrons
+                geom.append([t[0].split("-")[0], float(t[1]), float(t[2]), float(t[3])])
                 i += 1
             continue
         if section == "QM" and low.startswith("reference="):
@@ -116,6 +118,8 @@ SCF = {
         "B_swap_GHF_sto-3g.bin.ref", "scf/serial/ghf/B_swap_GHF_sto-3g.inp"),
     "kcakr_sto3g_ghf": (
         "KCaKr_sto-3G.bin.ref", "scf/serial/ghf/KCaKr_sto-3G.inp"),
+    "nehe_sto3g_rhf": (
+        "NeHe_rhf_sto-3g_iso.bin.ref", "scf/serial/rhf/NeHe_rhf_sto-3g_iso.inp"),
 }
 
 COMPS = ["SCALAR", "MZ", "MY", "MX"]
@@ -153,6 +157,11 @@ def main():
                 if ds in have:
                     arrs[key + c] = colmajor(h5_dataset(path, ds))
         arrs["OVERLAP"] = colmajor(h5_dataset(path, "/INTS/OVERLAP"))
+        # one-electron pieces and the Lowdin transformation, where written
+        for ds, key in (("/INTS/KINETIC", "KINETIC"), ("/INTS/POTENTIAL", "POTENTIAL"),
+                        ("/SCF/ORTHO", "ORTHO")):
+            if ds in have:
+                arrs[key] = colmajor(h5_dataset(path, ds))
         for e in ("TOTAL_ENERGY", "MANY_BODY_ENERGY", "ONE_BODY_ENERGY"):
             arrs[e] = h5_dataset(path, "/SCF/" + e).reshape(-1)
         np.savez_compressed(os.path.join(HERE, f"scf_{name}.npz"), **arrs)

@@ -13,7 +13,7 @@ from oracle.oracle import OracleBasis
 
 def declared_symbols():
     text = open(os.path.join(ROOT, "include", "cqx.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(cqx_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*(cqx_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_header():
@@ -51,7 +51,8 @@ def test_basis_errors_raise():
 
 def test_every_spd_class_has_a_specialised_kernel():
     """Every (la lb|lc ld) class with l <= 2 (bra class >= ket class) is
-    routed to a lane-group or wave kernel, not to the generic fallback."""
+    routed to a lane-group or wave kernel, not to the generic fallback, in
+    FP64 and in FP32 (mixed precision)."""
     import ctypes as C
     from chronusq_public_amd import cqx
     L = cqx.lib()
@@ -65,6 +66,8 @@ def test_every_spd_class_has_a_specialised_kernel():
     for bi, b in enumerate(pairs):
         for k in pairs[:bi + 1]:
             cls = f"{b[0]}, {b[1]}, {k[0]}, {k[1]}"
-            if f"eri_wave_kernel<{cls}>" not in syms and f"eri_fast_kernel<{cls}, 16>" not in syms:
-                missing.append(cls)
+            for r in ("double", "float"):
+                if (f"eri_wave_kernel<{cls}, {r}>" not in syms
+                        and f"eri_fast_kernel<{cls}, 16, {r}>" not in syms):
+                    missing.append((cls, r))
     assert not missing, missing

@@ -1,0 +1,108 @@
+"""Mixed precision on the GPU: quartets whose Schwarz product Q_ab Q_cd is
+below a cut are evaluated in FP32 (FP32 Rys table, FP32 2D integrals and
+contraction); their digestion and every accumulator stay FP64.
+
+Bars (written per test): the mixed J/K of an RHF build against the FP64
+oracle within max|dG| <= 1e-7 and |dE| <= 1e-6 Eh, E2 = 1/4 Tr(D G) with
+G = 2J - K (quantum.hpp:147-212); the all-FP32 evaluation within 1e-5 of
+the largest element (FP32 rounding, not a wrong term)."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle.oracle import COULOMB, EXCHANGE, OracleBasis
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True, file=sys.stderr)
+
+
+def _engine(mol, gbs):
+    from chronusq_public_amd import cqx
+    b = cqx.BasisSet(gbs, mol)
+    tpi = cqx.DirectTPI(b, 1e-12)
+    return b, tpi, cqx.GTODirectTPIContraction(tpi)
+
+
+def _contract(gto, Xs, types, hers):
+    from chronusq_public_amd import cqx
+    cplx = any(np.iscomplexobj(x) for x in Xs)
+    dt = np.complex128 if cplx else np.float64
+    AX = [np.zeros(x.shape, dtype=dt, order="F") for x in Xs]
+    gto.twoBodyContract([cqx.TwoBodyContraction(np.asarray(x, dtype=dt), a, h, t)
+                         for x, a, t, h in zip(Xs, AX, types, hers)])
+    return AX
+
+
+def test_mixed_alkane_rhf_within_microhartree():
+    """C8H18 / 6-31G*, RHF density D = C C^T (C: NB x 33, N(0, 1/sqrt NB)),
+    cut 1e-4: energy and Fock against the FP64 oracle; most quartets in FP32."""
+    from chronusq_public_amd.chem import alkane_chain, basis_text
+    mol, gbs = alkane_chain(8, 0), basis_text("6-31G*")
+    say("case: mixed C8H18 6-31G*")
+    b, tpi, gto = _engine(mol, gbs)
+    nb = b.nBasis
+    rng = np.random.Generator(np.random.PCG64(0))
+    C_ = rng.normal(0, 1 / np.sqrt(nb), size=(nb, 33))
+    D = C_ @ C_.T
+    D = 0.5 * (D + D.T)
+    Xs, types, hers = [D, D], [COULOMB, EXCHANGE], [True, True]
+    ref = OracleBasis(gbs, mol.Z, mol.xyz).contract(Xs, types, hers)
+    f64 = _contract(gto, Xs, types, hers)
+    n64 = tpi.last_stats()["n_quartets"]
+    tpi.set_option("fp32_cut", 1e-4)
+    mix = _contract(gto, Xs, types, hers)
+    st = tpi.last_stats()
+    assert st["n_quartets"] == n64
+    assert st["n_quartets_f32"] > 0.5 * n64
+    G_ref = 2 * ref[0] - ref[1]
+    G_mix = 2 * mix[0] - mix[1]
+    dF = np.abs(G_mix - G_ref).max()
+    dE = abs(0.25 * np.sum(D * (G_mix - G_ref)))
+    say(f"  fp32 share {st['n_quartets_f32'] / n64:.3f}  max|dG| {dF:.2e}  |dE| {dE:.2e}")
+    assert dF <= 1e-7
+    assert dE <= 1e-6
+    for g, r in zip(f64, ref):
+        assert np.abs(g - r).max() < 1e-10
+    # the cut back at 0 returns the FP64 path
+    tpi.set_option("fp32_cut", 0.0)
+    back = _contract(gto, Xs, types, hers)
+    assert tpi.last_stats()["n_quartets_f32"] == 0
+    for g, r in zip(back, ref):
+        assert np.abs(g - r).max() < 1e-10
+
+
+def test_all_fp32_every_kernel_and_plane_kind():
+    """Every s/p/d quartet in FP32 (cut above every Schwarz product) on
+    (H2O)_2 / 6-31G(d), complex density, Hermitian and non-Hermitian J and
+    K, through the lane-group and the wave kernels: within 1e-5 of the
+    largest element of the FP64 result."""
+    from chronusq_public_amd.chem import basis_text, water_cluster
+    mol, gbs = water_cluster(2, 4.0, seed=5), basis_text("6-31G(d)")
+    say("case: all-FP32 water2 6-31G(d)")
+    b, tpi, gto = _engine(mol, gbs)
+    nb = b.nBasis
+    rng = np.random.Generator(np.random.PCG64(12))
+    C_ = rng.normal(size=(nb, 5)) + 1j * rng.normal(size=(nb, 5))
+    D = C_ @ C_.conj().T
+    X = rng.normal(size=(nb, nb)) + 1j * rng.normal(size=(nb, nb))
+    Xs = [D, D, X, X]
+    types, hers = [COULOMB, EXCHANGE, COULOMB, EXCHANGE], [True, True, False, False]
+    f64 = _contract(gto, Xs, types, hers)
+    for fast_max_nc in (27, 0):  # lane-group + wave kernels, then wave kernels only
+        tpi.set_option("fast_max_nc", fast_max_nc)
+        tpi.set_option("fp32_cut", 1e300)
+        f32 = _contract(gto, Xs, types, hers)
+        st = tpi.last_stats()
+        if fast_max_nc:
+            assert st["n_quartets_f32"] == st["n_quartets"]
+        else:  # the small classes fall to the generic kernel, which is FP64 only
+            assert 0 < st["n_quartets_f32"] < st["n_quartets"]
+        for g, r in zip(f32, f64):
+            err = np.abs(g - r).max() / np.abs(r).max()
+            say(f"  fast_max_nc {fast_max_nc} rel err {err:.2e}")
+            assert err < 1e-5
+        tpi.set_option("fp32_cut", 0.0)

@@ -1,0 +1,72 @@
+"""Device-resident RHF SCF against ChronusQ's converged SCF files: total,
+one-body and many-body energies, the density and the Fock matrix of
+tests/scf/reference/water_6-31Gd.bin.ref and NeHe_rhf_sto-3g_iso.bin.ref.
+
+Bars: the total energy within 1e-8 Eh (both SCFs are variational, so the
+energy error is second order in the density difference); the one- and
+many-body energies, density and Fock matrix within 1e-5 (ChronusQ stops
+at an RMS density change of 1e-7, SCFControls)."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import fixture_molecule, load_fixture
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True, file=sys.stderr)
+
+
+def _rhf(mol, gbs, **kw):
+    from chronusq_public_amd import cqx
+    b = cqx.BasisSet(gbs, mol)
+    tpi = cqx.DirectTPI(b, 1e-12)
+    return b, tpi, cqx.RHF(tpi, **kw)
+
+
+@pytest.mark.parametrize("name", ["water_631gd_rhf", "nehe_sto3g_rhf"])
+@pytest.mark.parametrize("guess", ["SAD", "CORE"])
+def test_scf_matches_chronusq(name, guess):
+    from chronusq_public_amd import cqx
+    meta, d = load_fixture("scf_" + name)
+    mol, gbs = fixture_molecule(meta)
+    say(f"case: SCF {name} guess {guess}")
+    b, tpi, scf = _rhf(mol, gbs, guess=guess)
+    S = scf.matrix(cqx.SCF_OVERLAP)
+    D0 = scf.matrix(cqx.SCF_DENSITY)
+    # the guess holds every electron: Tr(D S) = N
+    assert abs(np.sum(D0 * S) - mol.nelectron()) < 1e-8
+    r = scf.SCF()
+    say(f"  E {r['e_total']:.12f}  ref {d['TOTAL_ENERGY'][0]:.12f}  iterations {r['iterations']}")
+    assert r["converged"]
+    assert abs(r["e_total"] - d["TOTAL_ENERGY"][0]) < 1e-8
+    assert abs(r["e_one"] - d["ONE_BODY_ENERGY"][0]) < 1e-5
+    assert abs(r["e_two"] - d["MANY_BODY_ENERGY"][0]) < 1e-5
+    assert np.abs(scf.matrix(cqx.SCF_DENSITY) - d["PDM_SCALAR"]).max() < 1e-5
+    assert np.abs(scf.matrix(cqx.SCF_FOCK) - d["FOCK_SCALAR"]).max() < 1e-5
+    assert np.abs(scf.matrix(cqx.SCF_HCORE) - d["HCORE_SCALAR"]).max() < 1e-12 * np.abs(d["HCORE_SCALAR"]).max()
+
+
+def test_scf_energy_is_rotation_invariant_and_mixed_within_microhartree():
+    """(H2O)_4 / cc-pVDZ: the converged energy does not change under a rigid
+    rotation of the cluster (1e-8), and the mixed-precision SCF (quartets
+    with Schwarz product < 1e-4 in FP32) lands within 1 uEh of FP64."""
+    from chronusq_public_amd.chem import Molecule, basis_text, water_cluster
+    mol = water_cluster(4, 8.0, seed=0)
+    gbs = basis_text("cc-pVDZ")
+    say("case: SCF water4 rotation + mixed")
+    _, _, scf = _rhf(mol, gbs)
+    e64 = scf.SCF()["e_total"]
+    th = 0.7
+    R = np.array([[np.cos(th), -np.sin(th), 0.0], [np.sin(th), np.cos(th), 0.0], [0.0, 0.0, 1.0]])
+    rot = Molecule(mol.Z, mol.xyz @ R.T + np.array([1.0, -2.0, 0.5]))
+    _, _, scf_r = _rhf(rot, gbs)
+    er = scf_r.SCF()["e_total"]
+    _, _, scf_m = _rhf(mol, gbs, fp32_cut=1e-4)
+    em = scf_m.SCF()["e_total"]
+    say(f"  E {e64:.12f} rotated {er:.12f} mixed {em:.12f}")
+    assert abs(er - e64) < 1e-8
+    assert abs(em - e64) < 1e-6

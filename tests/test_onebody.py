@@ -1,0 +1,55 @@
+"""One-electron integrals (host C++ in libcqx) against ChronusQ's own
+/INTS/OVERLAP, /INTS/KINETIC and /INTS/POTENTIAL (libint2, point nuclei),
+and the non-relativistic core Hamiltonian in Pauli form, H_S = 2 (T + V)
+(/INTS/CORE_HAMILTONIAN_SCALAR).  CPU only: no device is touched."""
+import numpy as np
+import pytest
+
+from conftest import SCF_FIXTURES, fixture_molecule, load_fixture
+
+
+@pytest.mark.parametrize("name", SCF_FIXTURES)
+def test_overlap_kinetic_potential_match_chronusq(name):
+    from chronusq_public_amd import cqx
+    meta, d = load_fixture("scf_" + name)
+    mol, gbs = fixture_molecule(meta)
+    b = cqx.BasisSet(gbs, mol)
+    S, T = b.one_body(cqx.OVERLAP), b.one_body(cqx.KINETIC)
+    scale = lambda M: max(1.0, float(np.abs(M).max()))
+    assert np.abs(S - d["OVERLAP"]).max() <= 1e-13
+    assert np.abs(T - d["KINETIC"]).max() <= 1e-14 * scale(d["KINETIC"])
+    if "POTENTIAL" in d:  # the X2C file stores no point-nucleus potential
+        V = b.one_body(cqx.POTENTIAL)
+        assert np.abs(V - d["POTENTIAL"]).max() <= 1e-14 * scale(d["POTENTIAL"])
+        if meta["reference"].upper() not in ("X2CHF",):
+            assert np.abs(2 * (T + V) - d["HCORE_SCALAR"]).max() <= 1e-14 * scale(d["HCORE_SCALAR"])
+
+
+def test_potential_closed_forms():
+    """Nuclear attraction of one normalised s Gaussian: <s|-1/r_C|s> about
+    its own centre is -2 sqrt(2 a / pi); about a distant charge it tends to
+    -1/R (the Gaussian looks like a point charge); linear in the charge."""
+    from chronusq_public_amd import cqx
+    from chronusq_public_amd.chem import Molecule
+    a = 0.7
+    g94 = "****\nH 0\nS 1 1.00\n%.10f 1.0\n****\n" % a
+    b = cqx.BasisSet(g94, Molecule([1], [[0.0, 0.0, 0.0]]))
+    v0 = b.one_body(cqx.POTENTIAL, Z=[1.0], xyz=[0.0, 0.0, 0.0])[0, 0]
+    assert abs(v0 - (-2.0 * np.sqrt(2 * a / np.pi))) < 1e-14
+    R = 40.0
+    vR = b.one_body(cqx.POTENTIAL, Z=[1.0], xyz=[0.0, 0.0, R])[0, 0]
+    assert abs(vR + 1.0 / R) < 1e-14
+    v3 = b.one_body(cqx.POTENTIAL, Z=[3.0, -1.0], xyz=[0.0, 0.0, 1.0, 0.5, 0.0, 0.0])[0, 0]
+    v1 = b.one_body(cqx.POTENTIAL, Z=[1.0], xyz=[0.0, 0.0, 1.0])[0, 0]
+    v2 = b.one_body(cqx.POTENTIAL, Z=[1.0], xyz=[0.5, 0.0, 0.0])[0, 0]
+    assert abs(v3 - (3 * v1 - v2)) < 1e-14
+    # kinetic energy of a normalised s Gaussian: 3a/2
+    assert abs(b.one_body(cqx.KINETIC)[0, 0] - 1.5 * a) < 1e-14
+
+
+def test_one_body_rejects_bad_arguments():
+    from chronusq_public_amd import cqx
+    from chronusq_public_amd.chem import Molecule
+    b = cqx.BasisSet("STO-3G", Molecule([1], [[0.0, 0.0, 0.0]]))
+    with pytest.raises(cqx.CErr):
+        cqx._check(cqx.lib().cqx_one_body(b._h, 7, 0, None, None, cqx._ptr(np.zeros(1))))
