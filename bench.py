@@ -16,7 +16,9 @@ their device time from HIP events on the launch stream.  CPU baseline: the
 FP64 oracle (a C/OpenMP restatement of the reference scaffold) timed on one
 full build of the same workload on the host cores.
 
-The "mixed" object of the line is the mixed-precision comparison on the
+The "serving" object reports SCF p50/p99 latency and jobs/s of the serving
+loop (see serving()).  The "mixed" object of the line is the
+mixed-precision comparison on the
 linear alkane C40H82 (all-trans, C-C 1.53 A, C-H 1.09 A, seed 0 +-0.05 A
 jitter), 6-31G* (724 functions), RHF density D = C C^T with C 724 x 161,
 N(0, 1/sqrt(724)): builds/s of the FP64 build and of the mixed build
@@ -129,12 +131,18 @@ class Build:
         return [a.download() for a in self.dA]
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def mixed_comparison(steps, warmup, cut=1e-4):
     """FP64 and mixed builds of the alkane workload, timed alike; the mixed
     result is compared with the FP64 one (the FP64 path itself is held to
     1e-10 of the oracle by the GPU parity tests)."""
+    progress("mixed precision: C40H82 / 6-31G* FP64 builds")
     b = Build("alkane40", nocc=161)
     dt64, eri64, fl64, _ = b.timed(steps, warmup)
+    progress("mixed precision: mixed builds")
     J, K = b.outputs()
     st64 = b.tpi.last_stats()
     b.tpi.set_option("fp32_cut", cut)
@@ -164,6 +172,45 @@ def mixed_comparison(steps, warmup, cut=1e-4):
     }
 
 
+def serving(njobs, max_inflight=8, cut=1e-4, seed=0):
+    """SCF serving on water clusters (H2O)_n, n ~ U{4..48} from seed, in a
+    cube of side 12 (n/16)^(1/3) A (the density of the GHF workload),
+    cc-pVDZ, SAD guess, CDIIS(8), converged at RMS dD < 1e-8.  A closed-loop
+    FP64 run (every job at t = 0) gives the peak throughput and the FP64
+    energies; the measured run has Poisson arrivals at 80% of that rate and
+    mixed-precision Fock builds (FP32 when Q_ab Q_cd < cut)."""
+    from chronusq_public_amd import cqx
+    from chronusq_public_amd.chem import water_cluster
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = [int(n) for n in rng.integers(4, 49, size=njobs)]
+    mols = [water_cluster(n, 12.0 * (n / 16.0) ** (1.0 / 3.0), seed=1000 + i) for i, n in enumerate(sizes)]
+    progress(f"serving: closed-loop FP64 run of {njobs} jobs, sizes {sizes}")
+    ref = cqx.serve([("cc-pVDZ", m, 0.0) for m in mols], max_inflight=max_inflight, verbose=1)
+    peak = njobs / max(r["t_done"] for r in ref)
+    lam = 0.8 * peak
+    arrivals = np.concatenate([[0.0], np.cumsum(rng.exponential(1.0 / lam, size=njobs - 1))])
+    progress(f"serving: Poisson run at {lam:.3f} jobs/s, mixed precision")
+    res = cqx.serve([("cc-pVDZ", m, float(a)) for m, a in zip(mols, arrivals)],
+                    max_inflight=max_inflight, fp32_cut=cut, verbose=1)
+    lat = np.array([r["t_done"] - r["t_arrival"] for r in res])
+    dE = np.array([abs(r["e_total"] - q["e_total"]) for r, q in zip(res, ref)])
+    return {
+        "workload": ("RHF SCF jobs, (H2O)_n n ~ U{4..48} (seed %d), cc-pVDZ (NB 96-1152), SAD guess, "
+                     "CDIIS(8), RMS dD < 1e-8; Poisson arrivals at 0.8 x closed-loop peak; "
+                     "mixed precision FP32 when Q_ab Q_cd < %g; %d slots" % (seed, cut, max_inflight)),
+        "jobs": njobs,
+        "sizes": sizes,
+        "peak_jobs_per_s_fp64_closed_loop": peak,
+        "arrival_rate_per_s": lam,
+        "jobs_per_s": njobs / max(r["t_done"] for r in res),
+        "p50_latency_s": float(np.percentile(lat, 50)),
+        "p99_latency_s": float(np.percentile(lat, 99)),
+        "max_abs_dE_vs_fp64": float(dE.max()),
+        "iterations_mean": float(np.mean([r["iterations"] for r in res])),
+        "all_converged": all(r["converged"] for r in res) and all(r["converged"] for r in ref),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +218,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="ghf16")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=24,
+                    help="SCF jobs of the serving measurement (0 skips it)")
     ap.add_argument("--mixed-steps", type=int, default=3,
                     help="timed builds per precision of the mixed-precision comparison (0 skips it)")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -196,6 +245,7 @@ def main():
         cqx._check(cqx.lib().cqx_memcpy(tpi._h, dst, src, nbytes, 2), tpi._h)
         cqx.lib().cqx_device_free(tpi._h, src)
         cqx.lib().cqx_device_free(tpi._h, dst)
+    progress(f"{args.workload}: {args.warmup} warmup + {args.steps} timed builds")
     dt, eri_avg, flops_avg, _ = bench.timed(args.steps, args.warmup)
     st = tpi.last_stats()
     ms_step = 1e3 * dt / args.steps
@@ -240,7 +290,10 @@ def main():
     }
     if args.mixed_steps > 0:
         line["mixed"] = mixed_comparison(args.mixed_steps, 1)
+    if args.serve_jobs > 0:
+        line["serving"] = serving(args.serve_jobs)
     if not args.no_cpu:
+        progress("cpu baseline: one oracle build")
         cb, err = cpu_baseline(gbs, mol, mats, types, cplx, out)
         line["cpu_baseline"] = cb
         line["max_abs_dJK_vs_oracle"] = err

@@ -54,7 +54,7 @@ SYMBOLS = ["cqx_basis_create", "cqx_basis_from_g94", "cqx_basis_destroy",
            "cqx_memcpy", "cqx_device_zero", "cqx_one_body", "cqx_scf_default_opts",
            "cqx_scf_create", "cqx_scf_iterate", "cqx_scf_run", "cqx_scf_result_get",
            "cqx_scf_matrix", "cqx_scf_orbital_energies", "cqx_scf_destroy",
-           "cqx_scf_last_error"]
+           "cqx_scf_last_error", "cqx_serve_default_opts", "cqx_serve"]
 
 OVERLAP, KINETIC, POTENTIAL = 0, 1, 2
 
@@ -423,3 +423,58 @@ class RHF:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.cqx_scf_destroy(self._h)
             self._h = None
+
+
+class _Job(C.Structure):
+    _fields_ = [("g94", C.c_char_p), ("natom", C.c_int), ("Z", C.c_void_p), ("xyz", C.c_void_p),
+                ("nocc", C.c_int), ("arrival_s", C.c_double)]
+
+
+class ServeOpts(C.Structure):
+    _fields_ = [("max_inflight", C.c_int), ("hbm_budget_bytes", C.c_double), ("device", C.c_int),
+                ("thresh", C.c_double), ("list_cap", C.c_longlong), ("scf", ScfOpts),
+                ("verbose", C.c_int)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("converged", C.c_int), ("iterations", C.c_int),
+                ("e_total", C.c_double), ("t_arrival", C.c_double), ("t_admit", C.c_double),
+                ("t_done", C.c_double), ("footprint_bytes", C.c_double), ("ms_fock", C.c_double)]
+
+
+def serve(jobs, max_inflight=8, hbm_budget_bytes=200e9, fp32_cut=0.0, guess="SAD",
+          list_cap=1 << 25, device=0, rmsd_tol=1e-8, verbose=0):
+    """Run SCF jobs [(basis name or G94 text, Molecule, arrival seconds)]
+    through the serving loop; returns one dict per job (latency =
+    t_done - t_arrival)."""
+    L = _scf_lib()
+    if not getattr(L, "_serve_typed", False):
+        L.cqx_serve_default_opts.argtypes = [C.POINTER(ServeOpts)]
+        L.cqx_serve_default_opts.restype = None
+        L.cqx_serve.argtypes = [C.c_void_p, C.c_int, C.POINTER(ServeOpts), C.c_void_p]
+        L._serve_typed = True
+    o = ServeOpts()
+    L.cqx_serve_default_opts(C.byref(o))
+    o.max_inflight, o.hbm_budget_bytes, o.device, o.list_cap = int(max_inflight), float(hbm_budget_bytes), int(device), int(list_cap)
+    o.verbose = int(verbose)
+    o.scf.fp32_cut, o.scf.guess, o.scf.rmsd_tol = fp32_cut, GUESS[guess.upper()], rmsd_tol
+    arr = (_Job * len(jobs))()
+    keep = []
+    for i, (bas, mol, t) in enumerate(jobs):
+        text = (bas if "****" in bas else basis_text(bas)).encode()
+        Z = np.ascontiguousarray(mol.Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(mol.xyz, dtype=np.float64)
+        keep += [text, Z, xyz]
+        ne = mol.nelectron()
+        if ne % 2:
+            raise CErr("RHF serving jobs need an even electron count")
+        arr[i].g94, arr[i].natom, arr[i].Z, arr[i].xyz = text, len(Z), Z.ctypes.data, xyz.ctypes.data
+        arr[i].nocc, arr[i].arrival_s = ne // 2, float(t)
+    res = (JobResult * len(jobs))()
+    rc = L.cqx_serve(C.cast(arr, C.c_void_p), len(jobs), C.byref(o), C.cast(res, C.c_void_p))
+    out = [{k: getattr(r, k) for k, _ in JobResult._fields_} for r in res]
+    if rc:
+        bad = [i for i, r in enumerate(out) if r["status"]]
+        raise CErr(f"FATAL (cqx status {rc}): serving jobs {bad} failed")
+    return out
+

@@ -91,6 +91,7 @@ struct cqx_ctx {
   int fast_max_nc = 27;   // classes up to this Cartesian size use the lane-group kernels
   int use_wave = 1;       // templated wave-per-quartet kernels for the other s/p/d classes
   double fp32_cut = 0.0;  // quartets with Q_ab Q_cd below this are evaluated in FP32
+  long long list_cap = 1ll << 28;  // quartet-list entries per batch (8 B each, x2 when mixed)
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
@@ -552,6 +553,11 @@ int cqx_set_option(cqx_ctx *c, const char *name, double value) {
     c->use_wave = value != 0.0;
     return CQX_OK;
   }
+  if (!std::strcmp(name, "list_cap")) {
+    if (!(value >= 1.0)) return fail(c, CQX_ERR_ARG, "list_cap must be >= 1");
+    c->list_cap = (long long)value;
+    return CQX_OK;
+  }
   if (!std::strcmp(name, "fp32_cut")) {
     if (!(value >= 0.0)) return fail(c, CQX_ERR_ARG, "fp32_cut must be >= 0");
     c->fp32_cut = value;
@@ -690,9 +696,10 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
       double bmax = 0.0;
       for (double v : Bg) bmax = std::max(bmax, v);
       // candidate lists per class pair (bra class >= ket class)
-      struct CP { int cb, ck; long long seg, ncand; std::vector<long long> off; };
+      struct CP { int cb, ck; long long ncand; size_t coff; std::vector<long long> off; };
       std::vector<CP> cps;
       long long total = 0;
+      size_t cand_total = 0;
       for (int cb = 0; cb < kNumClass; ++cb)
         for (int ck = 0; ck <= cb; ++ck) {
           const auto &vb = c->cls[cb], &vk = c->cls[ck];
@@ -723,7 +730,8 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           }
           cp.ncand = cp.off.back();
           if (cp.ncand == 0) continue;
-          cp.seg = total;
+          cp.coff = cand_total;
+          cand_total += cp.off.size();
           total += cp.ncand;
           cps.push_back(std::move(cp));
         }
@@ -735,18 +743,35 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         if (c->use_wave && cqx::eri_wave_available(la, lb, lc, ld)) return 1;
         return 2;
       };
-      // mixed precision: a second list per class pair (FP32 evaluation) for
-      // the classes whose kernel family has an FP32 instance
       const bool mixed = c->fp32_cut > 0.0;
-      const size_t ncp = cps.size(), nslot = std::max<size_t>(2 * ncp, 1);
-      c->d_quart.ensure((size_t)std::max<long long>(total, 1) * (mixed ? 2 : 1));
-      c->d_count.ensure(nslot);
-      // counters cleared by a fill kernel (no memset nodes if the caller captures a graph)
-      HIPCHK(cqx::launch_zero(reinterpret_cast<double *>(c->d_count.p), nslot, s));
-      c->d_psum.ensure(nslot);
-      HIPCHK(cqx::launch_zero(c->d_psum.p, nslot, s));
-      size_t cand_total = 0;
-      for (auto &cp : cps) cand_total += cp.off.size();
+      // bra-range pieces of at most list_cap candidates, grouped into batches
+      // of at most list_cap candidates: the quartet lists of one batch are
+      // built, counted (one host synchronisation) and evaluated before the
+      // next, so the list workspace stays bounded for any molecule
+      struct Piece { int cp; int i0, i1; long long base, ncand, seg; };
+      std::vector<std::vector<Piece>> batches(1);
+      long long in_batch = 0, cap_need = 1;
+      for (int k = 0; k < (int)cps.size(); ++k) {
+        const auto &cp = cps[k];
+        const int nbra = (int)cp.off.size() - 1;
+        int i0 = 0;
+        while (i0 < nbra) {
+          int i1 = i0 + 1;  // at least one bra pair per piece
+          while (i1 < nbra && cp.off[i1 + 1] - cp.off[i0] <= c->list_cap) ++i1;
+          const long long n = cp.off[i1] - cp.off[i0];
+          if (n > 0) {
+            if (in_batch > 0 && in_batch + n > c->list_cap) {
+              batches.emplace_back();
+              in_batch = 0;
+            }
+            batches.back().push_back({k, i0, i1, cp.off[i0], n, in_batch});
+            in_batch += n;
+            cap_need = std::max(cap_need, in_batch);
+          }
+          i0 = i1;
+        }
+      }
+      c->d_quart.ensure((size_t)cap_need * (mixed ? 2 : 1));
       c->d_cand.ensure(std::max<size_t>(cand_total, 1));
       std::vector<long long> cand_all;
       cand_all.reserve(cand_total);
@@ -754,90 +779,102 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
       if (!cand_all.empty())
         HIPCHK(hipMemcpyAsync(c->d_cand.p, cand_all.data(), sizeof(long long) * cand_all.size(),
                               hipMemcpyHostToDevice, s));
-      size_t coff = 0;
-      for (size_t k = 0; k < cps.size(); ++k) {
-        auto &cp = cps[k];
-        cqx::ListLaunch LL{};
-        LL.bra = c->d_pairs[cp.cb].p;
-        LL.ket = c->d_pairs[cp.ck].p;
-        LL.same_class = cp.cb == cp.ck;
-        LL.cand_off = c->d_cand.p + coff;
-        LL.nbra = (int)c->cls[cp.cb].size();
-        LL.ncand = cp.ncand;
-        LL.Bg = c->d_Bg.p;
-        LL.Bm = c->d_Bm.p;
-        LL.nmat = nmat;
-        LL.ns = NS;
-        LL.thresh = c->thresh;
-        LL.screen = screen;
-        LL.out = c->d_quart.p + cp.seg;
-        LL.count = c->d_count.p + k;
-        LL.prim_sum = c->d_psum.p + k;
-        LL.cap = cp.ncand;
-        int lb_, la_, lc_, ld_;
-        class_ls(cp.cb, la_, lb_);
-        class_ls(cp.ck, lc_, ld_);
-        if (mixed && route(la_, lb_, lc_, ld_) < 2) {
-          LL.cut32 = c->fp32_cut;
-          LL.out32 = c->d_quart.p + total + cp.seg;
-          LL.count32 = c->d_count.p + ncp + k;
-          LL.prim_sum32 = c->d_psum.p + ncp + k;
+      for (const auto &batch : batches) {
+        const size_t np = batch.size();
+        if (!np) continue;
+        const size_t nslot = 2 * np;
+        c->d_count.ensure(nslot);
+        c->d_psum.ensure(nslot);
+        // counters cleared by a fill kernel (no memset nodes if the caller captures a graph)
+        HIPCHK(cqx::launch_zero(reinterpret_cast<double *>(c->d_count.p), nslot, s));
+        HIPCHK(cqx::launch_zero(c->d_psum.p, nslot, s));
+        for (size_t k = 0; k < np; ++k) {
+          const Piece &pc = batch[k];
+          const auto &cp = cps[pc.cp];
+          cqx::ListLaunch LL{};
+          LL.bra = c->d_pairs[cp.cb].p + pc.i0;
+          LL.ket = c->d_pairs[cp.ck].p;
+          LL.same_class = cp.cb == cp.ck;
+          LL.cand_off = c->d_cand.p + cp.coff + pc.i0;
+          LL.cand_base = pc.base;
+          LL.nbra = pc.i1 - pc.i0;
+          LL.ncand = pc.ncand;
+          LL.Bg = c->d_Bg.p;
+          LL.Bm = c->d_Bm.p;
+          LL.nmat = nmat;
+          LL.ns = NS;
+          LL.thresh = c->thresh;
+          LL.screen = screen;
+          LL.out = c->d_quart.p + pc.seg;
+          LL.count = c->d_count.p + k;
+          LL.prim_sum = c->d_psum.p + k;
+          LL.cap = pc.ncand;
+          int la_, lb_, lc_, ld_;
+          class_ls(cp.cb, la_, lb_);
+          class_ls(cp.ck, lc_, ld_);
+          if (mixed && route(la_, lb_, lc_, ld_) < 2) {
+            LL.cut32 = c->fp32_cut;
+            LL.out32 = c->d_quart.p + cap_need + pc.seg;
+            LL.count32 = c->d_count.p + np + k;
+            LL.prim_sum32 = c->d_psum.p + np + k;
+          }
+          HIPCHK(cqx::launch_quartet_list(LL, s));
         }
-        HIPCHK(cqx::launch_quartet_list(LL, s));
-        coff += cp.off.size();
-      }
-      std::vector<unsigned long long> counts(2 * ncp);
-      std::vector<double> psum(2 * ncp);
-      if (ncp) {
-        HIPCHK(hipMemcpyAsync(counts.data(), c->d_count.p, sizeof(unsigned long long) * 2 * ncp,
+        std::vector<unsigned long long> counts(nslot);
+        std::vector<double> psum(nslot);
+        HIPCHK(hipMemcpyAsync(counts.data(), c->d_count.p, sizeof(unsigned long long) * nslot,
                               hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(psum.data(), c->d_psum.p, sizeof(double) * 2 * ncp,
-                              hipMemcpyDeviceToHost, s));
-      }
-      HIPCHK(hipStreamSynchronize(s));
-      // ERI + digestion
-      HIPCHK(hipEventRecord(c->ev0, s));
-      for (size_t kk = 0; kk < 2 * ncp; ++kk) {
-        if (!counts[kk]) continue;
-        const size_t k = kk % ncp;
-        const bool f32 = kk >= ncp;
-        auto &cp = cps[k];
-        cqx::EriLaunch E{};
-        class_ls(cp.cb, E.la, E.lb);
-        class_ls(cp.ck, E.lc, E.ld);
-        E.quartets = c->d_quart.p + (f32 ? total : 0) + cp.seg;
-        E.nq = (long long)counts[kk];
-        E.bra = c->d_pairs[cp.cb].p;
-        E.ket = c->d_pairs[cp.ck].p;
-        E.prims = c->d_prims.p;
-        E.rys = c->d_rys.p;
-        E.rys32 = c->d_rys32.p;
-        E.fp32 = f32 ? 1 : 0;
-        E.planes = ps;
-        E.store_mode = 0;
-        const int r = route(E.la, E.lb, E.lc, E.ld);
-        hipError_t e;
-        if (r == 0)
-          e = cqx::launch_eri_fast(E, s);
-        else if (r == 1)
-          e = cqx::launch_eri_wave(E, s);
-        else if (!f32)
-          e = cqx::launch_eri_team(E, s);
-        else
-          throw std::runtime_error("FP32 quartet list for a class without an FP32 kernel");
-        HIPCHK(e);
-        const double fl = class_pair_flops(E.la, E.lb, E.lc, E.ld, pure_d, psum[kk], (double)counts[kk],
-                                           is_complex ? 2 : 1, nJ, nK);
-        c->stats.n_quartets += (long long)counts[kk];
-        c->stats.n_prim_quartets_max += (long long)psum[kk];
-        c->stats.flops_model += fl;
-        if (f32) {
-          c->stats.n_quartets_f32 += (long long)counts[kk];
-          c->stats.flops_model_f32 += fl;
+        HIPCHK(hipMemcpyAsync(psum.data(), c->d_psum.p, sizeof(double) * nslot, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        // ERI + digestion
+        HIPCHK(hipEventRecord(c->ev0, s));
+        for (size_t kk = 0; kk < nslot; ++kk) {
+          if (!counts[kk]) continue;
+          const size_t k = kk % np;
+          const bool f32 = kk >= np;
+          const Piece &pc = batch[k];
+          const auto &cp = cps[pc.cp];
+          cqx::EriLaunch E{};
+          class_ls(cp.cb, E.la, E.lb);
+          class_ls(cp.ck, E.lc, E.ld);
+          E.quartets = c->d_quart.p + (f32 ? cap_need : 0) + pc.seg;
+          E.nq = (long long)counts[kk];
+          E.bra = c->d_pairs[cp.cb].p + pc.i0;
+          E.ket = c->d_pairs[cp.ck].p;
+          E.prims = c->d_prims.p;
+          E.rys = c->d_rys.p;
+          E.rys32 = c->d_rys32.p;
+          E.fp32 = f32 ? 1 : 0;
+          E.planes = ps;
+          E.store_mode = 0;
+          const int r = route(E.la, E.lb, E.lc, E.ld);
+          hipError_t e;
+          if (r == 0)
+            e = cqx::launch_eri_fast(E, s);
+          else if (r == 1)
+            e = cqx::launch_eri_wave(E, s);
+          else if (!f32)
+            e = cqx::launch_eri_team(E, s);
+          else
+            throw std::runtime_error("FP32 quartet list for a class without an FP32 kernel");
+          HIPCHK(e);
+          const double fl = class_pair_flops(E.la, E.lb, E.lc, E.ld, pure_d, psum[kk], (double)counts[kk],
+                                             is_complex ? 2 : 1, nJ, nK);
+          c->stats.n_quartets += (long long)counts[kk];
+          c->stats.n_prim_quartets_max += (long long)psum[kk];
+          c->stats.flops_model += fl;
+          if (f32) {
+            c->stats.n_quartets_f32 += (long long)counts[kk];
+            c->stats.flops_model_f32 += fl;
+          }
+          c->stats.n_launch_eri++;
         }
-        c->stats.n_launch_eri++;
+        HIPCHK(hipEventRecord(c->ev1, s));
+        HIPCHK(hipEventSynchronize(c->ev1));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        eri_ms_total += ms;
       }
-      HIPCHK(hipEventRecord(c->ev1, s));
       // back to pure functions, symmetrise, accumulate into AX
       for (int m = 0; m < nmat; ++m) {
         const bool two = is_complex && !(L[m].her && L[m].type == CQX_COULOMB);
@@ -850,9 +887,6 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
       for (int m = 0; m < nmat; ++m)
         HIPCHK(hipMemcpyAsync(L[m].AX, c->d_AX.p + m * nb2 * ew, sizeof(double) * nb2 * ew, dk, s));
       HIPCHK(hipStreamSynchronize(s));
-      float ms = 0.0f;
-      HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-      eri_ms_total += ms;
       start += nmat;
     }
     c->stats.ms_eri = eri_ms_total;

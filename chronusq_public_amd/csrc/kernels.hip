@@ -224,7 +224,8 @@ __global__ __launch_bounds__(256) void quartet_list_kernel(ListLaunch a) {
     // the block's candidates span bra indices [s_lo, s_hi]: two searches per
     // block, then each thread searches that (usually short) range
     if (threadIdx.x == 0 || threadIdx.x == blockDim.x - 1) {
-      const long long tt = threadIdx.x == 0 ? base : min(base + (long long)blockDim.x, a.ncand) - 1;
+      const long long tt =
+          a.cand_base + (threadIdx.x == 0 ? base : min(base + (long long)blockDim.x, a.ncand) - 1);
       int lo = 0, hi = a.nbra - 1;
       while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
@@ -236,14 +237,15 @@ __global__ __launch_bounds__(256) void quartet_list_kernel(ListLaunch a) {
     const int blo = s_lo, bhi = s_hi;
     __syncthreads();
     if (t < a.ncand) {
-      // bra index: last i with cand_off[i] <= t
+      // bra index: last i with cand_off[i] <= t (absolute candidate index)
+      const long long ta = a.cand_base + t;
       int lo = blo, hi = bhi;
       while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        if (a.cand_off[mid] <= t) lo = mid; else hi = mid - 1;
+        if (a.cand_off[mid] <= ta) lo = mid; else hi = mid - 1;
       }
       bi = lo;
-      kj = (int)(t - a.cand_off[lo]);
+      kj = (int)(ta - a.cand_off[lo]);
       const PairRec P = a.bra[bi], Q = a.ket[kj];
       const bool p_first = (P.s1 > Q.s1) || (P.s1 == Q.s1 && P.s2 >= Q.s2);
       const int s1 = p_first ? P.s1 : Q.s1, s2 = p_first ? P.s2 : Q.s2;

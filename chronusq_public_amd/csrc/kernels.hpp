@@ -37,7 +37,8 @@ bool eri_wave_available(int la, int lb, int lc, int ld);
 struct ListLaunch {
   const PairRec *bra, *ket;
   int same_class;
-  const long long *cand_off;  // nbra + 1 prefix of candidate counts
+  const long long *cand_off;  // nbra + 1 prefix of candidate counts (from cand_base)
+  long long cand_base;        // cand_off[0]: the list covers candidates cand_base + [0, ncand)
   int nbra;
   long long ncand;
   const double *Bg;           // NS x NS global block norms

@@ -125,7 +125,7 @@ struct cqx_scf {
   double e_nuc = 0.0;
   bool frac = false;      // fractional occupation (spin-averaged atoms of the SAD guess)
   double nelec = 0.0;     // electrons, fractional mode
-  Dev<double> S, H, X, D, Dp, J, K, G, F, W1, W2, W3, eps, ework, fscale, Fh, Eh;
+  Dev<double> S, H, X, D, Dp, J, K, G, F, Fx, W1, W2, W3, eps, ework, fscale, Fh, Eh;
   Dev<rocblas_int> info;
   std::vector<double> B;  // CDIIS error overlaps, keep x keep
   std::vector<double> eps_host;
@@ -164,8 +164,9 @@ struct cqx_scf {
   }
 
   // eigenvectors of Fm in the Lowdin basis -> C = X V in W3, eigenvalues
-  // of Fm / 2 in eps_host
+  // of Fm / 2 in eps_host (Fm must not be one of the work buffers W1-W3)
   void diagonalise(const double *Fm) {
+    if (Fm == W1.p || Fm == W2.p || Fm == W3.p) throw ScfError(CQX_ERR_ARG, "diagonalise: aliased input");
     gemm(rocblas_operation_none, rocblas_operation_none, nb, nb, nb, 1.0, Fm, X.p, 0.0, W1.p);
     gemm(rocblas_operation_transpose, rocblas_operation_none, nb, nb, nb, 1.0, X.p, W1.p, 0.0, W2.p);
     BLASOK(rocsolver_dsyevd(blas, rocblas_evect_original, rocblas_fill_upper, nb, W2.p, nb, eps.p, ework.p,
@@ -214,7 +215,9 @@ struct cqx_scf {
   }
 
   // CDIIS: store F and the Lowdin-basis commutator X^T (F D S - S D F) X,
-  // extrapolate F from the stored ones (nothing to combine on iteration 0)
+  // extrapolate F from the stored ones (nothing to combine on iteration 0);
+  // the extrapolated matrix goes to Fx, which no later step writes before
+  // the diagonalisation has read it
   const double *diis() {
     const int keep = o.diis_keep;
     if (keep <= 1) return F.p;
@@ -237,11 +240,11 @@ struct cqx_scf {
     for (int i = 0; i < nstored; ++i)
       for (int j = 0; j < nstored; ++j) Bn[(size_t)i * nstored + j] = B[(size_t)i * keep + j];
     if (!diis_solve(nstored, Bn, c)) return F.p;
-    HIPOK(cqx::launch_zero(W1.p, n2(), stream));
+    HIPOK(cqx::launch_zero(Fx.p, n2(), stream));
     for (int i = 0; i < nstored; ++i)
-      BLASOK(rocblas_daxpy(blas, (rocblas_int)n2(), &c[i], Fh.p + (size_t)i * n2(), 1, W1.p, 1));
+      BLASOK(rocblas_daxpy(blas, (rocblas_int)n2(), &c[i], Fh.p + (size_t)i * n2(), 1, Fx.p, 1));
     HIPOK(hipStreamSynchronize(stream));  // c lives on this stack frame
-    return W1.p;
+    return Fx.p;
   }
 
   // one iteration; returns true once converged
@@ -283,7 +286,7 @@ struct cqx_scf {
 
   void alloc_all() {
     const size_t m = n2();
-    for (auto *b : {&S, &H, &X, &D, &Dp, &J, &K, &G, &F, &W1, &W2, &W3}) b->alloc(m);
+    for (auto *b : {&S, &H, &X, &D, &Dp, &J, &K, &G, &F, &Fx, &W1, &W2, &W3}) b->alloc(m);
     eps.alloc(nb);
     ework.alloc(nb);
     fscale.alloc(nb);

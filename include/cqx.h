@@ -159,7 +159,10 @@ int cqx_last_stats(const cqx_ctx *ctx, cqx_stats *st);
  * wave-per-quartet ones), "fp32_cut" (mixed precision: quartets whose
  * Schwarz product Q_ab Q_cd is below this value are evaluated in FP32 with
  * an FP32 Rys table, digested and accumulated in FP64; default 0 = every
- * quartet in FP64; classes with an f shell always stay FP64). */
+ * quartet in FP64; classes with an f shell always stay FP64), "list_cap"
+ * (screened quartet-list entries built per batch, default 2^28: the list
+ * workspace is 8 B per entry, twice that with fp32_cut, whatever the
+ * molecule size). */
 int cqx_set_option(cqx_ctx *ctx, const char *name, double value);
 
 /* ---------------------------------------------------------------------
@@ -232,6 +235,46 @@ enum { CQX_SCF_DENSITY = 0, CQX_SCF_FOCK = 1, CQX_SCF_HCORE = 2, CQX_SCF_OVERLAP
 int cqx_scf_orbital_energies(const cqx_scf *s, double *eps);
 int cqx_scf_destroy(cqx_scf *s);
 const char *cqx_scf_last_error(const cqx_scf *s);
+
+/* ---------------------------------------------------------------------
+ * Serving loop (no counterpart in the reference, which runs one SCF per
+ * process): SCF jobs arrive over time and run concurrently on one GPU.
+ * The oldest waiting job is admitted as soon as fewer than max_inflight
+ * jobs run and its footprint estimate fits hbm_budget_bytes beside the
+ * running ones; each admitted job runs on its own context and stream and
+ * keeps its state on the device until it converges.  Blocks until every
+ * job has finished; returns the first non-zero job status, if any.
+ * ------------------------------------------------------------------- */
+typedef struct {
+  const char *g94;     /* basis set (G94 text) */
+  int natom;
+  const int *Z;        /* atomic numbers (neutral nuclei) */
+  const double *xyz;   /* 3 natom, bohr */
+  int nocc;            /* doubly occupied orbitals */
+  double arrival_s;    /* arrival time, seconds after the call starts */
+} cqx_job;
+
+typedef struct {
+  int max_inflight;        /* jobs on the device at once, default 8 */
+  double hbm_budget_bytes; /* admission budget, default 200e9 */
+  int device;
+  double thresh;           /* Schwarz threshold, default 1e-12 */
+  long long list_cap;      /* per-job quartet-list batch (see "list_cap"), default 2^25 */
+  cqx_scf_opts scf;        /* per-job SCF options (nocc taken from the job) */
+  int verbose;             /* 1: one line per finished job on stderr */
+} cqx_serve_opts;
+void cqx_serve_default_opts(cqx_serve_opts *o);
+
+typedef struct {
+  int status;              /* cqx_status of the job */
+  int converged, iterations;
+  double e_total;
+  double t_arrival, t_admit, t_done; /* seconds after the call started */
+  double footprint_bytes;  /* admission estimate */
+  double ms_fock;          /* the job's Fock-build time */
+} cqx_job_result;
+int cqx_serve(const cqx_job *jobs, int njobs, const cqx_serve_opts *opts,
+              cqx_job_result *out);
 
 #ifdef __cplusplus
 }
