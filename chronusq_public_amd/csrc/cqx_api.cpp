@@ -90,6 +90,7 @@ struct cqx_ctx {
   double prim_thresh = 1e-18;
   int fast_max_nc = 27;   // classes up to this Cartesian size use the lane-group kernels
   int use_wave = 1;       // templated wave-per-quartet kernels for the other s/p/d classes
+  int fast_lanes = 0;     // lanes per quartet of the lane-group kernels (0: per launch)
   double fp32_cut = 0.0;  // quartets with Q_ab Q_cd below this are evaluated in FP32
   long long list_cap = 1ll << 28;  // quartet-list entries per batch (8 B each, x2 when mixed)
   int device = 0;
@@ -553,6 +554,13 @@ int cqx_set_option(cqx_ctx *c, const char *name, double value) {
     c->use_wave = value != 0.0;
     return CQX_OK;
   }
+  if (!std::strcmp(name, "fast_lanes")) {
+    const int g = (int)value;
+    if (g != 0 && g != 1 && g != 2 && g != 4 && g != 8 && g != 16)
+      return fail(c, CQX_ERR_ARG, "fast_lanes must be 0, 1, 2, 4, 8 or 16");
+    c->fast_lanes = g;
+    return CQX_OK;
+  }
   if (!std::strcmp(name, "list_cap")) {
     if (!(value >= 1.0)) return fail(c, CQX_ERR_ARG, "list_cap must be >= 1");
     c->list_cap = (long long)value;
@@ -682,7 +690,11 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         P.mat = pd[p].m;
         (pd[p].part ? slot_im : slot_re)[pd[p].m] = p;
         if (ps.kcount[P.kind]++ == 0) ps.kstart[P.kind] = p;
+        ps.mats |= (unsigned long long)pd[p].m << (4 * p);
+        if (P.sign < 0.0) ps.negs |= 1u << p;
       }
+      ps.xbase = c->d_planes.p;
+      ps.abase = c->d_acc.p;
       for (int m = 0; m < nmat; ++m)
         HIPCHK(cqx::launch_to_cart(c->cmap, c->d_X.p + m * nb2 * ew, is_complex, c->d_planes.p + slot_re[m],
                                    slot_im[m] >= 0 ? c->d_planes.p + slot_im[m] : nullptr, PS, s));
@@ -845,6 +857,8 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           E.rys = c->d_rys.p;
           E.rys32 = c->d_rys32.p;
           E.fp32 = f32 ? 1 : 0;
+          E.mean_prims = psum[kk] / (double)counts[kk];
+          E.force_g = c->fast_lanes;
           E.planes = ps;
           E.store_mode = 0;
           const int r = route(E.la, E.lb, E.lc, E.ld);

@@ -51,6 +51,10 @@ struct PlaneSet {
   int nbc;  // Cartesian dimension NBc
   int ps;   // interleave stride (number of plane slots)
   int kstart[4], kcount[4];  // planes are ordered by kind: [kstart, kstart + kcount)
+  const double *xbase;       // density planes: slot s of element (i, j) at xbase[(i + j NBc) ps + s]
+  double *abase;             // accumulator planes, same layout
+  unsigned long long mats;   // contraction index of slot s in bits [4s, 4s + 4)
+  unsigned negs;             // bit s: the slot's K coefficient is negated (conj of Im X)
 };
 
 // A quartet: bra / ket pair indices within their class arrays plus the

@@ -1,0 +1,100 @@
+// J/K digestion of a tile of contracted Cartesian shell quartets of one
+// class, staged in LDS, by all 64 lanes of a wave.
+//
+// The updates are the reference's (direct.hpp:1858-2002) in the kernel's
+// canonical labels a, b, c, d: each update term
+//   A(ar, ac) += coef * sum_{xr, xc} I(a, b, c, d) X(xr, xc)
+// becomes work items (quartet, target element, plane slot) with the slot
+// fastest, so the lanes that share a target read and add consecutive
+// doubles of the interleaved planes and their loads and atomics coalesce.
+// Class sizes are compile-time constants (the index arithmetic folds); the
+// number of planes of a kind is padded to a power of two.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_types.hpp"
+
+namespace cqx {
+
+template <class K, int AR, int AC, int XR, int XC>
+__device__ __forceinline__ void digest_term_tile(const double *__restrict__ sI, const int (*__restrict__ sbf)[4],
+                                                 const unsigned *__restrict__ smask, int nq,
+                                                 const PlaneSet &ps, int p0, int np, int lgp, double base_coef,
+                                                 bool neg_ok, int lane) {
+  constexpr int NAC = K::n(AC), NT = K::n(AR) * NAC;
+  constexpr int ST[4] = {K::NCB * K::NCC * K::NCD, K::NCC * K::NCD, K::NCD, 1};
+  const int N = ps.nbc, S = ps.ps;
+  const int items = (nq * NT) << lgp;
+  for (int it = lane; it < items; it += 64) {
+    const int p = it & ((1 << lgp) - 1);
+    if (p >= np) continue;
+    const int r = it >> lgp;
+    const int qi = r / NT, tgt = r - qi * NT;
+    const int t0 = tgt / NAC, t1 = tgt - t0 * NAC;
+    const int slot = p0 + p;
+    const int mat = (int)((ps.mats >> (4 * slot)) & 15ull);
+    if (!((smask[qi] >> mat) & 1u)) continue;
+    const int *bf = sbf[qi];
+    const double *__restrict__ Iq = sI + qi * K::NC + t0 * ST[AR] + t1 * ST[AC];
+    const double *__restrict__ X = ps.xbase + slot + ((size_t)bf[XR] + (size_t)bf[XC] * N) * S;
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int u0 = 0; u0 < K::n(XR); ++u0)
+#pragma unroll
+      for (int u1 = 0; u1 < K::n(XC); ++u1) {
+        const double v = Iq[u0 * ST[XR] + u1 * ST[XC]] * X[((size_t)u0 + (size_t)u1 * N) * S];
+        if ((u0 * K::n(XC) + u1) & 1) s1 += v; else s0 += v;
+      }
+    const double coef = (neg_ok && ((ps.negs >> slot) & 1u)) ? -base_coef : base_coef;
+    unsafeAtomicAdd(ps.abase + slot + ((size_t)(bf[AR] + t0) + (size_t)(bf[AC] + t1) * N) * S, coef * (s0 + s1));
+  }
+}
+
+// Every update of every plane kind for nq quartets: sI holds quartet q at
+// sI[q * NC] (degeneracy and 1/2 applied), sbf[q] its first Cartesian
+// functions in roles a, b, c, d, smask[q] the contractions it passed.
+template <class K>
+__device__ __forceinline__ void digest_tile(const double *sI, const int (*sbf)[4], const unsigned *smask, int nq,
+                                            const PlaneSet &ps, int lane) {
+  for (int kind = 0; kind < 4; ++kind) {
+    const int np = ps.kcount[kind];
+    if (np == 0) continue;
+    const int p0 = ps.kstart[kind];
+    const int lgp = np <= 1 ? 0 : np <= 2 ? 1 : np <= 4 ? 2 : np <= 8 ? 3 : 4;
+    switch (kind) {
+      case kJHer:  // direct.hpp:1863-1883, real parts
+        digest_term_tile<K, 0, 1, 3, 2>(sI, sbf, smask, nq, ps, p0, np, lgp, 1.0, false, lane);
+        digest_term_tile<K, 3, 2, 0, 1>(sI, sbf, smask, nq, ps, p0, np, lgp, 1.0, false, lane);
+        break;
+      case kKHer:  // :1889-1917, 1/2 conj(X)
+        digest_term_tile<K, 0, 2, 3, 1>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, true, lane);
+        digest_term_tile<K, 3, 1, 0, 2>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, true, lane);
+        digest_term_tile<K, 3, 0, 1, 2>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, true, lane);
+        digest_term_tile<K, 1, 2, 3, 0>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, true, lane);
+        break;
+      case kJNonHer:  // :1925-1960
+        digest_term_tile<K, 0, 1, 3, 2>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 0, 1, 2, 3>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 1, 0, 3, 2>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 1, 0, 2, 3>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 2, 3, 1, 0>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 2, 3, 0, 1>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 3, 2, 1, 0>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 3, 2, 0, 1>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
+        break;
+      default:  // kKNonHer, :1961-1998
+        digest_term_tile<K, 2, 0, 3, 1>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 3, 1, 2, 0>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 3, 0, 2, 1>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 2, 1, 3, 0>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 0, 2, 1, 3>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 1, 3, 0, 2>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 0, 3, 1, 2>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 1, 2, 0, 3>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
+        break;
+    }
+  }
+}
+
+}  // namespace cqx

@@ -30,9 +30,15 @@ bool eri_fast_available(int la, int lb, int lc, int ld) {
 hipError_t launch_eri_fast(const EriLaunch &a, hipStream_t s) {
   if (a.nq <= 0) return hipSuccess;
   if (a.store_mode) return hipErrorNotSupported;
-  // lanes per quartet: the plane count rounded up to a power of two (<= 16)
+  // lanes per quartet: about a quarter of the mean primitive quartets per
+  // shell quartet (a power of two, 1..16), at least 2 for the larger
+  // classes so that a wave's LDS tile stays small
+  const int nc = ((a.la + 1) * (a.la + 2) / 2) * ((a.lb + 1) * (a.lb + 2) / 2) *
+                 ((a.lc + 1) * (a.lc + 2) / 2) * ((a.ld + 1) * (a.ld + 2) / 2);
   int G = 1;
-  while (G < a.planes.n && G < 16) G *= 2;
+  while (G < 16 && 4.0 * G < a.mean_prims) G *= 2;
+  if (nc > 9 && G < 2) G = 2;
+  if (a.force_g == 1 || a.force_g == 2 || a.force_g == 4 || a.force_g == 8 || a.force_g == 16) G = a.force_g;
   return a.fp32 ? launch_fast_r<float>(a, G, s) : launch_fast_r<double>(a, G, s);
 }
 

@@ -3,14 +3,15 @@
 // number G of lanes that share one shell quartet.
 //
 // Every size is a compile-time constant, so the Rys nodes/weights, the 2D
-// recursions and the contracted Cartesian quartet stay in registers.  The
-// digestion follows the reference formulas (direct.hpp:1858-2002) in the
-// kernel's canonical labels a,b,c,d; each update is reduced in registers
-// over the summed indices before one atomic add per target element.
+// recursions and the contracted Cartesian quartet stay in registers; the
+// quartets of a wave are then staged in LDS and digested by the whole wave
+// (digest_tile.hpp) following the reference formulas (direct.hpp:1858-2002).
 #pragma once
 
 #include <hip/hip_runtime.h>
 
+#include "digest.hpp"
+#include "digest_tile.hpp"
 #include "kernels.hpp"
 #include "rys.hpp"
 
@@ -48,6 +49,30 @@ struct Cls {
     return ((a * (LB + 1) + b) * (LC + 1) + c) * (LD + 1) + d;
   }
 };
+
+// For each Cartesian component o of the quartet, the offsets of its x, y
+// and z factors in the 2D-integral tables; built at compile time so that an
+// unrolled loop over o indexes the tables with constants.
+template <int LA, int LB, int LC, int LD>
+struct HTab {
+  int x[Cls<LA, LB, LC, LD>::NC], y[Cls<LA, LB, LC, LD>::NC], z[Cls<LA, LB, LC, LD>::NC];
+};
+
+template <int LA, int LB, int LC, int LD>
+constexpr HTab<LA, LB, LC, LD> make_htab() {
+  using K = Cls<LA, LB, LC, LD>;
+  HTab<LA, LB, LC, LD> t{};
+  for (int ia = 0; ia < K::NCA; ++ia)
+    for (int ib = 0; ib < K::NCB; ++ib)
+      for (int ic = 0; ic < K::NCC; ++ic)
+        for (int id = 0; id < K::NCD; ++id) {
+          const int o = K::oidx(ia, ib, ic, id);
+          t.x[o] = K::hidx(cexp(LA, ia, 0), cexp(LB, ib, 0), cexp(LC, ic, 0), cexp(LD, id, 0));
+          t.y[o] = K::hidx(cexp(LA, ia, 1), cexp(LB, ib, 1), cexp(LC, ic, 1), cexp(LD, id, 1));
+          t.z[o] = K::hidx(cexp(LA, ia, 2), cexp(LB, ib, 2), cexp(LC, ic, 2), cexp(LD, id, 2));
+        }
+  return t;
+}
 
 // 2D integrals of one root along one direction: VRR on (i, k), then HRR to
 // (ia, ib, ic, id); H has Cls::NH entries.
@@ -104,89 +129,30 @@ __device__ __forceinline__ void rys_2d_full(R C00, R D00, R B10, R B01, R B00, R
     }
 }
 
-// A(AR, AC) += coef * sum_{XR, XC} I * X(XR, XC), roles 0..3 = a, b, c, d.
-// Planes are interleaved with stride PS, so the lanes of a group (one plane
-// each) touch consecutive doubles: their loads and atomics coalesce.
-template <class K, int AR, int AC, int XR, int XC>
-__device__ __forceinline__ void digest_term(const double (&I)[K::NC], const int (&bf)[4],
-                                            const double *__restrict__ X, double *__restrict__ A,
-                                            int N, int PS, double coef) {
-#pragma unroll
-  for (int t0 = 0; t0 < K::n(AR); ++t0)
-#pragma unroll
-    for (int t1 = 0; t1 < K::n(AC); ++t1) {
-      double s = 0.0;
-#pragma unroll
-      for (int u0 = 0; u0 < K::n(XR); ++u0)
-#pragma unroll
-        for (int u1 = 0; u1 < K::n(XC); ++u1) {
-          int idx[4] = {0, 0, 0, 0};
-          idx[AR] = t0;
-          idx[AC] = t1;
-          idx[XR] = u0;
-          idx[XC] = u1;
-          const double x = X[((bf[XR] + u0) + (size_t)(bf[XC] + u1) * N) * PS];
-          s = fma(I[K::oidx(idx[0], idx[1], idx[2], idx[3])], x, s);
-        }
-      atomic_add(&A[((bf[AR] + t0) + (size_t)(bf[AC] + t1) * N) * PS], coef * s);
-    }
-}
-
-template <class K>
-__device__ __forceinline__ void digest_plane(const double (&I)[K::NC], const int (&bf)[4],
-                                             const Plane &P, int N, int PS) {
-  const double *X = P.X;
-  double *A = P.A;
-  switch (P.kind) {
-    case kJHer:
-      digest_term<K, 0, 1, 3, 2>(I, bf, X, A, N, PS, 1.0);
-      digest_term<K, 3, 2, 0, 1>(I, bf, X, A, N, PS, 1.0);
-      break;
-    case kKHer: {
-      const double h = 0.5 * P.sign;
-      digest_term<K, 0, 2, 3, 1>(I, bf, X, A, N, PS, h);
-      digest_term<K, 3, 1, 0, 2>(I, bf, X, A, N, PS, h);
-      digest_term<K, 3, 0, 1, 2>(I, bf, X, A, N, PS, h);
-      digest_term<K, 1, 2, 3, 0>(I, bf, X, A, N, PS, h);
-    } break;
-    case kJNonHer:
-      digest_term<K, 0, 1, 3, 2>(I, bf, X, A, N, PS, 0.5);
-      digest_term<K, 0, 1, 2, 3>(I, bf, X, A, N, PS, 0.5);
-      digest_term<K, 1, 0, 3, 2>(I, bf, X, A, N, PS, 0.5);
-      digest_term<K, 1, 0, 2, 3>(I, bf, X, A, N, PS, 0.5);
-      digest_term<K, 2, 3, 1, 0>(I, bf, X, A, N, PS, 0.5);
-      digest_term<K, 2, 3, 0, 1>(I, bf, X, A, N, PS, 0.5);
-      digest_term<K, 3, 2, 1, 0>(I, bf, X, A, N, PS, 0.5);
-      digest_term<K, 3, 2, 0, 1>(I, bf, X, A, N, PS, 0.5);
-      break;
-    case kKNonHer:
-      digest_term<K, 2, 0, 3, 1>(I, bf, X, A, N, PS, 0.5);
-      digest_term<K, 3, 1, 2, 0>(I, bf, X, A, N, PS, 0.5);
-      digest_term<K, 3, 0, 2, 1>(I, bf, X, A, N, PS, 0.5);
-      digest_term<K, 2, 1, 3, 0>(I, bf, X, A, N, PS, 0.5);
-      digest_term<K, 0, 2, 1, 3>(I, bf, X, A, N, PS, 0.5);
-      digest_term<K, 1, 3, 0, 2>(I, bf, X, A, N, PS, 0.5);
-      digest_term<K, 0, 3, 1, 2>(I, bf, X, A, N, PS, 0.5);
-      digest_term<K, 1, 2, 0, 3>(I, bf, X, A, N, PS, 0.5);
-      break;
-  }
-}
-
 // Rys table of the evaluation precision
 template <class R> __device__ __forceinline__ const R *rys_table(const EriLaunch &a);
 template <> __device__ __forceinline__ const double *rys_table<double>(const EriLaunch &a) { return a.rys; }
 template <> __device__ __forceinline__ const float *rys_table<float>(const EriLaunch &a) { return a.rys32; }
 
 // G lanes per quartet: the lanes split the primitive quartets of the shell
-// quartet, combine the contracted Cartesian integrals with a butterfly of
-// lane shuffles, and then lane g digests planes g, g + G, ...
+// quartet and combine the contracted Cartesian integrals with a butterfly of
+// lane shuffles; the wave's 64 / G quartets then go to LDS and all 64 lanes
+// digest them (digest_tile.hpp), so the registers of the evaluation and of
+// the digestion are never live together.
 // R is the evaluation precision (Rys nodes, 2D integrals, the contracted
 // quartet); the digestion and the accumulators are FP64 either way.
 template <int LA, int LB, int LC, int LD, int G, class R>
 __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
   using K = Cls<LA, LB, LC, LD>;
-  const int g = threadIdx.x & (G - 1);
-  const long long q = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  constexpr int QPW = 64 / G;  // quartets per wave
+  __shared__ double s_I[4][QPW * K::NC];
+  __shared__ int s_bf[4][QPW][4];
+  __shared__ unsigned s_mask[4][QPW];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane & (G - 1), qw = lane / G;
+  const long long q0 = ((long long)blockIdx.x * 4 + wv) * QPW;
+  if (q0 >= a.nq) return;  // uniform across the wave; no block barriers below
+  const long long q = q0 + qw;
   const bool valid = q < a.nq;
   Quartet qt{0u, 0u};
   PairRec bp{}, kp{};
@@ -243,20 +209,9 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
           rys_2d_full<LA, LB, LC, LD, R>(C00, D00, B10, B01, B00, (R)bp.AB[d], (R)kp.AB[d],
                                          d == 2 ? pref * w[r] : (R)1, H[d]);
         }
+        constexpr HTab<LA, LB, LC, LD> HT = make_htab<LA, LB, LC, LD>();
 #pragma unroll
-        for (int ia = 0; ia < K::NCA; ++ia)
-#pragma unroll
-          for (int ib = 0; ib < K::NCB; ++ib)
-#pragma unroll
-            for (int ic = 0; ic < K::NCC; ++ic)
-#pragma unroll
-              for (int id = 0; id < K::NCD; ++id) {
-                const R hx = H[0][K::hidx(cexp(LA, ia, 0), cexp(LB, ib, 0), cexp(LC, ic, 0), cexp(LD, id, 0))];
-                const R hy = H[1][K::hidx(cexp(LA, ia, 1), cexp(LB, ib, 1), cexp(LC, ic, 1), cexp(LD, id, 1))];
-                const R hz = H[2][K::hidx(cexp(LA, ia, 2), cexp(LB, ib, 2), cexp(LC, ic, 2), cexp(LD, id, 2))];
-                R &o = I[K::oidx(ia, ib, ic, id)];
-                o = fma(hx * hy, hz, o);
-              }
+        for (int o = 0; o < K::NC; ++o) I[o] = fma(H[0][HT.x[o]] * H[1][HT.y[o]], H[2][HT.z[o]], I[o]);
       }
     }
   }
@@ -266,26 +221,29 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
 #pragma unroll
       for (int o = 0; o < K::NC; ++o) I[o] += __shfl_xor(I[o], off);
   }
-  if (!valid) return;
   // degeneracy (direct.hpp:1664-1667, 1793-1802) and the 1/2 of :1849-1850
   const bool same_pair = (bp.a == kp.a && bp.b == kp.b);
   const double scale = 0.5 * (bp.a == bp.b ? 1.0 : 2.0) * (kp.a == kp.b ? 1.0 : 2.0) * (same_pair ? 1.0 : 2.0);
-  double Id[K::NC];
+  double *sI = s_I[wv] + qw * K::NC;
 #pragma unroll
-  for (int o = 0; o < K::NC; ++o) Id[o] = (double)I[o] * scale;
-  const int bf[4] = {bp.bfa, bp.bfb, kp.bfa, kp.bfb};
-  const unsigned mask = qt.ket_mask >> 24;
-  for (int p = g; p < a.planes.n; p += G) {
-    const Plane &P = a.planes.p[p];
-    if (!((mask >> P.mat) & 1u)) continue;
-    digest_plane<K>(Id, bf, P, a.planes.nbc, a.planes.ps);
+  for (int o = 0; o < K::NC; ++o)
+    if (o % G == g) sI[o] = (double)I[o] * scale;
+  if (g == 0) {
+    s_bf[wv][qw][0] = bp.bfa;
+    s_bf[wv][qw][1] = bp.bfb;
+    s_bf[wv][qw][2] = kp.bfa;
+    s_bf[wv][qw][3] = kp.bfb;
+    s_mask[wv][qw] = valid ? (qt.ket_mask >> 24) : 0u;
   }
+  wave_sync();
+  const long long left = a.nq - q0;
+  digest_tile<K>(s_I[wv], s_bf[wv], s_mask[wv], left < QPW ? (int)left : QPW, a.planes, lane);
 }
 
 template <int LA, int LB, int LC, int LD, int G, class R>
 hipError_t launch_cls(const EriLaunch &a, hipStream_t s) {
-  const long long threads = a.nq * G;
-  hipLaunchKernelGGL((eri_fast_kernel<LA, LB, LC, LD, G, R>), dim3((unsigned)((threads + 255) / 256)),
+  constexpr long long per_block = 4 * (64 / G);
+  hipLaunchKernelGGL((eri_fast_kernel<LA, LB, LC, LD, G, R>), dim3((unsigned)((a.nq + per_block - 1) / per_block)),
                      dim3(256), 0, s, a);
   return hipGetLastError();
 }

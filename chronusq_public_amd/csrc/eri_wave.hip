@@ -6,7 +6,7 @@
 // < 3 NR build the 2D integrals of their root and direction in registers
 // and store them to the wave's LDS slice, and every lane accumulates its
 // own share of the contracted Cartesian quartet.  After the primitive loop
-// the quartet goes to LDS and the lanes digest it (digest.hpp).
+// the quartet goes to LDS and the lanes digest it (digest_tile.hpp).
 #include <hip/hip_runtime.h>
 
 #include "digest.hpp"
@@ -27,6 +27,8 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
   constexpr int HSZ = 3 * K::NR * K::NH;
   __shared__ R s_H[kWavesPerBlock][HSZ];
   __shared__ double s_I[kWavesPerBlock][K::NC];
+  __shared__ int s_bf[kWavesPerBlock][1][4];
+  __shared__ unsigned s_mask[kWavesPerBlock][1];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long long q = (long long)blockIdx.x * kWavesPerBlock + wv;
   if (q >= a.nq) return;  // uniform across the wave; no block barriers below
@@ -114,10 +116,15 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
 #pragma unroll
   for (int k = 0; k < NOWN; ++k)
     if (lane + 64 * k < K::NC) sI[lane + 64 * k] = (double)acc[k] * scale;
+  if (lane == 0) {
+    s_bf[wv][0][0] = bp.bfa;
+    s_bf[wv][0][1] = bp.bfb;
+    s_bf[wv][0][2] = kp.bfa;
+    s_bf[wv][0][3] = kp.bfb;
+    s_mask[wv][0] = qt.ket_mask >> 24;
+  }
   wave_sync();
-  const int n[4] = {K::NCA, K::NCB, K::NCC, K::NCD};
-  const int bf[4] = {bp.bfa, bp.bfb, kp.bfa, kp.bfb};
-  digest_wave(sI, n, bf, a.planes, qt.ket_mask >> 24, lane, 64);
+  digest_tile<K>(sI, s_bf[wv], s_mask[wv], 1, a.planes, lane);
 }
 
 template <int LA, int LB, int LC, int LD>

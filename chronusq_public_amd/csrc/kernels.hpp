@@ -18,6 +18,8 @@ struct EriLaunch {
   const double *rys;
   const float *rys32;            // the same table rounded to FP32
   int fp32;                      // evaluate the quartets in FP32 (digestion stays FP64)
+  double mean_prims;             // mean primitive quartets per listed shell quartet
+  int force_g;                   // lanes per quartet of the lane-group kernels (0: chosen per launch)
   PlaneSet planes;
   double *store;                 // STORE mode: contracted Cartesian quartets
   const long long *store_off;    // per quartet offset into store
