@@ -678,7 +678,18 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         for (int k = 0; k < (two ? 2 : 1); ++k)
           pd.push_back({m, k, kind, (k == 1 && L[m].her) ? -1.0 : 1.0});  // conj of Im X
       }
-      std::stable_sort(pd.begin(), pd.end(), [](const PlaneDesc &x, const PlaneDesc &y) { return x.kind < y.kind; });
+      // the kind with the most planes first, so that its slots start on an
+      // interleave-group boundary and the lanes digesting one element
+      // touch as few 64-byte segments as possible
+      int kcnt[4] = {0, 0, 0, 0}, krank[4];
+      for (const auto &d : pd) ++kcnt[d.kind];
+      for (int k = 0; k < 4; ++k) {
+        krank[k] = 0;
+        for (int j = 0; j < 4; ++j)
+          if (kcnt[j] > kcnt[k] || (kcnt[j] == kcnt[k] && j < k)) ++krank[k];
+      }
+      std::stable_sort(pd.begin(), pd.end(),
+                       [&](const PlaneDesc &x, const PlaneDesc &y) { return krank[x.kind] < krank[y.kind]; });
       std::vector<int> slot_re(nmat, -1), slot_im(nmat, -1);
       for (int k = 0; k < 4; ++k) { ps.kstart[k] = 0; ps.kcount[k] = 0; }
       for (int p = 0; p < (int)pd.size(); ++p) {

@@ -22,7 +22,7 @@ __device__ __forceinline__ void digest_term_tile(const double *__restrict__ sI, 
                                                  const unsigned *__restrict__ smask, int nq,
                                                  const PlaneSet &ps, int p0, int np, int lgp, double base_coef,
                                                  bool neg_ok, int lane) {
-  constexpr int NAC = K::n(AC), NT = K::n(AR) * NAC;
+  constexpr int NAR = K::n(AR), NT = NAR * K::n(AC);
   constexpr int ST[4] = {K::NCB * K::NCC * K::NCD, K::NCC * K::NCD, K::NCD, 1};
   const int N = ps.nbc, S = ps.ps;
   const int items = (nq * NT) << lgp;
@@ -31,7 +31,8 @@ __device__ __forceinline__ void digest_term_tile(const double *__restrict__ sI, 
     if (p >= np) continue;
     const int r = it >> lgp;
     const int qi = r / NT, tgt = r - qi * NT;
-    const int t0 = tgt / NAC, t1 = tgt - t0 * NAC;
+    // target rows fastest: neighbouring lanes hit neighbouring elements
+    const int t1 = tgt / NAR, t0 = tgt - t1 * NAR;
     const int slot = p0 + p;
     const int mat = (int)((ps.mats >> (4 * slot)) & 15ull);
     if (!((smask[qi] >> mat) & 1u)) continue;
