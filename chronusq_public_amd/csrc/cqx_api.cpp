@@ -658,15 +658,8 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
                                      c->d_sh_nf.p, c->d_Bm.p, c->d_Bg.p, s));
       std::vector<double> Bg((size_t)NS * NS);
       HIPCHK(hipMemcpyAsync(Bg.data(), c->d_Bg.p, sizeof(double) * Bg.size(), hipMemcpyDeviceToHost, s));
-      // Cartesian density planes and accumulators, interleaved with stride PS
-      int PS = 1;
-      while (PS < nplane) PS *= 2;
-      c->d_planes.ensure(PS * nbc2);
-      c->d_acc.ensure(PS * nbc2);
-      HIPCHK(cqx::launch_zero(c->d_acc.p, PS * nbc2, s));
       cqx::PlaneSet ps{};
       ps.nbc = NBC;
-      ps.ps = PS;
       // planes (one per real part), ordered by kind; slot = position, so the
       // planes that one digestion term visits sit in consecutive slots
       struct PlaneDesc { int m, part, kind; double sign; };
@@ -690,12 +683,35 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
       }
       std::stable_sort(pd.begin(), pd.end(),
                        [&](const PlaneDesc &x, const PlaneDesc &y) { return krank[x.kind] < krank[y.kind]; });
+      // Cartesian density planes and accumulators: one interleaved array per
+      // kind, stride kps[k] (its plane count rounded up to a power of two)
+      size_t koff[4], ktot = 0;
+      for (int k = 0; k < 4; ++k) {
+        ps.kps[k] = 1;
+        while (ps.kps[k] < kcnt[k]) ps.kps[k] *= 2;
+        koff[k] = ktot;
+        if (kcnt[k]) ktot += (size_t)ps.kps[k] * nbc2;
+      }
+      c->d_planes.ensure(std::max<size_t>(ktot, 1));
+      c->d_acc.ensure(std::max<size_t>(ktot, 1));
+      HIPCHK(cqx::launch_zero(c->d_acc.p, ktot, s));
+      for (int k = 0; k < 4; ++k) {
+        ps.kx[k] = c->d_planes.p + koff[k];
+        ps.ka[k] = c->d_acc.p + koff[k];
+      }
       std::vector<int> slot_re(nmat, -1), slot_im(nmat, -1);
+      std::vector<double *> pX(nplane), pA(nplane);
+      std::vector<int> pS(nplane);
       for (int k = 0; k < 4; ++k) { ps.kstart[k] = 0; ps.kcount[k] = 0; }
       for (int p = 0; p < (int)pd.size(); ++p) {
         cqx::Plane &P = ps.p[p];
-        P.X = c->d_planes.p + p;
-        P.A = c->d_acc.p + p;
+        const int k = pd[p].kind;
+        const int idx = ps.kcount[k];  // position within its kind
+        P.X = c->d_planes.p + koff[k] + idx;
+        P.A = c->d_acc.p + koff[k] + idx;
+        pX[p] = c->d_planes.p + koff[k] + idx;
+        pA[p] = c->d_acc.p + koff[k] + idx;
+        pS[p] = ps.kps[k];
         P.kind = pd[p].kind;
         P.sign = pd[p].sign;
         P.mat = pd[p].m;
@@ -704,11 +720,10 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         ps.mats |= (unsigned long long)pd[p].m << (4 * p);
         if (P.sign < 0.0) ps.negs |= 1u << p;
       }
-      ps.xbase = c->d_planes.p;
-      ps.abase = c->d_acc.p;
+      // the real and imaginary planes of a contraction share a kind, hence a stride
       for (int m = 0; m < nmat; ++m)
-        HIPCHK(cqx::launch_to_cart(c->cmap, c->d_X.p + m * nb2 * ew, is_complex, c->d_planes.p + slot_re[m],
-                                   slot_im[m] >= 0 ? c->d_planes.p + slot_im[m] : nullptr, PS, s));
+        HIPCHK(cqx::launch_to_cart(c->cmap, c->d_X.p + m * nb2 * ew, is_complex, pX[slot_re[m]],
+                                   slot_im[m] >= 0 ? pX[slot_im[m]] : nullptr, pS[slot_re[m]], s));
       ps.n = nplane;
       int nJ = 0, nK = 0;
       for (int m = 0; m < nmat; ++m) (L[m].type == CQX_COULOMB ? nJ : nK)++;
@@ -903,9 +918,9 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
       // back to pure functions, symmetrise, accumulate into AX
       for (int m = 0; m < nmat; ++m) {
         const bool two = is_complex && !(L[m].her && L[m].type == CQX_COULOMB);
-        const double *are = c->d_acc.p + slot_re[m];
-        const double *aim = two ? c->d_acc.p + slot_im[m] : nullptr;
-        HIPCHK(cqx::launch_from_cart(c->cmap, are, aim, PS, L[m].her, is_complex,
+        const double *are = pA[slot_re[m]];
+        const double *aim = two ? pA[slot_im[m]] : nullptr;
+        HIPCHK(cqx::launch_from_cart(c->cmap, are, aim, pS[slot_re[m]], L[m].her, is_complex,
                                      c->d_AX.p + m * nb2 * ew, s));
       }
       const hipMemcpyKind dk = mem == CQX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;

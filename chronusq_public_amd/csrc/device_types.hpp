@@ -49,10 +49,15 @@ struct PlaneSet {
   Plane p[kMaxPlanes];
   int n;
   int nbc;  // Cartesian dimension NBc
-  int ps;   // interleave stride (number of plane slots)
   int kstart[4], kcount[4];  // planes are ordered by kind: [kstart, kstart + kcount)
-  const double *xbase;       // density planes: slot s of element (i, j) at xbase[(i + j NBc) ps + s]
-  double *abase;             // accumulator planes, same layout
+  // The planes of one kind are interleaved in an array of their own: plane
+  // kstart[k] + i of element (r, c) sits at kx[k][(r + c NBc) kps[k] + i]
+  // (accumulators: ka[k]), kps[k] = kcount[k] rounded up to a power of two.
+  // A lone plane thus has adjacent rows adjacent in memory, and a group of
+  // planes shares one 64-byte segment per element.
+  const double *kx[4];
+  double *ka[4];
+  int kps[4];
   unsigned long long mats;   // contraction index of slot s in bits [4s, 4s + 4)
   unsigned negs;             // bit s: the slot's K coefficient is negated (conj of Im X)
 };

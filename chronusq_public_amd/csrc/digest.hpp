@@ -4,9 +4,9 @@
 // The updates are those of the reference (direct.hpp:1858-2002) written in
 // the kernel's canonical labels a, b, c, d (roles 0..3).  Each update
 //   A(ar, ac) += coef * sum_{xr, xc} I(a,b,c,d) X(xr, xc)
-// is spread over the lanes with the plane index running fastest: planes are
-// interleaved in memory, so lanes that share a target element read and add
-// consecutive doubles and their loads / atomics coalesce.
+// is spread over the lanes with the plane index running fastest: the planes
+// of a kind are interleaved in memory, so lanes that share a target element
+// read and add consecutive doubles and their loads / atomics coalesce.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -39,12 +39,13 @@ __device__ __constant__ const DigestTerms kDigestTerms = {
 
 __device__ inline void digest_wave(const double *__restrict__ sI, const int n[4], const int bf[4],
                                    const PlaneSet &ps, unsigned mask, int lane, int W) {
-  const int N = ps.nbc, S = ps.ps;
+  const int N = ps.nbc;
   // stride of each role in the quartet layout I[((a n_b + b) n_c + c) n_d + d]
   const int st[4] = {n[1] * n[2] * n[3], n[2] * n[3], n[3], 1};
   for (int kind = 0; kind < 4; ++kind) {
     const int p0 = ps.kstart[kind], np = ps.kcount[kind];
     if (np == 0) continue;
+    const int S = ps.kps[kind];
     const double base_coef = kind == kJHer ? 1.0 : 0.5;
     for (int term = 0; term < kDigestTerms.n[kind]; ++term) {
       const int ar = kDigestTerms.t[kind][term][0], ac = kDigestTerms.t[kind][term][1];

@@ -6,7 +6,9 @@
 //   A(ar, ac) += coef * sum_{xr, xc} I(a, b, c, d) X(xr, xc)
 // becomes work items (quartet, target element, plane slot) with the slot
 // fastest, so the lanes that share a target read and add consecutive
-// doubles of the interleaved planes and their loads and atomics coalesce.
+// doubles of the kind's interleaved planes and their loads and atomics
+// coalesce; target rows come next, so with one plane neighbouring lanes
+// hit neighbouring doubles.
 // Class sizes are compile-time constants (the index arithmetic folds); the
 // number of planes of a kind is padded to a power of two.
 #pragma once
@@ -24,7 +26,10 @@ __device__ __forceinline__ void digest_term_tile(const double *__restrict__ sI, 
                                                  bool neg_ok, int lane) {
   constexpr int NAR = K::n(AR), NT = NAR * K::n(AC);
   constexpr int ST[4] = {K::NCB * K::NCC * K::NCD, K::NCC * K::NCD, K::NCD, 1};
-  const int N = ps.nbc, S = ps.ps;
+  const int N = ps.nbc;
+  const int kind = ps.p[p0].kind, S = ps.kps[kind];
+  const double *__restrict__ xk = ps.kx[kind];
+  double *ak = ps.ka[kind];
   const int items = (nq * NT) << lgp;
   for (int it = lane; it < items; it += 64) {
     const int p = it & ((1 << lgp) - 1);
@@ -38,7 +43,7 @@ __device__ __forceinline__ void digest_term_tile(const double *__restrict__ sI, 
     if (!((smask[qi] >> mat) & 1u)) continue;
     const int *bf = sbf[qi];
     const double *__restrict__ Iq = sI + qi * K::NC + t0 * ST[AR] + t1 * ST[AC];
-    const double *__restrict__ X = ps.xbase + slot + ((size_t)bf[XR] + (size_t)bf[XC] * N) * S;
+    const double *__restrict__ X = xk + p + ((size_t)bf[XR] + (size_t)bf[XC] * N) * S;
     double s0 = 0.0, s1 = 0.0;
 #pragma unroll
     for (int u0 = 0; u0 < K::n(XR); ++u0)
@@ -48,7 +53,7 @@ __device__ __forceinline__ void digest_term_tile(const double *__restrict__ sI, 
         if ((u0 * K::n(XC) + u1) & 1) s1 += v; else s0 += v;
       }
     const double coef = (neg_ok && ((ps.negs >> slot) & 1u)) ? -base_coef : base_coef;
-    unsafeAtomicAdd(ps.abase + slot + ((size_t)(bf[AR] + t0) + (size_t)(bf[AC] + t1) * N) * S, coef * (s0 + s1));
+    unsafeAtomicAdd(ak + p + ((size_t)(bf[AR] + t0) + (size_t)(bf[AC] + t1) * N) * S, coef * (s0 + s1));
   }
 }
 
