@@ -51,7 +51,7 @@ SYMBOLS = ["cqx_basis_create", "cqx_basis_from_g94", "cqx_basis_destroy",
            "cqx_ctx_create", "cqx_ctx_destroy", "cqx_last_error", "cqx_ctx_schwarz",
            "cqx_ctx_shell_pairs", "cqx_two_body_contract", "cqx_eri_quartets",
            "cqx_last_stats", "cqx_set_option", "cqx_device_alloc", "cqx_device_free",
-           "cqx_memcpy", "cqx_device_zero", "cqx_one_body", "cqx_scf_default_opts",
+           "cqx_memcpy", "cqx_device_zero", "cqx_one_body", "cqx_basis_batch", "cqx_scf_default_opts",
            "cqx_scf_create", "cqx_scf_iterate", "cqx_scf_run", "cqx_scf_result_get",
            "cqx_scf_matrix", "cqx_scf_orbital_energies", "cqx_scf_destroy",
            "cqx_scf_last_error", "cqx_serve_default_opts", "cqx_serve"]
@@ -88,6 +88,7 @@ def lib():
         L.cqx_memcpy.argtypes = [P, P, P, C.c_size_t, I]
         L.cqx_device_zero.argtypes = [P, P, C.c_size_t]
         L.cqx_one_body.argtypes = [P, I, I, P, P, P]
+        L.cqx_basis_batch.argtypes = [I, P, C.POINTER(P)]
         _lib = L
     return _lib
 
@@ -131,6 +132,23 @@ class BasisSet:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.cqx_basis_destroy(self._h)
             self._h = None
+
+    @classmethod
+    def batch(cls, parts):
+        """Several molecules' BasisSets side by side (cqx_basis_batch): a
+        DirectTPI on the result contracts all molecules in shared launches;
+        its twoBodyContract takes the molecules' contractions back to back."""
+        arr = (C.c_void_p * len(parts))(*[p._h.value for p in parts])
+        h = C.c_void_p()
+        _check(lib().cqx_basis_batch(len(parts), C.cast(arr, C.c_void_p), C.byref(h)))
+        self = cls.__new__(cls)
+        self._h = h
+        self.mol = None
+        self.parts = list(parts)
+        dims = [C.c_int() for _ in range(4)]
+        _check(lib().cqx_basis_dims(self._h, *[C.byref(d) for d in dims]))
+        self.nShell, self.nBasis, self.maxPrim, self.maxL = (d.value for d in dims)
+        return self
 
     def one_body(self, kind, Z=None, xyz=None):
         """S, T or V (point nuclei; default: this basis' molecule) as an

@@ -257,7 +257,19 @@ void Basis::finalise() {
   max_l = 0;
   bf.clear();
   bf_cart.clear();
-  for (auto &s : shells) {
+  nmol = 0;
+  mol_sh0.clear();
+  mol_bf0.clear();
+  mol_bfc0.clear();
+  for (size_t k = 0; k < shells.size(); ++k) {
+    const auto &s = shells[k];
+    if (k == 0 || s.mol != shells[k - 1].mol) {
+      if (s.mol != nmol) throw std::runtime_error("batched shells must be grouped by molecule in order");
+      mol_sh0.push_back((int)k);
+      mol_bf0.push_back(nbasis);
+      mol_bfc0.push_back(ncart_total);
+      ++nmol;
+    }
     bf.push_back(nbasis);
     bf_cart.push_back(ncart_total);
     nbasis += s.nfunc();
@@ -265,6 +277,27 @@ void Basis::finalise() {
     max_prim = std::max(max_prim, (int)s.alpha.size());
     max_l = std::max(max_l, s.l);
   }
+  if (nmol == 0) nmol = 1, mol_sh0.push_back(0), mol_bf0.push_back(0), mol_bfc0.push_back(0);
+  mol_sh0.push_back((int)shells.size());
+  mol_bf0.push_back(nbasis);
+  mol_bfc0.push_back(ncart_total);
+}
+
+Basis Basis::batch(const std::vector<const Basis *> &parts) {
+  Basis out;
+  int atom0 = 0;
+  for (size_t m = 0; m < parts.size(); ++m) {
+    int amax = -1;
+    for (auto sh : parts[m]->shells) {
+      amax = std::max(amax, sh.atom);
+      sh.atom += atom0;  // distinct centre labels across molecules
+      sh.mol = (int)m;
+      out.shells.push_back(sh);
+    }
+    atom0 += amax + 1;
+  }
+  out.finalise();
+  return out;
 }
 
 // Contracted overlap block (pure where the shell is pure), row-major n1 x n2.
@@ -337,14 +370,16 @@ std::vector<double> Basis::overlap_matrix() const {
 }
 
 // Significant pairs s2 <= s1: same centre (coordinates equal to 1e-5, the
-// centre match of BasisSet::update) or 2-norm of the overlap block >= 1e-12.
+// centre match of BasisSet::update) or 2-norm of the overlap block >= 1e-12;
+// in a batched set only pairs within one molecule.
 std::vector<std::pair<int, int>> Basis::significant_pairs() const {
   std::vector<std::vector<double>> sph;
   for (int l = 0; l <= kMaxL; ++l) sph.push_back(solid_harmonics(l));
   std::vector<std::pair<int, int>> out;
   int ns = (int)shells.size();
   for (int s1 = 0; s1 < ns; ++s1)
-    for (int s2 = 0; s2 <= s1; ++s2) {
+    for (int s2 = mol_sh0.empty() ? 0 : mol_sh0[shells[s1].mol]; s2 <= s1; ++s2) {
+      // pairs never cross molecules of a batch (their shells are contiguous)
       const Shell &A = shells[s1], &B = shells[s2];
       bool same = std::fabs(A.origin[0] - B.origin[0]) < 1e-5 &&
                   std::fabs(A.origin[1] - B.origin[1]) < 1e-5 &&

@@ -98,8 +98,11 @@ struct cqx_ctx {
   std::string err;
 
   int ns = 0, nb = 0, nbc = 0;
+  int nmol = 1;                                // molecules of a batched context
   std::vector<std::pair<int, int>> ref_pairs;  // (s1 >= s2) reference order
-  std::vector<std::vector<PairRec>> cls;       // per class, sorted by Q desc
+  std::vector<std::vector<PairRec>> cls;       // per class, sorted by (molecule, Q desc)
+  std::vector<std::vector<int>> seg;           // per class: molecule m's pairs [seg[m], seg[m + 1])
+  std::vector<long long> mol_boff;             // molecule m's NS_m^2 block-norm offset
   std::vector<PrimPair> prims;
   std::vector<double> schwarz;                 // NS x NS
 
@@ -114,7 +117,10 @@ struct cqx_ctx {
   // per-call workspace
   DevBuf<double> d_X, d_AX, d_Bm, d_Bg, d_planes, d_acc;
   DevBuf<int> d_her;
-  DevBuf<long long> d_cand;
+  DevBuf<long long> d_cand, d_mol_boff;
+  DevBuf<int> d_mol_ns, d_ket_lo;
+  DevBuf<double> d_molmax;
+  DevBuf<cqx::MolPlanes> d_molplanes;
   DevBuf<Quartet> d_quart;
   DevBuf<unsigned long long> d_count;
   DevBuf<double> d_psum;
@@ -250,9 +256,12 @@ std::vector<double> eri_store(cqx_ctx *c, const std::vector<PairRec> &bra,
 
 PairRec make_pair_rec(const cqx::Basis &B, int a, int b, int s1, int s2, int pp_off, int npp) {
   PairRec r{};
-  r.a = a; r.b = b; r.s1 = s1; r.s2 = s2;
+  const int m = B.shells[a].mol;
+  r.mol = m;
+  r.a = a; r.b = b;
+  r.s1 = s1 - B.mol_sh0[m]; r.s2 = s2 - B.mol_sh0[m];  // numbered within the molecule
   r.pp_off = pp_off; r.npp = npp;
-  r.bfa = B.bf_cart[a]; r.bfb = B.bf_cart[b];
+  r.bfa = B.bf_cart[a] - B.mol_bfc0[m]; r.bfb = B.bf_cart[b] - B.mol_bfc0[m];
   r.Q = 0.0;
   for (int d = 0; d < 3; ++d) r.AB[d] = B.shells[a].origin[d] - B.shells[b].origin[d];
   r.kk = B.shells[a].nprim() * B.shells[b].nprim();
@@ -299,6 +308,17 @@ void setup_device_maps(cqx_ctx *c) {
   c->cmap.nb = c->nb;
   c->cmap.nbc = c->nbc;
   c->cmap.ns = c->ns;
+  c->cmap.bf0 = 0;
+  c->cmap.bfc0 = 0;
+  // per-molecule block-norm layout
+  std::vector<int> mns(c->nmol);
+  c->mol_boff.assign(c->nmol + 1, 0);
+  for (int m = 0; m < c->nmol; ++m) {
+    mns[m] = B.mol_sh0[m + 1] - B.mol_sh0[m];
+    c->mol_boff[m + 1] = c->mol_boff[m] + (long long)mns[m] * mns[m];
+  }
+  c->d_mol_ns.upload(mns, c->stream);
+  c->d_mol_boff.upload(c->mol_boff, c->stream);
 }
 
 // Significant pairs, primitive pairs and GPU Schwarz bounds.
@@ -334,7 +354,7 @@ void build_pairs(cqx_ctx *c) {
   }
   // Schwarz: (ab|ab) for every significant pair, with all primitives
   // (the reference evaluates these at precision 0, aointegrals_utilities.cxx:70)
-  c->schwarz.assign((size_t)c->ns * c->ns, 0.0);
+  if (c->nmol == 1) c->schwarz.assign((size_t)c->ns * c->ns, 0.0);
   for (int k = 0; k < kNumClass; ++k) {
     auto &v = c->cls[k];
     if (v.empty()) continue;
@@ -355,7 +375,7 @@ void build_pairs(cqx_ctx *c) {
       auto pure = cart_to_pure(B, sh, ints.data() + q * ncq);
       const int na = B.shells[r.a].nfunc(), nbf = B.shells[r.b].nfunc();
       // rows of the infinity norm run over the functions of s1
-      const bool a_is_s1 = (r.a == r.s1);
+      const bool a_is_s1 = (r.a == r.s1 + B.mol_sh0[r.mol]);  // s1 is numbered within the molecule
       const int n1 = a_is_s1 ? na : nbf, n2 = a_is_s1 ? nbf : na;
       double nrm = 0.0;
       for (int i = 0; i < n1; ++i) {
@@ -368,10 +388,25 @@ void build_pairs(cqx_ctx *c) {
         nrm = std::max(nrm, row);
       }
       r.Q = std::sqrt(nrm);
-      c->schwarz[r.s1 + (size_t)r.s2 * c->ns] = r.Q;
-      c->schwarz[r.s2 + (size_t)r.s1 * c->ns] = r.Q;
+      if (c->nmol == 1) {
+        c->schwarz[r.s1 + (size_t)r.s2 * c->ns] = r.Q;
+        c->schwarz[r.s2 + (size_t)r.s1 * c->ns] = r.Q;
+      }
     }
-    std::stable_sort(v.begin(), v.end(), [](const PairRec &x, const PairRec &y) { return x.Q > y.Q; });
+    std::stable_sort(v.begin(), v.end(), [](const PairRec &x, const PairRec &y) {
+      return x.mol != y.mol ? x.mol < y.mol : x.Q > y.Q;
+    });
+  }
+  // molecule segments of every class
+  c->seg.assign(kNumClass, std::vector<int>(c->nmol + 1, 0));
+  for (int k = 0; k < kNumClass; ++k) {
+    auto &sg = c->seg[k];
+    const auto &v = c->cls[k];
+    size_t i = 0;
+    for (int m = 0; m <= c->nmol; ++m) {
+      while (i < v.size() && v[i].mol < m) ++i;
+      sg[m] = (int)i;
+    }
   }
   c->d_prims.upload(c->prims, c->stream);
   for (int k = 0; k < kNumClass; ++k) c->d_pairs[k].upload(c->cls[k], c->stream);
@@ -416,6 +451,24 @@ int cqx_basis_from_g94(const char *text, int natom, const int *Z, const double *
     auto b = std::make_unique<cqx_basis>();
     b->b = cqx::Basis::from_g94(text, std::vector<int>(Z, Z + natom),
                                 std::vector<double>(xyz, xyz + 3 * natom), force_cart != 0);
+    *out = b.release();
+    return CQX_OK;
+  } catch (const std::exception &e) {
+    return fail(nullptr, CQX_ERR_BASIS, e.what());
+  }
+}
+
+int cqx_basis_batch(int nmol, const cqx_basis *const *parts, cqx_basis **out) {
+  if (nmol <= 0 || !parts || !out) return fail(nullptr, CQX_ERR_ARG, "cqx_basis_batch: bad arguments");
+  try {
+    std::vector<const cqx::Basis *> v;
+    for (int m = 0; m < nmol; ++m) {
+      if (!parts[m]) return fail(nullptr, CQX_ERR_ARG, "cqx_basis_batch: null basis");
+      if (parts[m]->b.nmol != 1) return fail(nullptr, CQX_ERR_ARG, "cqx_basis_batch: parts must be single molecules");
+      v.push_back(&parts[m]->b);
+    }
+    auto b = std::make_unique<cqx_basis>();
+    b->b = cqx::Basis::batch(v);
     *out = b.release();
     return CQX_OK;
   } catch (const std::exception &e) {
@@ -492,6 +545,7 @@ int cqx_ctx_create(const cqx_basis *b, double thresh, int device, cqx_ctx **out)
     c->ns = (int)c->basis.shells.size();
     c->nb = c->basis.nbasis;
     c->nbc = c->basis.ncart_total;
+    c->nmol = c->basis.nmol;
     std::vector<double> rys(kRysTable, kRysTable + sizeof(kRysTable) / sizeof(double));
     c->d_rys.upload(rys, c->stream);
     std::vector<float> rys32(rys.begin(), rys.end());
@@ -520,6 +574,7 @@ int cqx_ctx_destroy(cqx_ctx *c) {
 
 int cqx_ctx_schwarz(cqx_ctx *c, double *Q) {
   if (!c || !Q) return fail(c, CQX_ERR_ARG, "cqx_ctx_schwarz: bad arguments");
+  if (c->nmol != 1) return fail(c, CQX_ERR_UNSUPPORTED, "cqx_ctx_schwarz: not kept for a batched context");
   std::copy(c->schwarz.begin(), c->schwarz.end(), Q);
   return CQX_OK;
 }
@@ -619,57 +674,86 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
   for (int i = 0; i < ncon; ++i)
     if (!list[i].X || !list[i].AX || (list[i].type != CQX_COULOMB && list[i].type != CQX_EXCHANGE))
       return fail(c, CQX_ERR_ARG, "invalid contraction " + std::to_string(i));
+  const int nmol = c->nmol;
+  if (ncon % nmol)
+    return fail(c, CQX_ERR_ARG, "a batched context takes the same number of contractions for every molecule");
+  const int nper = ncon / nmol;
+  for (int m = 1; m < nmol; ++m)
+    for (int i = 0; i < nper; ++i)
+      if (list[m * nper + i].type != list[i].type || (list[m * nper + i].her != 0) != (list[i].her != 0))
+        return fail(c, CQX_ERR_ARG, "batched molecules must share the contraction types and HER flags");
   const auto t_start = std::chrono::steady_clock::now();
   c->stats = cqx_stats{};
   try {
     HIPCHK(hipSetDevice(c->device));
     hipStream_t s = c->stream;
-    const int NB = c->nb, NBC = c->nbc, NS = c->ns;
-    const size_t nb2 = (size_t)NB * NB, nbc2 = (size_t)NBC * NBC;
+    const cqx::Basis &BS = c->basis;
     const size_t ew = is_complex ? 2 : 1;  // doubles per element
+    // per-molecule sizes
+    std::vector<int> mnb(nmol), mnbc(nmol), mns(nmol);
+    std::vector<size_t> mnb2(nmol), mnbc2(nmol);
+    for (int m = 0; m < nmol; ++m) {
+      mnb[m] = BS.mol_bf0[m + 1] - BS.mol_bf0[m];
+      mnbc[m] = BS.mol_bfc0[m + 1] - BS.mol_bfc0[m];
+      mns[m] = BS.mol_sh0[m + 1] - BS.mol_sh0[m];
+      mnb2[m] = (size_t)mnb[m] * mnb[m];
+      mnbc2[m] = (size_t)mnbc[m] * mnbc[m];
+    }
     float eri_ms_total = 0.0f;
-    // groups of at most kMaxMat contractions / kMaxPlanes planes
+    bool pure_d = false;
+    for (auto &sh : BS.shells)
+      if (sh.l > 1 && sh.pure) pure_d = true;
+    // groups of at most kMaxMat contractions / kMaxPlanes planes (per molecule)
     int start = 0;
-    while (start < ncon) {
+    while (start < nper) {
       int nmat = 0, nplane = 0;
-      while (start + nmat < ncon && nmat < cqx::kMaxMat) {
+      while (start + nmat < nper && nmat < cqx::kMaxMat) {
         const auto &cc = list[start + nmat];
         int np = (!is_complex || (cc.her && cc.type == CQX_COULOMB)) ? 1 : 2;
         if (nplane + np > cqx::kMaxPlanes) break;
         nplane += np;
         ++nmat;
       }
-      const cqx_contraction *L = list + start;
-      // densities and outputs on the device
-      c->d_X.ensure(nmat * nb2 * ew);
-      c->d_AX.ensure(nmat * nb2 * ew);
+      const cqx_contraction *L = list + start;  // molecule 0's contractions of the group
+      // densities and outputs on the device: molecule m's nmat matrices at xoff[m]
+      std::vector<size_t> xoff(nmol + 1, 0);
+      for (int m = 0; m < nmol; ++m) xoff[m + 1] = xoff[m] + (size_t)nmat * mnb2[m] * ew;
+      c->d_X.ensure(std::max<size_t>(xoff[nmol], 1));
+      c->d_AX.ensure(std::max<size_t>(xoff[nmol], 1));
       const hipMemcpyKind hk = mem == CQX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-      for (int m = 0; m < nmat; ++m) {
-        HIPCHK(hipMemcpyAsync(c->d_X.p + m * nb2 * ew, L[m].X, sizeof(double) * nb2 * ew, hk, s));
-        HIPCHK(hipMemcpyAsync(c->d_AX.p + m * nb2 * ew, L[m].AX, sizeof(double) * nb2 * ew, hk, s));
-      }
+      for (int m = 0; m < nmol; ++m)
+        for (int j = 0; j < nmat; ++j) {
+          const cqx_contraction &cc = list[m * nper + start + j];
+          const size_t o = xoff[m] + (size_t)j * mnb2[m] * ew, n = sizeof(double) * mnb2[m] * ew;
+          HIPCHK(hipMemcpyAsync(c->d_X.p + o, cc.X, n, hk, s));
+          HIPCHK(hipMemcpyAsync(c->d_AX.p + o, cc.AX, n, hk, s));
+        }
       std::vector<int> her(nmat);
-      for (int m = 0; m < nmat; ++m) her[m] = L[m].her ? 1 : 0;
+      for (int j = 0; j < nmat; ++j) her[j] = L[j].her ? 1 : 0;
       c->d_her.upload(her, s);
-      // shell-block norms
-      c->d_Bm.ensure((size_t)nmat * NS * NS);
-      c->d_Bg.ensure((size_t)NS * NS);
-      HIPCHK(cqx::launch_block_norms(c->d_X.p, nmat, is_complex, c->d_her.p, NB, NS, c->d_sh_bf.p,
-                                     c->d_sh_nf.p, c->d_Bm.p, c->d_Bg.p, s));
-      std::vector<double> Bg((size_t)NS * NS);
-      HIPCHK(hipMemcpyAsync(Bg.data(), c->d_Bg.p, sizeof(double) * Bg.size(), hipMemcpyDeviceToHost, s));
+      // shell-block norms, per molecule, and each molecule's largest one
+      const long long nbl = c->mol_boff[nmol];
+      c->d_Bm.ensure((size_t)nmat * nbl);
+      c->d_Bg.ensure((size_t)nbl);
+      for (int m = 0; m < nmol; ++m)
+        HIPCHK(cqx::launch_block_norms(c->d_X.p + xoff[m], nmat, is_complex, c->d_her.p, mnb[m], mns[m],
+                                       c->d_sh_bf.p + BS.mol_sh0[m], BS.mol_bf0[m], c->d_sh_nf.p + BS.mol_sh0[m],
+                                       c->d_Bm.p + (size_t)c->mol_boff[m] * nmat, c->d_Bg.p + c->mol_boff[m], s));
+      c->d_molmax.ensure(nmol);
+      HIPCHK(cqx::launch_mol_max(c->d_Bg.p, c->d_mol_boff.p, c->d_mol_ns.p, nmol, c->d_molmax.p, s));
+      std::vector<double> bmax_mol(nmol);
+      HIPCHK(hipMemcpyAsync(bmax_mol.data(), c->d_molmax.p, sizeof(double) * nmol, hipMemcpyDeviceToHost, s));
       cqx::PlaneSet ps{};
-      ps.nbc = NBC;
       // planes (one per real part), ordered by kind; slot = position, so the
       // planes that one digestion term visits sit in consecutive slots
       struct PlaneDesc { int m, part, kind; double sign; };
       std::vector<PlaneDesc> pd;
-      for (int m = 0; m < nmat; ++m) {
-        const bool two = is_complex && !(L[m].her && L[m].type == CQX_COULOMB);
-        int kind = L[m].type == CQX_COULOMB ? (L[m].her ? cqx::kJHer : cqx::kJNonHer)
-                                            : (L[m].her ? cqx::kKHer : cqx::kKNonHer);
+      for (int j = 0; j < nmat; ++j) {
+        const bool two = is_complex && !(L[j].her && L[j].type == CQX_COULOMB);
+        int kind = L[j].type == CQX_COULOMB ? (L[j].her ? cqx::kJHer : cqx::kJNonHer)
+                                            : (L[j].her ? cqx::kKHer : cqx::kKNonHer);
         for (int k = 0; k < (two ? 2 : 1); ++k)
-          pd.push_back({m, k, kind, (k == 1 && L[m].her) ? -1.0 : 1.0});  // conj of Im X
+          pd.push_back({j, k, kind, (k == 1 && L[j].her) ? -1.0 : 1.0});  // conj of Im X
       }
       // the kind with the most planes first, so that its slots start on an
       // interleave-group boundary and the lanes digesting one element
@@ -683,58 +767,80 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
       }
       std::stable_sort(pd.begin(), pd.end(),
                        [&](const PlaneDesc &x, const PlaneDesc &y) { return krank[x.kind] < krank[y.kind]; });
-      // Cartesian density planes and accumulators: one interleaved array per
-      // kind, stride kps[k] (its plane count rounded up to a power of two)
-      size_t koff[4], ktot = 0;
+      // Cartesian density planes and accumulators: per molecule, one
+      // interleaved array per kind with stride kps[k] (its plane count
+      // rounded up to a power of two)
+      int kpsum = 0;
       for (int k = 0; k < 4; ++k) {
         ps.kps[k] = 1;
         while (ps.kps[k] < kcnt[k]) ps.kps[k] *= 2;
-        koff[k] = ktot;
-        if (kcnt[k]) ktot += (size_t)ps.kps[k] * nbc2;
+        if (kcnt[k]) kpsum += ps.kps[k];
       }
-      c->d_planes.ensure(std::max<size_t>(ktot, 1));
-      c->d_acc.ensure(std::max<size_t>(ktot, 1));
-      HIPCHK(cqx::launch_zero(c->d_acc.p, ktot, s));
-      for (int k = 0; k < 4; ++k) {
-        ps.kx[k] = c->d_planes.p + koff[k];
-        ps.ka[k] = c->d_acc.p + koff[k];
+      std::vector<size_t> poff(nmol + 1, 0);
+      for (int m = 0; m < nmol; ++m) poff[m + 1] = poff[m] + (size_t)kpsum * mnbc2[m];
+      c->d_planes.ensure(std::max<size_t>(poff[nmol], 1));
+      c->d_acc.ensure(std::max<size_t>(poff[nmol], 1));
+      HIPCHK(cqx::launch_zero(c->d_acc.p, poff[nmol], s));
+      std::vector<cqx::MolPlanes> mp(nmol);
+      for (int m = 0; m < nmol; ++m) {
+        size_t o = poff[m];
+        for (int k = 0; k < 4; ++k) {
+          mp[m].kx[k] = c->d_planes.p + o;
+          mp[m].ka[k] = c->d_acc.p + o;
+          if (kcnt[k]) o += (size_t)ps.kps[k] * mnbc2[m];
+        }
+        mp[m].nbc = mnbc[m];
       }
-      std::vector<int> slot_re(nmat, -1), slot_im(nmat, -1);
-      std::vector<double *> pX(nplane), pA(nplane);
-      std::vector<int> pS(nplane);
+      c->d_molplanes.upload(mp, s);
+      ps.mols = c->d_molplanes.p;
+      ps.m0 = mp[0];
+      ps.nmol = nmol;
+      std::vector<int> slot_re(nmat, -1), slot_im(nmat, -1), pidx(nplane);
       for (int k = 0; k < 4; ++k) { ps.kstart[k] = 0; ps.kcount[k] = 0; }
       for (int p = 0; p < (int)pd.size(); ++p) {
         cqx::Plane &P = ps.p[p];
         const int k = pd[p].kind;
-        const int idx = ps.kcount[k];  // position within its kind
-        P.X = c->d_planes.p + koff[k] + idx;
-        P.A = c->d_acc.p + koff[k] + idx;
-        pX[p] = c->d_planes.p + koff[k] + idx;
-        pA[p] = c->d_acc.p + koff[k] + idx;
-        pS[p] = ps.kps[k];
-        P.kind = pd[p].kind;
+        pidx[p] = ps.kcount[k];  // position within its kind
+        P.kind = k;
         P.sign = pd[p].sign;
         P.mat = pd[p].m;
         (pd[p].part ? slot_im : slot_re)[pd[p].m] = p;
-        if (ps.kcount[P.kind]++ == 0) ps.kstart[P.kind] = p;
+        if (ps.kcount[k]++ == 0) ps.kstart[k] = p;
         ps.mats |= (unsigned long long)pd[p].m << (4 * p);
         if (P.sign < 0.0) ps.negs |= 1u << p;
       }
-      // the real and imaginary planes of a contraction share a kind, hence a stride
-      for (int m = 0; m < nmat; ++m)
-        HIPCHK(cqx::launch_to_cart(c->cmap, c->d_X.p + m * nb2 * ew, is_complex, pX[slot_re[m]],
-                                   slot_im[m] >= 0 ? pX[slot_im[m]] : nullptr, pS[slot_re[m]], s));
       ps.n = nplane;
+      // Cartesian planes of every molecule (real and imaginary planes of a
+      // contraction share a kind, hence a stride)
+      auto mol_map = [&](int m) {
+        cqx::CartMap M = c->cmap;
+        M.cart_shell += BS.mol_bfc0[m];
+        M.cart_comp += BS.mol_bfc0[m];
+        M.bf_shell += BS.mol_bf0[m];
+        M.bf_comp += BS.mol_bf0[m];
+        M.nb = mnb[m];
+        M.nbc = mnbc[m];
+        M.ns = mns[m];
+        M.bf0 = BS.mol_bf0[m];
+        M.bfc0 = BS.mol_bfc0[m];
+        return M;
+      };
+      for (int m = 0; m < nmol; ++m) {
+        const cqx::CartMap M = mol_map(m);
+        for (int j = 0; j < nmat; ++j) {
+          const int pr = slot_re[j], pi = slot_im[j];
+          double *xre = const_cast<double *>(mp[m].kx[pd[pr].kind]) + pidx[pr];
+          double *xim = pi >= 0 ? const_cast<double *>(mp[m].kx[pd[pi].kind]) + pidx[pi] : nullptr;
+          HIPCHK(cqx::launch_to_cart(M, c->d_X.p + xoff[m] + (size_t)j * mnb2[m] * ew, is_complex, xre, xim,
+                                     ps.kps[pd[pr].kind], s));
+        }
+      }
       int nJ = 0, nK = 0;
-      for (int m = 0; m < nmat; ++m) (L[m].type == CQX_COULOMB ? nJ : nK)++;
-      bool pure_d = false;
-      for (auto &sh : c->basis.shells)
-        if (sh.l > 1 && sh.pure) pure_d = true;
+      for (int j = 0; j < nmat; ++j) (L[j].type == CQX_COULOMB ? nJ : nK)++;
       HIPCHK(hipStreamSynchronize(s));
-      double bmax = 0.0;
-      for (double v : Bg) bmax = std::max(bmax, v);
-      // candidate lists per class pair (bra class >= ket class)
-      struct CP { int cb, ck; long long ncand; size_t coff; std::vector<long long> off; };
+      // candidate lists per class pair (bra class >= ket class): bra i of
+      // molecule m meets a prefix of molecule m's kets (Q descending)
+      struct CP { int cb, ck; long long ncand; size_t coff; std::vector<long long> off; std::vector<int> klo; };
       std::vector<CP> cps;
       long long total = 0;
       size_t cand_total = 0;
@@ -746,25 +852,30 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           cp.cb = cb;
           cp.ck = ck;
           cp.off.resize(vb.size() + 1);
+          cp.klo.resize(vb.size());
           cp.off[0] = 0;
           for (size_t i = 0; i < vb.size(); ++i) {
-            long long jmax = (long long)vk.size();
-            if (screen) {
-              if (bmax * vb[i].Q * vk[0].Q < c->thresh) {
+            const int m = vb[i].mol;
+            const long long lo = c->seg[ck][m], hi = c->seg[ck][m + 1];
+            long long jmax = hi - lo;
+            const double bmax = bmax_mol[m];
+            if (screen && jmax > 0) {
+              if (bmax * vb[i].Q * vk[lo].Q < c->thresh) {
                 jmax = 0;
               } else {
                 // largest prefix with Q_i Q_j bmax >= thresh (Q sorted descending)
                 const double need = c->thresh / (bmax * vb[i].Q);
-                long long lo = 0, hi = (long long)vk.size();
-                while (lo < hi) {
-                  long long mid = (lo + hi) / 2;
-                  if (vk[mid].Q >= need) lo = mid + 1; else hi = mid;
+                long long a = lo, b = hi;
+                while (a < b) {
+                  long long mid = (a + b) / 2;
+                  if (vk[mid].Q >= need) a = mid + 1; else b = mid;
                 }
-                jmax = lo;
+                jmax = a - lo;
               }
             }
-            if (cb == ck) jmax = std::min<long long>(jmax, (long long)i + 1);
-            cp.off[i + 1] = cp.off[i] + jmax;
+            if (cb == ck) jmax = std::min<long long>(jmax, (long long)i - lo + 1);
+            cp.klo[i] = (int)lo;
+            cp.off[i + 1] = cp.off[i] + std::max<long long>(jmax, 0);
           }
           cp.ncand = cp.off.back();
           if (cp.ncand == 0) continue;
@@ -811,12 +922,22 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
       }
       c->d_quart.ensure((size_t)cap_need * (mixed ? 2 : 1));
       c->d_cand.ensure(std::max<size_t>(cand_total, 1));
+      c->d_ket_lo.ensure(std::max<size_t>(cand_total, 1));
       std::vector<long long> cand_all;
+      std::vector<int> klo_all;
       cand_all.reserve(cand_total);
-      for (auto &cp : cps) cand_all.insert(cand_all.end(), cp.off.begin(), cp.off.end());
-      if (!cand_all.empty())
+      klo_all.reserve(cand_total);
+      for (auto &cp : cps) {
+        cand_all.insert(cand_all.end(), cp.off.begin(), cp.off.end());
+        klo_all.insert(klo_all.end(), cp.klo.begin(), cp.klo.end());
+        klo_all.push_back(0);  // keep klo aligned with off (nbra + 1 entries)
+      }
+      if (!cand_all.empty()) {
         HIPCHK(hipMemcpyAsync(c->d_cand.p, cand_all.data(), sizeof(long long) * cand_all.size(),
                               hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->d_ket_lo.p, klo_all.data(), sizeof(int) * klo_all.size(),
+                              hipMemcpyHostToDevice, s));
+      }
       for (const auto &batch : batches) {
         const size_t np = batch.size();
         if (!np) continue;
@@ -835,12 +956,14 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           LL.same_class = cp.cb == cp.ck;
           LL.cand_off = c->d_cand.p + cp.coff + pc.i0;
           LL.cand_base = pc.base;
+          LL.ket_lo = c->d_ket_lo.p + cp.coff + pc.i0;
           LL.nbra = pc.i1 - pc.i0;
           LL.ncand = pc.ncand;
           LL.Bg = c->d_Bg.p;
           LL.Bm = c->d_Bm.p;
+          LL.mol_boff = c->d_mol_boff.p;
+          LL.mol_ns = c->d_mol_ns.p;
           LL.nmat = nmat;
-          LL.ns = NS;
           LL.thresh = c->thresh;
           LL.screen = screen;
           LL.out = c->d_quart.p + pc.seg;
@@ -864,7 +987,8 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
                               hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(psum.data(), c->d_psum.p, sizeof(double) * nslot, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        // ERI + digestion
+        // ERI + digestion: one launch per class pair and precision for every
+        // molecule of the batch
         HIPCHK(hipEventRecord(c->ev0, s));
         for (size_t kk = 0; kk < nslot; ++kk) {
           if (!counts[kk]) continue;
@@ -916,16 +1040,22 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         eri_ms_total += ms;
       }
       // back to pure functions, symmetrise, accumulate into AX
-      for (int m = 0; m < nmat; ++m) {
-        const bool two = is_complex && !(L[m].her && L[m].type == CQX_COULOMB);
-        const double *are = pA[slot_re[m]];
-        const double *aim = two ? pA[slot_im[m]] : nullptr;
-        HIPCHK(cqx::launch_from_cart(c->cmap, are, aim, pS[slot_re[m]], L[m].her, is_complex,
-                                     c->d_AX.p + m * nb2 * ew, s));
+      for (int m = 0; m < nmol; ++m) {
+        const cqx::CartMap M = mol_map(m);
+        for (int j = 0; j < nmat; ++j) {
+          const bool two = is_complex && !(L[j].her && L[j].type == CQX_COULOMB);
+          const int pr = slot_re[j], pi = slot_im[j];
+          const double *are = mp[m].ka[pd[pr].kind] + pidx[pr];
+          const double *aim = two ? mp[m].ka[pd[pi].kind] + pidx[pi] : nullptr;
+          HIPCHK(cqx::launch_from_cart(M, are, aim, ps.kps[pd[pr].kind], L[j].her, is_complex,
+                                       c->d_AX.p + xoff[m] + (size_t)j * mnb2[m] * ew, s));
+        }
       }
       const hipMemcpyKind dk = mem == CQX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-      for (int m = 0; m < nmat; ++m)
-        HIPCHK(hipMemcpyAsync(L[m].AX, c->d_AX.p + m * nb2 * ew, sizeof(double) * nb2 * ew, dk, s));
+      for (int m = 0; m < nmol; ++m)
+        for (int j = 0; j < nmat; ++j)
+          HIPCHK(hipMemcpyAsync(list[m * nper + start + j].AX, c->d_AX.p + xoff[m] + (size_t)j * mnb2[m] * ew,
+                                sizeof(double) * mnb2[m] * ew, dk, s));
       HIPCHK(hipStreamSynchronize(s));
       start += nmat;
     }

@@ -13,14 +13,14 @@ constexpr int kMaxMat = 8;     // contractions sharing one ERI evaluation
 // angular momentum (la >= lb).  s1 >= s2 is the same pair in the order the
 // reference loop visits it (direct.hpp:1659-1664), used for screening.
 struct PairRec {
-  int a, b;        // kernel-orientation shells
-  int s1, s2;      // reference-orientation shells
+  int a, b;        // kernel-orientation shells (global indices)
+  int s1, s2;      // reference-orientation shells, numbered within the molecule
   int pp_off, npp; // primitive pairs [pp_off, pp_off + npp)
-  int bfa, bfb;    // first Cartesian function of a and b
+  int bfa, bfb;    // first Cartesian function of a and b within the molecule
   double Q;        // Schwarz bound of the pair
   double AB[3];    // A - B
   int kk;          // Ka * Kb (contraction lengths, before primitive screening)
-  int pad;
+  int mol;         // molecule of a batched context (0 for one molecule)
 };
 
 // One surviving primitive pair of a shell pair (kernel orientation).
@@ -37,26 +37,33 @@ enum PlaneKind : int { kJHer = 0, kKHer = 1, kJNonHer = 2, kKNonHer = 3 };
 // Planes are interleaved: element (i, j) of plane p lives at
 // base[(i + j * NBc) * PlaneSet::ps + p]; X / A below already include + p.
 struct Plane {
-  const double *X;  // Cartesian density plane, NBc x NBc column-major
-  double *A;        // Cartesian accumulator plane
   int kind;         // PlaneKind
   double sign;      // +1 real part; -1 imaginary part of a conjugated X
   int mat;          // contraction index (screening mask bit)
   int pad;
 };
 
+// The Cartesian density and accumulator planes of one molecule: plane
+// kstart[k] + i of element (r, c) sits at kx[k][(r + c nbc) kps[k] + i]
+// (accumulators ka[k]); r, c count the molecule's Cartesian functions.
+struct MolPlanes {
+  const double *kx[4];
+  double *ka[4];
+  int nbc;
+  int pad;
+};
+
 struct PlaneSet {
   Plane p[kMaxPlanes];
   int n;
-  int nbc;  // Cartesian dimension NBc
   int kstart[4], kcount[4];  // planes are ordered by kind: [kstart, kstart + kcount)
-  // The planes of one kind are interleaved in an array of their own: plane
-  // kstart[k] + i of element (r, c) sits at kx[k][(r + c NBc) kps[k] + i]
-  // (accumulators: ka[k]), kps[k] = kcount[k] rounded up to a power of two.
-  // A lone plane thus has adjacent rows adjacent in memory, and a group of
-  // planes shares one 64-byte segment per element.
-  const double *kx[4];
-  double *ka[4];
+  // The planes of one kind are interleaved in an array of their own, stride
+  // kps[k] = kcount[k] rounded up to a power of two: a lone plane keeps
+  // adjacent rows adjacent in memory, a group of planes shares one 64-byte
+  // segment per element.  Per molecule: mols[PairRec::mol].
+  const MolPlanes *mols;
+  MolPlanes m0;  // mols[0], held in the kernel arguments for one-molecule launches
+  int nmol;
   int kps[4];
   unsigned long long mats;   // contraction index of slot s in bits [4s, 4s + 4)
   unsigned negs;             // bit s: the slot's K coefficient is negated (conj of Im X)

@@ -38,14 +38,17 @@ __device__ __constant__ const DigestTerms kDigestTerms = {
       {0, 2, 1, 3}, {1, 3, 0, 2}, {0, 3, 1, 2}, {1, 2, 0, 3}}}};
 
 __device__ inline void digest_wave(const double *__restrict__ sI, const int n[4], const int bf[4],
-                                   const PlaneSet &ps, unsigned mask, int lane, int W) {
-  const int N = ps.nbc;
+                                   const PlaneSet &ps, unsigned mask, int mol, int lane, int W) {
+  const MolPlanes &mp = ps.nmol == 1 ? ps.m0 : ps.mols[mol];
+  const int N = mp.nbc;
   // stride of each role in the quartet layout I[((a n_b + b) n_c + c) n_d + d]
   const int st[4] = {n[1] * n[2] * n[3], n[2] * n[3], n[3], 1};
   for (int kind = 0; kind < 4; ++kind) {
     const int p0 = ps.kstart[kind], np = ps.kcount[kind];
     if (np == 0) continue;
     const int S = ps.kps[kind];
+    const double *xk = mp.kx[kind];
+    double *ak = mp.ka[kind];
     const double base_coef = kind == kJHer ? 1.0 : 0.5;
     for (int term = 0; term < kDigestTerms.n[kind]; ++term) {
       const int ar = kDigestTerms.t[kind][term][0], ac = kDigestTerms.t[kind][term][1];
@@ -60,7 +63,7 @@ __device__ inline void digest_wave(const double *__restrict__ sI, const int n[4]
         if (!((mask >> P.mat) & 1u)) continue;
         const int t0 = tgt / n_ac, t1 = tgt - (tgt / n_ac) * n_ac;
         const double *__restrict__ Ib = sI + t0 * s_ar + t1 * s_ac;
-        const double *__restrict__ Xb = P.X + ((size_t)bf_xr + (size_t)bf_xc * N) * S;
+        const double *__restrict__ Xb = xk + (pl - p0) + ((size_t)bf_xr + (size_t)bf_xc * N) * S;
         double sum0 = 0.0, sum1 = 0.0;
         for (int u0 = 0; u0 < n_xr; ++u0) {
           const double *Iu = Ib + u0 * s_xr;
@@ -73,7 +76,7 @@ __device__ inline void digest_wave(const double *__restrict__ sI, const int n[4]
           if (u1 < n_xc) sum0 = fma(Iu[u1 * s_xc], Xu[(size_t)u1 * N * S], sum0);
         }
         const double coef = kind == kKHer ? base_coef * P.sign : base_coef;
-        unsafeAtomicAdd(&P.A[((bf_ar + t0) + (size_t)(bf_ac + t1) * N) * S], coef * (sum0 + sum1));
+        unsafeAtomicAdd(&ak[(pl - p0) + ((bf_ar + t0) + (size_t)(bf_ac + t1) * N) * S], coef * (sum0 + sum1));
       }
     }
   }

@@ -21,15 +21,12 @@ namespace cqx {
 
 template <class K, int AR, int AC, int XR, int XC>
 __device__ __forceinline__ void digest_term_tile(const double *__restrict__ sI, const int (*__restrict__ sbf)[4],
-                                                 const unsigned *__restrict__ smask, int nq,
-                                                 const PlaneSet &ps, int p0, int np, int lgp, double base_coef,
-                                                 bool neg_ok, int lane) {
+                                                 const unsigned *__restrict__ smask, const int *__restrict__ smol,
+                                                 int nq, const PlaneSet &ps, int kind, int p0, int np, int lgp,
+                                                 double base_coef, bool neg_ok, int lane) {
   constexpr int NAR = K::n(AR), NT = NAR * K::n(AC);
   constexpr int ST[4] = {K::NCB * K::NCC * K::NCD, K::NCC * K::NCD, K::NCD, 1};
-  const int N = ps.nbc;
-  const int kind = ps.p[p0].kind, S = ps.kps[kind];
-  const double *__restrict__ xk = ps.kx[kind];
-  double *ak = ps.ka[kind];
+  const int S = ps.kps[kind];
   const int items = (nq * NT) << lgp;
   for (int it = lane; it < items; it += 64) {
     const int p = it & ((1 << lgp) - 1);
@@ -42,6 +39,10 @@ __device__ __forceinline__ void digest_term_tile(const double *__restrict__ sI, 
     const int mat = (int)((ps.mats >> (4 * slot)) & 15ull);
     if (!((smask[qi] >> mat) & 1u)) continue;
     const int *bf = sbf[qi];
+    const MolPlanes &mp = ps.nmol == 1 ? ps.m0 : ps.mols[smol[qi]];
+    const int N = mp.nbc;
+    const double *__restrict__ xk = mp.kx[kind];
+    double *ak = mp.ka[kind];
     const double *__restrict__ Iq = sI + qi * K::NC + t0 * ST[AR] + t1 * ST[AC];
     const double *__restrict__ X = xk + p + ((size_t)bf[XR] + (size_t)bf[XC] * N) * S;
     double s0 = 0.0, s1 = 0.0;
@@ -59,10 +60,11 @@ __device__ __forceinline__ void digest_term_tile(const double *__restrict__ sI, 
 
 // Every update of every plane kind for nq quartets: sI holds quartet q at
 // sI[q * NC] (degeneracy and 1/2 applied), sbf[q] its first Cartesian
-// functions in roles a, b, c, d, smask[q] the contractions it passed.
+// functions in roles a, b, c, d, smask[q] the contractions it passed and
+// smol[q] its molecule.
 template <class K>
-__device__ __forceinline__ void digest_tile(const double *sI, const int (*sbf)[4], const unsigned *smask, int nq,
-                                            const PlaneSet &ps, int lane) {
+__device__ __forceinline__ void digest_tile(const double *sI, const int (*sbf)[4], const unsigned *smask,
+                                            const int *smol, int nq, const PlaneSet &ps, int lane) {
   for (int kind = 0; kind < 4; ++kind) {
     const int np = ps.kcount[kind];
     if (np == 0) continue;
@@ -70,34 +72,34 @@ __device__ __forceinline__ void digest_tile(const double *sI, const int (*sbf)[4
     const int lgp = np <= 1 ? 0 : np <= 2 ? 1 : np <= 4 ? 2 : np <= 8 ? 3 : 4;
     switch (kind) {
       case kJHer:  // direct.hpp:1863-1883, real parts
-        digest_term_tile<K, 0, 1, 3, 2>(sI, sbf, smask, nq, ps, p0, np, lgp, 1.0, false, lane);
-        digest_term_tile<K, 3, 2, 0, 1>(sI, sbf, smask, nq, ps, p0, np, lgp, 1.0, false, lane);
+        digest_term_tile<K, 0, 1, 3, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 1.0, false, lane);
+        digest_term_tile<K, 3, 2, 0, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 1.0, false, lane);
         break;
       case kKHer:  // :1889-1917, 1/2 conj(X)
-        digest_term_tile<K, 0, 2, 3, 1>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, true, lane);
-        digest_term_tile<K, 3, 1, 0, 2>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, true, lane);
-        digest_term_tile<K, 3, 0, 1, 2>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, true, lane);
-        digest_term_tile<K, 1, 2, 3, 0>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, true, lane);
+        digest_term_tile<K, 0, 2, 3, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, true, lane);
+        digest_term_tile<K, 3, 1, 0, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, true, lane);
+        digest_term_tile<K, 3, 0, 1, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, true, lane);
+        digest_term_tile<K, 1, 2, 3, 0>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, true, lane);
         break;
       case kJNonHer:  // :1925-1960
-        digest_term_tile<K, 0, 1, 3, 2>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 0, 1, 2, 3>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 1, 0, 3, 2>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 1, 0, 2, 3>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 2, 3, 1, 0>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 2, 3, 0, 1>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 3, 2, 1, 0>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 3, 2, 0, 1>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 0, 1, 3, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 0, 1, 2, 3>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 1, 0, 3, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 1, 0, 2, 3>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 2, 3, 1, 0>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 2, 3, 0, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 3, 2, 1, 0>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 3, 2, 0, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
         break;
       default:  // kKNonHer, :1961-1998
-        digest_term_tile<K, 2, 0, 3, 1>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 3, 1, 2, 0>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 3, 0, 2, 1>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 2, 1, 3, 0>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 0, 2, 1, 3>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 1, 3, 0, 2>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 0, 3, 1, 2>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 1, 2, 0, 3>(sI, sbf, smask, nq, ps, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 2, 0, 3, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 3, 1, 2, 0>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 3, 0, 2, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 2, 1, 3, 0>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 0, 2, 1, 3>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 1, 3, 0, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 0, 3, 1, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 1, 2, 0, 3>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
         break;
     }
   }

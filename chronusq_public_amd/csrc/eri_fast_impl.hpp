@@ -148,6 +148,7 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
   __shared__ double s_I[4][QPW * K::NC];
   __shared__ int s_bf[4][QPW][4];
   __shared__ unsigned s_mask[4][QPW];
+  __shared__ int s_mol[4][QPW];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane & (G - 1), qw = lane / G;
   const long long q0 = ((long long)blockIdx.x * 4 + wv) * QPW;
@@ -234,10 +235,11 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
     s_bf[wv][qw][2] = kp.bfa;
     s_bf[wv][qw][3] = kp.bfb;
     s_mask[wv][qw] = valid ? (qt.ket_mask >> 24) : 0u;
+    s_mol[wv][qw] = bp.mol;
   }
   wave_sync();
   const long long left = a.nq - q0;
-  digest_tile<K>(s_I[wv], s_bf[wv], s_mask[wv], left < QPW ? (int)left : QPW, a.planes, lane);
+  digest_tile<K>(s_I[wv], s_bf[wv], s_mask[wv], s_mol[wv], left < QPW ? (int)left : QPW, a.planes, lane);
 }
 
 template <int LA, int LB, int LC, int LD, int G, class R>

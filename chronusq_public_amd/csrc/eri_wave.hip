@@ -29,6 +29,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
   __shared__ double s_I[kWavesPerBlock][K::NC];
   __shared__ int s_bf[kWavesPerBlock][1][4];
   __shared__ unsigned s_mask[kWavesPerBlock][1];
+  __shared__ int s_mol[kWavesPerBlock][1];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long long q = (long long)blockIdx.x * kWavesPerBlock + wv;
   if (q >= a.nq) return;  // uniform across the wave; no block barriers below
@@ -122,9 +123,10 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
     s_bf[wv][0][2] = kp.bfa;
     s_bf[wv][0][3] = kp.bfb;
     s_mask[wv][0] = qt.ket_mask >> 24;
+    s_mol[wv][0] = bp.mol;
   }
   wave_sync();
-  digest_tile<K>(sI, s_bf[wv], s_mask[wv], 1, a.planes, lane);
+  digest_tile<K>(sI, s_bf[wv], s_mask[wv], s_mol[wv], 1, a.planes, lane);
 }
 
 template <int LA, int LB, int LC, int LD>

@@ -171,7 +171,7 @@ __global__ __launch_bounds__(64) void eri_team_kernel(EriLaunch a) {
       __syncthreads();
       const int n[4] = {D.nca, D.ncb, D.ncc, D.ncd};
       const int bf[4] = {bp.bfa, bp.bfb, kp.bfa, kp.bfb};
-      digest_wave(s_I, n, bf, a.planes, qt.ket_mask >> 24, lane, 64);
+      digest_wave(s_I, n, bf, a.planes, qt.ket_mask >> 24, bp.mol, lane, 64);
     }
     __syncthreads();
   }
@@ -245,22 +245,23 @@ __global__ __launch_bounds__(256) void quartet_list_kernel(ListLaunch a) {
         if (a.cand_off[mid] <= ta) lo = mid; else hi = mid - 1;
       }
       bi = lo;
-      kj = (int)(ta - a.cand_off[lo]);
+      kj = a.ket_lo[bi] + (int)(ta - a.cand_off[lo]);
       const PairRec P = a.bra[bi], Q = a.ket[kj];
       const bool p_first = (P.s1 > Q.s1) || (P.s1 == Q.s1 && P.s2 >= Q.s2);
       const int s1 = p_first ? P.s1 : Q.s1, s2 = p_first ? P.s2 : Q.s2;
       const int s3 = p_first ? Q.s1 : P.s1, s4 = p_first ? Q.s2 : P.s2;
-      const int ns = a.ns;
+      const long long boff = a.mol_boff[P.mol];
+      const int ns = a.mol_ns[P.mol];
       if (!a.screen) {
         pass = true;
         mask = (1u << a.nmat) - 1u;
       } else {
         const double qq = P.Q * Q.Q;
         auto bmax = [&](const double *B, int ld, int m) {
-          double v = fmax(B[(s1 + (size_t)s2 * ns) * ld + m], fmax(B[(s1 + (size_t)s3 * ns) * ld + m],
-                                                                  B[(s2 + (size_t)s3 * ns) * ld + m]));
-          v = fmax(v, B[(s3 + (size_t)s4 * ns) * ld + m]);
-          v = fmax(v, fmax(B[(s1 + (size_t)s4 * ns) * ld + m], B[(s2 + (size_t)s4 * ns) * ld + m]));
+          auto at = [&](int x, int y) { return B[(boff + x + (size_t)y * ns) * ld + m]; };
+          double v = fmax(at(s1, s2), fmax(at(s1, s3), at(s2, s3)));
+          v = fmax(v, at(s3, s4));
+          v = fmax(v, fmax(at(s1, s4), at(s2, s4)));
           return v;
         };
         if (!(bmax(a.Bg, 1, 0) * qq < a.thresh)) {
@@ -288,7 +289,7 @@ __global__ __launch_bounds__(256) void quartet_list_kernel(ListLaunch a) {
 // the (s1, s2) block.
 // ---------------------------------------------------------------------------
 __global__ void block_norm_kernel(const double *X, int nmat, int cplx, int nb, int ns,
-                                  const int *sh_bf, const int *sh_nf, double *Bm) {
+                                  const int *sh_bf, int bf0, const int *sh_nf, double *Bm) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long tot = (long long)nmat * ns * ns;
   if (t >= tot) return;
@@ -302,7 +303,7 @@ __global__ void block_norm_kernel(const double *X, int nmat, int cplx, int nb, i
   for (int i = 0; i < sh_nf[s1]; ++i) {
     double row = 0.0;
     for (int j = 0; j < sh_nf[s2]; ++j) {
-      const size_t idx = (sh_bf[s1] + i) + (size_t)(sh_bf[s2] + j) * nb;
+      const size_t idx = (sh_bf[s1] - bf0 + i) + (size_t)(sh_bf[s2] - bf0 + j) * nb;
       row += cplx ? hypot(Xm[2 * idx], Xm[2 * idx + 1]) : fabs(Xm[idx]);
     }
     mx = fmax(mx, row);
@@ -366,7 +367,7 @@ __global__ void to_cart_kernel(CartMap M, const double *X, int cplx, double *Pre
     for (int nu = 0; nu < nfunc(M, sj); ++nu) {
       const double tj = tcoef(M, sj, cj, nu);
       if (tj == 0.0) continue;
-      const size_t idx = (M.sh_bf[si] + mu) + (size_t)(M.sh_bf[sj] + nu) * M.nb;
+      const size_t idx = (M.sh_bf[si] - M.bf0 + mu) + (size_t)(M.sh_bf[sj] - M.bf0 + nu) * M.nb;
       if (cplx) {
         re += ti * tj * X[2 * idx];
         im += ti * tj * X[2 * idx + 1];
@@ -394,8 +395,8 @@ __global__ void from_cart_kernel(CartMap M, const double *Are, const double *Aim
     for (int j = 0; j < ncn; ++j) {
       const double tj = tcoef(M, sn, j, cn);
       if (tj == 0.0) continue;
-      const size_t ij = (M.sh_bfc[sm] + i) + (size_t)(M.sh_bfc[sn] + j) * M.nbc;
-      const size_t ji = (M.sh_bfc[sn] + j) + (size_t)(M.sh_bfc[sm] + i) * M.nbc;
+      const size_t ij = (M.sh_bfc[sm] - M.bfc0 + i) + (size_t)(M.sh_bfc[sn] - M.bfc0 + j) * M.nbc;
+      const size_t ji = (M.sh_bfc[sn] - M.bfc0 + j) + (size_t)(M.sh_bfc[sm] - M.bfc0 + i) * M.nbc;
       a_re += ti * tj * Are[ij * ps];
       b_re += ti * tj * Are[ji * ps];
       if (Aim) {
@@ -418,6 +419,23 @@ __global__ void from_cart_kernel(CartMap M, const double *Are, const double *Aim
   } else {
     AX[t] += re;
   }
+}
+
+// max of each molecule's block of global block norms (one block per molecule)
+__global__ void mol_max_kernel(const double *Bg, const long long *mol_boff, const int *mol_ns, double *out) {
+  const int m = blockIdx.x;
+  const long long n = (long long)mol_ns[m] * mol_ns[m];
+  const double *B = Bg + mol_boff[m];
+  double v = 0.0;
+  for (long long i = threadIdx.x; i < n; i += blockDim.x) v = fmax(v, B[i]);
+  __shared__ double red[256];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[m] = red[0];
 }
 
 __global__ void zero_kernel(double *p, size_t n) {
@@ -458,11 +476,11 @@ hipError_t launch_quartet_list(const ListLaunch &a, hipStream_t s) {
 }
 
 hipError_t launch_block_norms(const double *X, int nmat, int is_complex, const int *her, int nb,
-                              int ns, const int *sh_bf, const int *sh_nf, double *Bm, double *Bg,
+                              int ns, const int *sh_bf, int bf0, const int *sh_nf, double *Bm, double *Bg,
                               hipStream_t s) {
   const long long n1 = (long long)nmat * ns * ns;
   hipLaunchKernelGGL(block_norm_kernel, dim3(nblocks(n1, 256)), dim3(256), 0, s, X, nmat,
-                     is_complex, nb, ns, sh_bf, sh_nf, Bm);
+                     is_complex, nb, ns, sh_bf, bf0, sh_nf, Bm);
   hipLaunchKernelGGL(block_norm_symm_kernel, dim3(nblocks(n1, 256)), dim3(256), 0, s, nmat, ns,
                      her, Bm);
   hipLaunchKernelGGL(block_norm_finish_kernel, dim3(nblocks((long long)ns * ns, 256)), dim3(256),
@@ -481,6 +499,13 @@ hipError_t launch_from_cart(const CartMap &m, const double *are, const double *a
                             int is_complex, double *AX, hipStream_t s) {
   hipLaunchKernelGGL(from_cart_kernel, dim3(nblocks((long long)m.nb * m.nb, 256)), dim3(256), 0,
                      s, m, are, aim, ps, her, is_complex, AX);
+  return hipGetLastError();
+}
+
+hipError_t launch_mol_max(const double *Bg, const long long *mol_boff, const int *mol_ns, int nmol,
+                          double *out, hipStream_t s) {
+  if (nmol <= 0) return hipSuccess;
+  hipLaunchKernelGGL(mol_max_kernel, dim3(nmol), dim3(256), 0, s, Bg, mol_boff, mol_ns, out);
   return hipGetLastError();
 }
 

@@ -41,11 +41,16 @@ struct ListLaunch {
   int same_class;
   const long long *cand_off;  // nbra + 1 prefix of candidate counts (from cand_base)
   long long cand_base;        // cand_off[0]: the list covers candidates cand_base + [0, ncand)
+  const int *ket_lo;          // per bra: first ket of its molecule in the ket class
   int nbra;
   long long ncand;
-  const double *Bg;           // NS x NS global block norms
-  const double *Bm;           // per-matrix norms, Bm[(s1 + s2 NS) nmat + m]
-  int nmat, ns;
+  // block norms of molecule m: Bg[boff + s1 + s2 ns], per matrix
+  // Bm[(boff + s1 + s2 ns) nmat + k], boff = mol_boff[m], ns = mol_ns[m]
+  const double *Bg;
+  const double *Bm;
+  const long long *mol_boff;
+  const int *mol_ns;
+  int nmat;
   double thresh;
   int screen;
   Quartet *out;
@@ -65,8 +70,11 @@ hipError_t launch_quartet_list(const ListLaunch &a, hipStream_t s);
 // matrices; symmetrised for non-Hermitian ones; global max over matrices.
 hipError_t launch_block_norms(const double *X, int nmat, int is_complex,
                               const int *her, int nb, int ns,
-                              const int *sh_bf, const int *sh_nf,
+                              const int *sh_bf, int bf0, const int *sh_nf,
                               double *Bm, double *Bg, hipStream_t s);
+// out[m] = max of molecule m's global block norms
+hipError_t launch_mol_max(const double *Bg, const long long *mol_boff, const int *mol_ns, int nmol,
+                          double *out, hipStream_t s);
 
 // Cartesian <-> pure transforms of whole matrices.
 struct CartMap {
@@ -80,6 +88,10 @@ struct CartMap {
   const int *bf_shell;    // pure-or-Cartesian function -> shell
   const int *bf_comp;     // pure-or-Cartesian function -> index within shell
   int nb, nbc, ns;
+  // one molecule of a batch: the function tables above start at its first
+  // function, and bf0 / bfc0 (its first function / Cartesian function) are
+  // subtracted from the global sh_bf / sh_bfc values
+  int bf0, bfc0;
 };
 // X (NB x NB, real or interleaved complex) -> real planes of T X T^T, written
 // with interleave stride ps (plane pointers already offset to their slot)

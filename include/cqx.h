@@ -82,6 +82,15 @@ int cqx_basis_shell(const cqx_basis *b, int s, int *l, int *pure,
 int cqx_basis_shell_prims(const cqx_basis *b, int s, double *alpha,
                           double *coef, double *origin);
 
+/* Several molecules' shell sets side by side, for a batched context: a
+ * context created on the result evaluates every molecule's quartets in the
+ * same launches (one per class pair and precision), never pairing shells of
+ * different molecules.  cqx_two_body_contract on it takes nmol x n
+ * contractions, molecule-major (molecule m's n at [m n, (m + 1) n), NB_m x
+ * NB_m each, the same types and HER flags for every molecule).  No
+ * counterpart in the reference, which contracts one molecule per call. */
+int cqx_basis_batch(int nmol, const cqx_basis *const *parts, cqx_basis **out);
+
 /* One-electron integrals of a basis into out (column-major NB x NB, pure
  * functions where the shell is pure); host computation, no device needed.
  * Stands in for OnePInts::computeAOInts with libint2's overlap, kinetic
