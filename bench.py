@@ -37,6 +37,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Fock builds/sec, mixed vs FP64 at |ΔE|≤1µEh; SCF p50/p99 latency"
+COULOMB_, EXCHANGE_ = 0, 1
 FP64_VECTOR_PEAK = 78.6  # TFLOP/s, MI355X
 FP32_VECTOR_PEAK = 157.3  # TFLOP/s, MI355X
 
@@ -211,6 +212,97 @@ def serving(njobs, max_inflight=8, cut=1e-4, seed=0):
     }
 
 
+def batched_comparison(nmol, steps, cut=1e-4, seed=0):
+    """Batched uneven-size Fock builds: nmol water clusters (H2O)_n, n ~
+    U{2..24} from seed, in cubes of side 12 (n/16)^(1/3) A, cc-pVDZ,
+    D = C C^T with C NB x 5n, N(0, 1/sqrt NB).  The plain FP64 GPU build is
+    one context per molecule, contracted one after the other; the batched
+    build is one context over all molecules (every class pair one launch for
+    the batch) with mixed precision (FP32 when Q_ab Q_cd < cut).  A "build"
+    is one Fock build of one molecule."""
+    from chronusq_public_amd import cqx
+    from chronusq_public_amd.chem import water_cluster
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = [int(n) for n in rng.integers(2, 25, size=nmol)]
+    progress(f"batched: {nmol} clusters, sizes {min(sizes)}..{max(sizes)}")
+    bases, dens = [], []
+    for i, n in enumerate(sizes):
+        b = cqx.BasisSet("cc-pVDZ", water_cluster(n, 12.0 * (n / 16.0) ** (1.0 / 3.0), seed=2000 + i))
+        C = rng.normal(0, 1 / np.sqrt(b.nBasis), size=(b.nBasis, 5 * n))
+        D = C @ C.T
+        bases.append(b)
+        dens.append(0.5 * (D + D.T))
+
+    def make(tpi, nb, D):
+        X = cqx.DeviceMatrix(tpi, nb).upload(D)
+        J, K = cqx.DeviceMatrix(tpi, nb), cqx.DeviceMatrix(tpi, nb)
+        return X, J, K
+
+    # plain FP64: one context per molecule, sequential
+    singles = []
+    for b, D in zip(bases, dens):
+        tpi = cqx.DirectTPI(b, 1e-12, device=0)
+        X, J, K = make(tpi, b.nBasis, D)
+        singles.append((tpi, cqx.GTODirectTPIContraction(tpi), X, J, K))
+
+    def round_single():
+        for tpi, gto, X, J, K in singles:
+            J.zero()
+            K.zero()
+            gto.twoBodyContract([cqx.TwoBodyContraction(X, J, True, COULOMB_),
+                                 cqx.TwoBodyContraction(X, K, True, EXCHANGE_)])
+
+    round_single()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        round_single()
+    t_single = (time.perf_counter() - t0) / steps
+    ref = [(J.download(), K.download()) for _, _, _, J, K in singles]
+    # batched, mixed precision
+    progress("batched: one context over all molecules, mixed precision")
+    bb = cqx.BasisSet.batch(bases)
+    tpi = cqx.DirectTPI(bb, 1e-12, device=0)
+    tpi.set_option("fp32_cut", cut)
+    gto = cqx.GTODirectTPIContraction(tpi)
+    mats = [make(tpi, b.nBasis, D) for b, D in zip(bases, dens)]
+    cont = []
+    for X, J, K in mats:
+        cont += [cqx.TwoBodyContraction(X, J, True, COULOMB_), cqx.TwoBodyContraction(X, K, True, EXCHANGE_)]
+
+    def round_batch():
+        for _, J, K in mats:
+            J.zero()
+            K.zero()
+        gto.twoBodyContract(cont)
+
+    round_batch()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        round_batch()
+    t_batch = (time.perf_counter() - t0) / steps
+    st = tpi.last_stats()
+    dF, dE = 0.0, 0.0
+    for (_, J, K), (J0, K0), D in zip(mats, ref, dens):
+        dG = (2 * J.download() - K.download()) - (2 * J0 - K0)
+        dF = max(dF, float(np.abs(dG).max()))
+        dE = max(dE, float(abs(0.25 * np.sum(D * dG))))
+    return {
+        "workload": ("RHF J/K of %d water clusters (H2O)_n, n ~ U{2..24} (seed %d), cc-pVDZ, "
+                     "D = C C^T with C NB x 5n; batched = one context and one launch per class pair for "
+                     "all molecules, FP32 when Q_ab Q_cd < %g" % (nmol, seed, cut)),
+        "molecules": nmol,
+        "fp64_single_builds_per_s": nmol / t_single,
+        "batched_mixed_builds_per_s": nmol / t_batch,
+        "speedup": t_single / t_batch,
+        "batched_ms_eri": st["ms_eri"],
+        "batched_ms_total": st["ms_total"],
+        "quartets": st["n_quartets"],
+        "quartets_fp32": st["n_quartets_f32"],
+        "max_abs_dF": dF,
+        "max_abs_dE": dE,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,6 +310,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="ghf16")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-mols", type=int, default=256,
+                    help="molecules of the batched-build measurement (0 skips it)")
     ap.add_argument("--serve-jobs", type=int, default=24,
                     help="SCF jobs of the serving measurement (0 skips it)")
     ap.add_argument("--mixed-steps", type=int, default=3,
@@ -290,6 +384,8 @@ def main():
     }
     if args.mixed_steps > 0:
         line["mixed"] = mixed_comparison(args.mixed_steps, 1)
+    if args.batch_mols > 0:
+        line["batched"] = batched_comparison(args.batch_mols, 2)
     if args.serve_jobs > 0:
         line["serving"] = serving(args.serve_jobs)
     if not args.no_cpu:

@@ -118,7 +118,8 @@ struct cqx_ctx {
   DevBuf<double> d_X, d_AX, d_Bm, d_Bg, d_planes, d_acc;
   DevBuf<int> d_her;
   DevBuf<long long> d_cand, d_mol_boff;
-  DevBuf<int> d_mol_ns, d_ket_lo;
+  DevBuf<int> d_mol_ns, d_ket_lo, d_seg;
+  DevBuf<char> d_scan_tmp;
   DevBuf<double> d_molmax;
   DevBuf<cqx::MolPlanes> d_molplanes;
   DevBuf<Quartet> d_quart;
@@ -408,6 +409,9 @@ void build_pairs(cqx_ctx *c) {
       sg[m] = (int)i;
     }
   }
+  std::vector<int> segs;
+  for (int k = 0; k < kNumClass; ++k) segs.insert(segs.end(), c->seg[k].begin(), c->seg[k].end());
+  c->d_seg.upload(segs, c->stream);
   c->d_prims.upload(c->prims, c->stream);
   for (int k = 0; k < kNumClass; ++k) c->d_pairs[k].upload(c->cls[k], c->stream);
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -741,8 +745,6 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
                                        c->d_Bm.p + (size_t)c->mol_boff[m] * nmat, c->d_Bg.p + c->mol_boff[m], s));
       c->d_molmax.ensure(nmol);
       HIPCHK(cqx::launch_mol_max(c->d_Bg.p, c->d_mol_boff.p, c->d_mol_ns.p, nmol, c->d_molmax.p, s));
-      std::vector<double> bmax_mol(nmol);
-      HIPCHK(hipMemcpyAsync(bmax_mol.data(), c->d_molmax.p, sizeof(double) * nmol, hipMemcpyDeviceToHost, s));
       cqx::PlaneSet ps{};
       // planes (one per real part), ordered by kind; slot = position, so the
       // planes that one digestion term visits sit in consecutive slots
@@ -837,53 +839,62 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
       }
       int nJ = 0, nK = 0;
       for (int j = 0; j < nmat; ++j) (L[j].type == CQX_COULOMB ? nJ : nK)++;
-      HIPCHK(hipStreamSynchronize(s));
       // candidate lists per class pair (bra class >= ket class): bra i of
-      // molecule m meets a prefix of molecule m's kets (Q descending)
-      struct CP { int cb, ck; long long ncand; size_t coff; std::vector<long long> off; std::vector<int> klo; };
+      // molecule m meets a prefix of molecule m's kets (Q descending); the
+      // counts and offsets are computed on the device
+      struct CP { int cb, ck, nbra; long long ncand; size_t coff; std::vector<long long> off; };
       std::vector<CP> cps;
-      long long total = 0;
       size_t cand_total = 0;
       for (int cb = 0; cb < kNumClass; ++cb)
         for (int ck = 0; ck <= cb; ++ck) {
-          const auto &vb = c->cls[cb], &vk = c->cls[ck];
-          if (vb.empty() || vk.empty()) continue;
-          CP cp;
-          cp.cb = cb;
-          cp.ck = ck;
-          cp.off.resize(vb.size() + 1);
-          cp.klo.resize(vb.size());
-          cp.off[0] = 0;
-          for (size_t i = 0; i < vb.size(); ++i) {
-            const int m = vb[i].mol;
-            const long long lo = c->seg[ck][m], hi = c->seg[ck][m + 1];
-            long long jmax = hi - lo;
-            const double bmax = bmax_mol[m];
-            if (screen && jmax > 0) {
-              if (bmax * vb[i].Q * vk[lo].Q < c->thresh) {
-                jmax = 0;
-              } else {
-                // largest prefix with Q_i Q_j bmax >= thresh (Q sorted descending)
-                const double need = c->thresh / (bmax * vb[i].Q);
-                long long a = lo, b = hi;
-                while (a < b) {
-                  long long mid = (a + b) / 2;
-                  if (vk[mid].Q >= need) a = mid + 1; else b = mid;
-                }
-                jmax = a - lo;
-              }
-            }
-            if (cb == ck) jmax = std::min<long long>(jmax, (long long)i - lo + 1);
-            cp.klo[i] = (int)lo;
-            cp.off[i + 1] = cp.off[i] + std::max<long long>(jmax, 0);
-          }
-          cp.ncand = cp.off.back();
-          if (cp.ncand == 0) continue;
-          cp.coff = cand_total;
-          cand_total += cp.off.size();
-          total += cp.ncand;
+          if (c->cls[cb].empty() || c->cls[ck].empty()) continue;
+          CP cp{cb, ck, (int)c->cls[cb].size(), 0, cand_total, {}};
+          cand_total += (size_t)cp.nbra + 1;
           cps.push_back(std::move(cp));
         }
+      c->d_cand.ensure(std::max<size_t>(cand_total, 1));
+      c->d_ket_lo.ensure(std::max<size_t>(cand_total, 1));
+      size_t scan_bytes = 0;
+      for (auto &cp : cps) {
+        cqx::CandLaunch CL{};
+        CL.nbra = cp.nbra;
+        CL.off = c->d_cand.p + cp.coff;
+        size_t b = 0;
+        HIPCHK(cqx::launch_candidates(CL, nullptr, b, s));
+        scan_bytes = std::max(scan_bytes, b);
+      }
+      c->d_scan_tmp.ensure(std::max<size_t>(scan_bytes, 1));
+      std::vector<long long> totals(cps.size());
+      for (size_t k = 0; k < cps.size(); ++k) {
+        auto &cp = cps[k];
+        cqx::CandLaunch CL{};
+        CL.bra = c->d_pairs[cp.cb].p;
+        CL.ket = c->d_pairs[cp.ck].p;
+        CL.nbra = cp.nbra;
+        CL.bra_base = 0;
+        CL.seg = c->d_seg.p + (size_t)cp.ck * (nmol + 1);
+        CL.molmax = c->d_molmax.p;
+        CL.thresh = c->thresh;
+        CL.screen = screen;
+        CL.same_class = cp.cb == cp.ck;
+        CL.off = c->d_cand.p + cp.coff;
+        CL.klo = c->d_ket_lo.p + cp.coff;
+        size_t b = c->d_scan_tmp.n;
+        HIPCHK(cqx::launch_candidates(CL, c->d_scan_tmp.p, b, s));
+        HIPCHK(hipMemcpyAsync(&totals[k], c->d_cand.p + cp.coff + cp.nbra, sizeof(long long),
+                              hipMemcpyDeviceToHost, s));
+      }
+      HIPCHK(hipStreamSynchronize(s));
+      long long total = 0;
+      for (size_t k = 0; k < cps.size(); ++k) {
+        cps[k].ncand = totals[k];
+        total += totals[k];
+        if (totals[k] > c->list_cap) {  // split into bra ranges on the host
+          cps[k].off.resize((size_t)cps[k].nbra + 1);
+          HIPCHK(hipMemcpy(cps[k].off.data(), c->d_cand.p + cps[k].coff, sizeof(long long) * cps[k].off.size(),
+                           hipMemcpyDeviceToHost));
+        }
+      }
       c->stats.n_candidates += total;
       // kernel family of a class: 0 lane-group, 1 wave-per-quartet, 2 generic
       auto route = [&](int la, int lb, int lc, int ld) {
@@ -900,44 +911,31 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
       struct Piece { int cp; int i0, i1; long long base, ncand, seg; };
       std::vector<std::vector<Piece>> batches(1);
       long long in_batch = 0, cap_need = 1;
+      auto add_piece = [&](int k, int i0, int i1, long long base, long long n) {
+        if (n <= 0) return;
+        if (in_batch > 0 && in_batch + n > c->list_cap) {
+          batches.emplace_back();
+          in_batch = 0;
+        }
+        batches.back().push_back({k, i0, i1, base, n, in_batch});
+        in_batch += n;
+        cap_need = std::max(cap_need, in_batch);
+      };
       for (int k = 0; k < (int)cps.size(); ++k) {
         const auto &cp = cps[k];
-        const int nbra = (int)cp.off.size() - 1;
+        if (cp.off.empty()) {  // the whole class pair in one piece
+          add_piece(k, 0, cp.nbra, 0, cp.ncand);
+          continue;
+        }
         int i0 = 0;
-        while (i0 < nbra) {
+        while (i0 < cp.nbra) {
           int i1 = i0 + 1;  // at least one bra pair per piece
-          while (i1 < nbra && cp.off[i1 + 1] - cp.off[i0] <= c->list_cap) ++i1;
-          const long long n = cp.off[i1] - cp.off[i0];
-          if (n > 0) {
-            if (in_batch > 0 && in_batch + n > c->list_cap) {
-              batches.emplace_back();
-              in_batch = 0;
-            }
-            batches.back().push_back({k, i0, i1, cp.off[i0], n, in_batch});
-            in_batch += n;
-            cap_need = std::max(cap_need, in_batch);
-          }
+          while (i1 < cp.nbra && cp.off[i1 + 1] - cp.off[i0] <= c->list_cap) ++i1;
+          add_piece(k, i0, i1, cp.off[i0], cp.off[i1] - cp.off[i0]);
           i0 = i1;
         }
       }
       c->d_quart.ensure((size_t)cap_need * (mixed ? 2 : 1));
-      c->d_cand.ensure(std::max<size_t>(cand_total, 1));
-      c->d_ket_lo.ensure(std::max<size_t>(cand_total, 1));
-      std::vector<long long> cand_all;
-      std::vector<int> klo_all;
-      cand_all.reserve(cand_total);
-      klo_all.reserve(cand_total);
-      for (auto &cp : cps) {
-        cand_all.insert(cand_all.end(), cp.off.begin(), cp.off.end());
-        klo_all.insert(klo_all.end(), cp.klo.begin(), cp.klo.end());
-        klo_all.push_back(0);  // keep klo aligned with off (nbra + 1 entries)
-      }
-      if (!cand_all.empty()) {
-        HIPCHK(hipMemcpyAsync(c->d_cand.p, cand_all.data(), sizeof(long long) * cand_all.size(),
-                              hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(c->d_ket_lo.p, klo_all.data(), sizeof(int) * klo_all.size(),
-                              hipMemcpyHostToDevice, s));
-      }
       for (const auto &batch : batches) {
         const size_t np = batch.size();
         if (!np) continue;

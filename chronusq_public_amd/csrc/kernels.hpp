@@ -66,6 +66,22 @@ struct ListLaunch {
 };
 hipError_t launch_quartet_list(const ListLaunch &a, hipStream_t s);
 
+// Candidate kets of every bra pair of one class pair: off[i + 1] - off[i]
+// kets starting at klo[i] (the bra's molecule's first ket), off an
+// inclusive prefix (off[0] = 0).  tmp == nullptr queries tmp_bytes.
+struct CandLaunch {
+  const PairRec *bra, *ket;
+  int nbra;
+  int bra_base;            // index of bra[0] in its class array (same-class limit j <= i)
+  const int *seg;          // ket class: molecule m's kets [seg[m], seg[m + 1])
+  const double *molmax;    // per molecule: largest shell-block norm
+  double thresh;
+  int screen, same_class;
+  long long *off;          // nbra + 1
+  int *klo;                // nbra
+};
+hipError_t launch_candidates(const CandLaunch &a, void *tmp, size_t &tmp_bytes, hipStream_t s);
+
 // Shell-block infinity norms (max absolute row sum) of nmat spherical
 // matrices; symmetrised for non-Hermitian ones; global max over matrices.
 hipError_t launch_block_norms(const double *X, int nmat, int is_complex,
