@@ -107,6 +107,7 @@ struct cqx_ctx {
   std::vector<double> schwarz;                 // NS x NS
 
   DevBuf<PairRec> d_pairs[kNumClass];
+  DevBuf<cqx::PairScr> d_scr[kNumClass];
   DevBuf<PrimPair> d_prims;
   DevBuf<double> d_rys, d_sph;
   DevBuf<float> d_rys32;
@@ -413,7 +414,21 @@ void build_pairs(cqx_ctx *c) {
   for (int k = 0; k < kNumClass; ++k) segs.insert(segs.end(), c->seg[k].begin(), c->seg[k].end());
   c->d_seg.upload(segs, c->stream);
   c->d_prims.upload(c->prims, c->stream);
-  for (int k = 0; k < kNumClass; ++k) c->d_pairs[k].upload(c->cls[k], c->stream);
+  for (int k = 0; k < kNumClass; ++k) {
+    c->d_pairs[k].upload(c->cls[k], c->stream);
+    std::vector<cqx::PairScr> scr;
+    for (const auto &r : c->cls[k]) {
+      if (r.s1 > 0xFFFF || r.s2 > 0xFFFF || r.kk > 0xFFFF || r.mol > 0xFFFF)
+        throw std::runtime_error("shell, contraction or molecule index beyond 16 bits");
+      cqx::PairScr q;
+      q.s12 = (uint32_t)r.s1 | ((uint32_t)r.s2 << 16);
+      q.kk = (uint16_t)r.kk;
+      q.mol = (uint16_t)r.mol;
+      q.Q = r.Q;
+      scr.push_back(q);
+    }
+    c->d_scr[k].upload(scr, c->stream);
+  }
   HIPCHK(hipStreamSynchronize(c->stream));
 }
 
@@ -949,8 +964,8 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           const Piece &pc = batch[k];
           const auto &cp = cps[pc.cp];
           cqx::ListLaunch LL{};
-          LL.bra = c->d_pairs[cp.cb].p + pc.i0;
-          LL.ket = c->d_pairs[cp.ck].p;
+          LL.bra = c->d_scr[cp.cb].p + pc.i0;
+          LL.ket = c->d_scr[cp.ck].p;
           LL.same_class = cp.cb == cp.ck;
           LL.cand_off = c->d_cand.p + cp.coff + pc.i0;
           LL.cand_base = pc.base;

@@ -23,6 +23,16 @@ struct PairRec {
   int mol;         // molecule of a batched context (0 for one molecule)
 };
 
+// What the quartet screen reads of a shell pair, 16 bytes: shells s1, s2
+// (numbered within the molecule) in the low and high 16 bits of s12, the
+// contraction-length product Ka Kb, the molecule and the Schwarz bound.
+struct PairScr {
+  uint32_t s12;
+  uint16_t kk;
+  uint16_t mol;
+  double Q;
+};
+
 // One surviving primitive pair of a shell pair (kernel orientation).
 struct PrimPair {
   double p;        // alpha_a + alpha_b

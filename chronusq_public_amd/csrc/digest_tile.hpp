@@ -19,11 +19,12 @@
 
 namespace cqx {
 
-template <class K, int AR, int AC, int XR, int XC>
-__device__ __forceinline__ void digest_term_tile(const double *__restrict__ sI, const int (*__restrict__ sbf)[4],
-                                                 const unsigned *__restrict__ smask, const int *__restrict__ smol,
-                                                 int nq, const PlaneSet &ps, int kind, int p0, int np, int lgp,
-                                                 double base_coef, bool neg_ok, int lane) {
+// ONE: a one-molecule launch, whose planes sit in the kernel arguments
+template <class K, int AR, int AC, int XR, int XC, bool ONE>
+__device__ __forceinline__ void digest_term_items(const double *__restrict__ sI, const int (*__restrict__ sbf)[4],
+                                                  const unsigned *__restrict__ smask, const int *__restrict__ smol,
+                                                  int nq, const PlaneSet &ps, int kind, int p0, int np, int lgp,
+                                                  double base_coef, bool neg_ok, int lane) {
   constexpr int NAR = K::n(AR), NT = NAR * K::n(AC);
   constexpr int ST[4] = {K::NCB * K::NCC * K::NCD, K::NCC * K::NCD, K::NCD, 1};
   const int S = ps.kps[kind];
@@ -39,7 +40,7 @@ __device__ __forceinline__ void digest_term_tile(const double *__restrict__ sI, 
     const int mat = (int)((ps.mats >> (4 * slot)) & 15ull);
     if (!((smask[qi] >> mat) & 1u)) continue;
     const int *bf = sbf[qi];
-    const MolPlanes &mp = ps.nmol == 1 ? ps.m0 : ps.mols[smol[qi]];
+    const MolPlanes &mp = ONE ? ps.m0 : ps.mols[smol[qi]];
     const int N = mp.nbc;
     const double *__restrict__ xk = mp.kx[kind];
     double *ak = mp.ka[kind];
@@ -56,6 +57,19 @@ __device__ __forceinline__ void digest_term_tile(const double *__restrict__ sI, 
     const double coef = (neg_ok && ((ps.negs >> slot) & 1u)) ? -base_coef : base_coef;
     unsafeAtomicAdd(ak + p + ((size_t)(bf[AR] + t0) + (size_t)(bf[AC] + t1) * N) * S, coef * (s0 + s1));
   }
+}
+
+template <class K, int AR, int AC, int XR, int XC>
+__device__ __forceinline__ void digest_term_tile(const double *__restrict__ sI, const int (*__restrict__ sbf)[4],
+                                                 const unsigned *__restrict__ smask, const int *__restrict__ smol,
+                                                 int nq, const PlaneSet &ps, int kind, int p0, int np, int lgp,
+                                                 double base_coef, bool neg_ok, int lane) {
+  if (ps.nmol == 1)
+    digest_term_items<K, AR, AC, XR, XC, true>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, base_coef, neg_ok,
+                                               lane);
+  else
+    digest_term_items<K, AR, AC, XR, XC, false>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, base_coef,
+                                                neg_ok, lane);
 }
 
 // Every update of every plane kind for nq quartets: sI holds quartet q at

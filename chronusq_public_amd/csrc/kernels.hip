@@ -185,13 +185,12 @@ __global__ __launch_bounds__(64) void eri_team_kernel(EriLaunch a) {
 // ---------------------------------------------------------------------------
 // Wave-aggregated append of the lanes with take set: one atomic per wave for
 // the slots and one for the sum of Ka Kb Kc Kd (the primitive-quartet bound).
-__device__ __forceinline__ void append_wave(bool take, int bi, int kj, unsigned mask,
-                                            const PairRec *bra, const PairRec *ket, Quartet *out,
-                                            unsigned long long *count, double *prim_sum,
-                                            long long cap, int lane) {
+__device__ __forceinline__ void append_wave(bool take, int bi, int kj, unsigned mask, double kk4, Quartet *out,
+                                            unsigned long long *count, double *prim_sum, long long cap,
+                                            int lane) {
   const unsigned long long bal = __ballot(take);
   if (!bal) return;
-  double pk = take ? (double)bra[bi].kk * (double)ket[kj].kk : 0.0;
+  double pk = take ? kk4 : 0.0;
   for (int off = 32; off > 0; off >>= 1) pk += __shfl_xor(pk, off);
   const int leader = __ffsll((long long)bal) - 1;
   unsigned long long start = 0;
@@ -221,6 +220,7 @@ __global__ __launch_bounds__(256) void quartet_list_kernel(ListLaunch a) {
     bool pass = false;
     unsigned mask = 0;
     int bi = 0, kj = 0;
+    double qq = 0.0, kk4 = 0.0;
     // the block's candidates span bra indices [s_lo, s_hi]: two searches per
     // block, then each thread searches that (usually short) range
     if (threadIdx.x == 0 || threadIdx.x == blockDim.x - 1) {
@@ -246,17 +246,20 @@ __global__ __launch_bounds__(256) void quartet_list_kernel(ListLaunch a) {
       }
       bi = lo;
       kj = a.ket_lo[bi] + (int)(ta - a.cand_off[lo]);
-      const PairRec P = a.bra[bi], Q = a.ket[kj];
-      const bool p_first = (P.s1 > Q.s1) || (P.s1 == Q.s1 && P.s2 >= Q.s2);
-      const int s1 = p_first ? P.s1 : Q.s1, s2 = p_first ? P.s2 : Q.s2;
-      const int s3 = p_first ? Q.s1 : P.s1, s4 = p_first ? Q.s2 : P.s2;
-      const long long boff = a.mol_boff[P.mol];
-      const int ns = a.mol_ns[P.mol];
+      const PairScr P = a.bra[bi], Q = a.ket[kj];
+      const int ps1 = (int)(P.s12 & 0xFFFFu), ps2 = (int)(P.s12 >> 16);
+      const int qs1 = (int)(Q.s12 & 0xFFFFu), qs2 = (int)(Q.s12 >> 16);
+      const bool p_first = (ps1 > qs1) || (ps1 == qs1 && ps2 >= qs2);
+      const int s1 = p_first ? ps1 : qs1, s2 = p_first ? ps2 : qs2;
+      const int s3 = p_first ? qs1 : ps1, s4 = p_first ? qs2 : ps2;
+      qq = P.Q * Q.Q;
+      kk4 = (double)P.kk * (double)Q.kk;
       if (!a.screen) {
         pass = true;
         mask = (1u << a.nmat) - 1u;
       } else {
-        const double qq = P.Q * Q.Q;
+        const long long boff = a.mol_boff[P.mol];
+        const int ns = a.mol_ns[P.mol];
         auto bmax = [&](const double *B, int ld, int m) {
           auto at = [&](int x, int y) { return B[(boff + x + (size_t)y * ns) * ld + m]; };
           double v = fmax(at(s1, s2), fmax(at(s1, s3), at(s2, s3)));
@@ -277,10 +280,9 @@ __global__ __launch_bounds__(256) void quartet_list_kernel(ListLaunch a) {
       }
     }
     // precision split on the Schwarz product (the FP32 list when below cut32)
-    const bool lo32 = pass && a.cut32 > 0.0 && a.bra[bi].Q * a.ket[kj].Q < a.cut32;
-    append_wave(pass && !lo32, bi, kj, mask, a.bra, a.ket, a.out, a.count, a.prim_sum, a.cap, lane);
-    if (a.cut32 > 0.0)
-      append_wave(lo32, bi, kj, mask, a.bra, a.ket, a.out32, a.count32, a.prim_sum32, a.cap, lane);
+    const bool lo32 = pass && a.cut32 > 0.0 && qq < a.cut32;
+    append_wave(pass && !lo32, bi, kj, mask, kk4, a.out, a.count, a.prim_sum, a.cap, lane);
+    if (a.cut32 > 0.0) append_wave(lo32, bi, kj, mask, kk4, a.out32, a.count32, a.prim_sum32, a.cap, lane);
   }
 }
 

@@ -37,7 +37,7 @@ hipError_t launch_eri_wave(const EriLaunch &a, hipStream_t s);
 bool eri_wave_available(int la, int lb, int lc, int ld);
 
 struct ListLaunch {
-  const PairRec *bra, *ket;
+  const PairScr *bra, *ket;   // screening records of the bra / ket classes
   int same_class;
   const long long *cand_off;  // nbra + 1 prefix of candidate counts (from cand_base)
   long long cand_base;        // cand_off[0]: the list covers candidates cand_base + [0, ncand)
