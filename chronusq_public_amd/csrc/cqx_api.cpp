@@ -108,6 +108,7 @@ struct cqx_ctx {
 
   DevBuf<PairRec> d_pairs[kNumClass];
   DevBuf<cqx::PairScr> d_scr[kNumClass];
+  DevBuf<double> d_pn[kNumClass];  // per build: each pair's own block norms
   DevBuf<PrimPair> d_prims;
   DevBuf<double> d_rys, d_sph;
   DevBuf<float> d_rys32;
@@ -760,6 +761,13 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
                                        c->d_Bm.p + (size_t)c->mol_boff[m] * nmat, c->d_Bg.p + c->mol_boff[m], s));
       c->d_molmax.ensure(nmol);
       HIPCHK(cqx::launch_mol_max(c->d_Bg.p, c->d_mol_boff.p, c->d_mol_ns.p, nmol, c->d_molmax.p, s));
+      for (int k = 0; k < kNumClass; ++k) {
+        const int np_k = (int)c->cls[k].size();
+        if (!np_k) continue;
+        c->d_pn[k].ensure((size_t)np_k * (nmat + 1));
+        HIPCHK(cqx::launch_pair_norms(c->d_scr[k].p, np_k, c->d_Bg.p, c->d_Bm.p, c->d_mol_boff.p, c->d_mol_ns.p,
+                                      nmat, c->d_pn[k].p, s));
+      }
       cqx::PlaneSet ps{};
       // planes (one per real part), ordered by kind; slot = position, so the
       // planes that one digestion term visits sit in consecutive slots
@@ -966,6 +974,8 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           cqx::ListLaunch LL{};
           LL.bra = c->d_scr[cp.cb].p + pc.i0;
           LL.ket = c->d_scr[cp.ck].p;
+          LL.bra_pn = c->d_pn[cp.cb].p + (size_t)pc.i0 * (nmat + 1);
+          LL.ket_pn = c->d_pn[cp.ck].p;
           LL.same_class = cp.cb == cp.ck;
           LL.cand_off = c->d_cand.p + cp.coff + pc.i0;
           LL.cand_base = pc.base;

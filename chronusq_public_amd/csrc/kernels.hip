@@ -183,38 +183,20 @@ __global__ __launch_bounds__(64) void eri_team_kernel(EriLaunch a) {
 // labels (direct.hpp:1689-1789); survivors appended to a list with the mask
 // of contractions that pass their own separated screen.
 // ---------------------------------------------------------------------------
-// Wave-aggregated append of the lanes with take set: one atomic per wave for
-// the slots and one for the sum of Ka Kb Kc Kd (the primitive-quartet bound).
-__device__ __forceinline__ void append_wave(bool take, int bi, int kj, unsigned mask, double kk4, Quartet *out,
-                                            unsigned long long *count, double *prim_sum, long long cap,
-                                            int lane) {
-  const unsigned long long bal = __ballot(take);
-  if (!bal) return;
-  double pk = take ? kk4 : 0.0;
-  for (int off = 32; off > 0; off >>= 1) pk += __shfl_xor(pk, off);
-  const int leader = __ffsll((long long)bal) - 1;
-  unsigned long long start = 0;
-  if (lane == leader) {
-    start = atomicAdd(count, (unsigned long long)__popcll(bal));
-    atomic_add(prim_sum, pk);
-  }
-  start = __shfl(start, leader);
-  if (take) {
-    const unsigned long long below = bal & ((1ull << lane) - 1ull);
-    const unsigned long long slot = start + __popcll(below);
-    if ((long long)slot < cap) {
-      Quartet qv;
-      qv.bra = (uint32_t)bi;
-      qv.ket_mask = (uint32_t)kj | (mask << 24);
-      out[slot] = qv;
-    }
-  }
-}
+constexpr int kListThreads = 1024;
 
-__global__ __launch_bounds__(256) void quartet_list_kernel(ListLaunch a) {
+// Survivors are appended block-wide: the waves' counts meet in LDS and one
+// thread reserves the block's slots with one atomic per list, so the
+// shared counters see one atomic per 1024 candidates; the Ka Kb Kc Kd sums
+// stay in registers until the end.
+__global__ __launch_bounds__(kListThreads) void quartet_list_kernel(ListLaunch a) {
+  constexpr int NW = kListThreads / 64;
   const long long stride = (long long)gridDim.x * blockDim.x;
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   __shared__ int s_lo, s_hi;
+  __shared__ unsigned s_cnt[2][NW];
+  __shared__ unsigned long long s_start[2][NW];
+  double psum64 = 0.0, psum32 = 0.0;
   for (long long base = (long long)blockIdx.x * blockDim.x; base < a.ncand; base += stride) {
     const long long t = base + threadIdx.x;
     bool pass = false;
@@ -254,10 +236,30 @@ __global__ __launch_bounds__(256) void quartet_list_kernel(ListLaunch a) {
       const int s3 = p_first ? qs1 : ps1, s4 = p_first ? qs2 : ps2;
       qq = P.Q * Q.Q;
       kk4 = (double)P.kk * (double)Q.kk;
+      // lower bounds from the pairs' own block norms: enough to accept
+      const double *pb = a.bra_pn + (size_t)bi * (a.nmat + 1), *pk = a.ket_pn + (size_t)kj * (a.nmat + 1);
+      bool decided = false;
       if (!a.screen) {
         pass = true;
         mask = (1u << a.nmat) - 1u;
-      } else {
+        decided = true;
+      } else if (!(fmax(pb[0], pk[0]) * qq < a.thresh)) {
+        if (a.nmat == 1) {
+          pass = true;
+          mask = 1u;
+          decided = true;
+        } else {
+          unsigned lm = 0;
+          for (int m = 0; m < a.nmat; ++m)
+            if (!(fmax(pb[1 + m], pk[1 + m]) * qq < a.thresh)) lm |= 1u << m;
+          if (lm == (1u << a.nmat) - 1u) {
+            pass = true;
+            mask = lm;
+            decided = true;
+          }
+        }
+      }
+      if (!decided) {
         const long long boff = a.mol_boff[P.mol];
         const int ns = a.mol_ns[P.mol];
         auto bmax = [&](const double *B, int ld, int m) {
@@ -281,8 +283,48 @@ __global__ __launch_bounds__(256) void quartet_list_kernel(ListLaunch a) {
     }
     // precision split on the Schwarz product (the FP32 list when below cut32)
     const bool lo32 = pass && a.cut32 > 0.0 && qq < a.cut32;
-    append_wave(pass && !lo32, bi, kj, mask, kk4, a.out, a.count, a.prim_sum, a.cap, lane);
-    if (a.cut32 > 0.0) append_wave(lo32, bi, kj, mask, kk4, a.out32, a.count32, a.prim_sum32, a.cap, lane);
+    const bool hi64 = pass && !lo32;
+    if (hi64) psum64 += kk4;
+    if (lo32) psum32 += kk4;
+    const unsigned long long bal[2] = {__ballot(hi64), __ballot(lo32)};
+    if (lane == 0) {
+      s_cnt[0][wv] = (unsigned)__popcll(bal[0]);
+      s_cnt[1][wv] = (unsigned)__popcll(bal[1]);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {  // thread l reserves list l's slots for the whole block
+      const int l = threadIdx.x;
+      unsigned long long tot = 0;
+      for (int w = 0; w < NW; ++w) tot += s_cnt[l][w];
+      unsigned long long st = 0;
+      if (tot) st = atomicAdd(l == 0 ? a.count : a.count32, tot);
+      for (int w = 0; w < NW; ++w) {
+        s_start[l][w] = st;
+        st += s_cnt[l][w];
+      }
+    }
+    __syncthreads();
+    const bool take[2] = {hi64, lo32};
+    for (int l = 0; l < 2; ++l) {
+      if (!take[l]) continue;
+      const unsigned long long slot = s_start[l][wv] + __popcll(bal[l] & ((1ull << lane) - 1ull));
+      if ((long long)slot < a.cap) {
+        Quartet qv;
+        qv.bra = (uint32_t)bi;
+        qv.ket_mask = (uint32_t)kj | (mask << 24);
+        (l == 0 ? a.out : a.out32)[slot] = qv;
+      }
+    }
+    __syncthreads();  // s_cnt / s_start are rewritten next iteration
+  }
+  // primitive-quartet sums: one atomic per wave and list at the end
+  for (int off = 32; off > 0; off >>= 1) {
+    psum64 += __shfl_xor(psum64, off);
+    psum32 += __shfl_xor(psum32, off);
+  }
+  if (lane == 0) {
+    if (psum64 != 0.0) atomic_add(a.prim_sum, psum64);
+    if (psum32 != 0.0) atomic_add(a.prim_sum32, psum32);
   }
 }
 
@@ -311,6 +353,17 @@ __global__ void block_norm_kernel(const double *X, int nmat, int cplx, int nb, i
     mx = fmax(mx, row);
   }
   Bm[t] = mx;
+}
+
+__global__ void pair_norm_kernel(const PairScr *pairs, int npair, const double *Bg, const double *Bm,
+                                 const long long *mol_boff, const int *mol_ns, int nmat, double *pn) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npair) return;
+  const PairScr P = pairs[i];
+  const int s1 = (int)(P.s12 & 0xFFFFu), s2 = (int)(P.s12 >> 16);
+  const size_t b = (size_t)mol_boff[P.mol] + s1 + (size_t)s2 * mol_ns[P.mol];
+  pn[(size_t)i * (nmat + 1)] = Bg[b];
+  for (int m = 0; m < nmat; ++m) pn[(size_t)i * (nmat + 1) + 1 + m] = Bm[b * nmat + m];
 }
 
 __global__ void block_norm_finish_kernel(int nmat, int ns, const int *her, double *Bm, double *Bg) {
@@ -471,9 +524,18 @@ hipError_t launch_eri_team(const EriLaunch &a, hipStream_t s) {
 
 hipError_t launch_quartet_list(const ListLaunch &a, hipStream_t s) {
   if (a.ncand <= 0) return hipSuccess;
-  long long blocks = (a.ncand + 255) / 256;
-  if (blocks > 262144) blocks = 262144;
-  hipLaunchKernelGGL(quartet_list_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a);
+  long long blocks = (a.ncand + kListThreads - 1) / kListThreads;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(quartet_list_kernel, dim3((unsigned)blocks), dim3(kListThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_norms(const PairScr *pairs, int npair, const double *Bg, const double *Bm,
+                             const long long *mol_boff, const int *mol_ns, int nmat, double *pn,
+                             hipStream_t s) {
+  if (npair <= 0) return hipSuccess;
+  hipLaunchKernelGGL(pair_norm_kernel, dim3(nblocks(npair, 256)), dim3(256), 0, s, pairs, npair, Bg, Bm,
+                     mol_boff, mol_ns, nmat, pn);
   return hipGetLastError();
 }
 

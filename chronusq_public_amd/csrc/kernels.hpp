@@ -50,6 +50,11 @@ struct ListLaunch {
   const double *Bm;
   const long long *mol_boff;
   const int *mol_ns;
+  // per pair of the bra / ket class: its own block norm B(s1, s2), global
+  // then per matrix (nmat + 1 doubles); their max bounds the six norms of a
+  // quartet from below, which accepts most candidates without the four
+  // cross-norm loads
+  const double *bra_pn, *ket_pn;
   int nmat;
   double thresh;
   int screen;
@@ -65,6 +70,11 @@ struct ListLaunch {
   double *prim_sum32;
 };
 hipError_t launch_quartet_list(const ListLaunch &a, hipStream_t s);
+// pn[i (nmat + 1) + k] = block norm of pair i's own shells (k = 0 global,
+// k = 1 + m matrix m)
+hipError_t launch_pair_norms(const PairScr *pairs, int npair, const double *Bg, const double *Bm,
+                             const long long *mol_boff, const int *mol_ns, int nmat, double *pn,
+                             hipStream_t s);
 
 // Candidate kets of every bra pair of one class pair: off[i + 1] - off[i]
 // kets starting at klo[i] (the bra's molecule's first ket), off an
