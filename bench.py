@@ -132,6 +132,17 @@ class Build:
         return [a.download() for a in self.dA]
 
 
+def traffic_per_build(workload):
+    """HBM bytes per build of the ERI launches measured by the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload (None when that
+    workload has no such measurement)."""
+    path = os.path.join(ROOT, "profiles", "eri_%s_traffic.json" % workload)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["traffic_bytes_per_build"]
+
+
 def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -377,7 +388,11 @@ def main():
             "peak": FP64_VECTOR_PEAK,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_VECTOR_PEAK,
-            "traffic": None,
+            "traffic": traffic_per_build(args.workload),
+            "traffic_note": ("HBM bytes per build of the ERI/digestion launches from rocprofv3 --pmc "
+                             "FETCH_SIZE and WRITE_SIZE runs of this workload, copy-calibrated "
+                             "(profiles/eri_ghf16_traffic.md); almost all of it is the memory-side "
+                             "FP64 atomic adds of the digestion"),
             "flops_per_build": flops_avg,
             "eri_ms_per_build": eri_avg,
         },
