@@ -108,6 +108,8 @@ class Build:
         self.dA = [cqx.DeviceMatrix(self.tpi, nb, self.cplx) for _ in self.mats]
         self.cont = [cqx.TwoBodyContraction(x, a, True, t)
                      for x, a, t in zip(self.dX, self.dA, self.types)]
+        # the first (warm-up) build times the lane counts of every class
+        self.tpi.set_option("autotune", 1)
 
     def step(self):
         for a in self.dA:
@@ -229,8 +231,9 @@ def batched_comparison(nmol, steps, cut=1e-4, seed=0):
     D = C C^T with C NB x 5n, N(0, 1/sqrt NB).  The plain FP64 GPU build is
     one context per molecule, contracted one after the other; the batched
     build is one context over all molecules (every class pair one launch for
-    the batch) with mixed precision (FP32 when Q_ab Q_cd < cut).  A "build"
-    is one Fock build of one molecule."""
+    the batch) with mixed precision (FP32 when Q_ab Q_cd < cut) and lane
+    counts autotuned per class in the warm-up round.  A "build" is one Fock
+    build of one molecule."""
     from chronusq_public_amd import cqx
     from chronusq_public_amd.chem import water_cluster
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -274,6 +277,7 @@ def batched_comparison(nmol, steps, cut=1e-4, seed=0):
     bb = cqx.BasisSet.batch(bases)
     tpi = cqx.DirectTPI(bb, 1e-12, device=0)
     tpi.set_option("fp32_cut", cut)
+    tpi.set_option("autotune", 1)  # the warm-up round picks the lanes per class
     gto = cqx.GTODirectTPIContraction(tpi)
     mats = [make(tpi, b.nBasis, D) for b, D in zip(bases, dens)]
     cont = []

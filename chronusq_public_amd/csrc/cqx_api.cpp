@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <numeric>
 #include <stdexcept>
@@ -91,6 +92,10 @@ struct cqx_ctx {
   int fast_max_nc = 27;   // classes up to this Cartesian size use the lane-group kernels
   int use_wave = 1;       // templated wave-per-quartet kernels for the other s/p/d classes
   int fast_lanes = 0;     // lanes per quartet of the lane-group kernels (0: per launch)
+  bool autotune_pending = false;  // time the lane counts of each class on the next call
+  std::map<int, int> tuned_lanes; // (class pair, precision) -> lanes per quartet
+  DevBuf<double> d_acc_tune;      // scratch accumulators of the timing launches
+  DevBuf<cqx::MolPlanes> d_molplanes_tune;
   double fp32_cut = 0.0;  // quartets with Q_ab Q_cd below this are evaluated in FP32
   long long list_cap = 1ll << 28;  // quartet-list entries per batch (8 B each, x2 when mixed)
   int device = 0;
@@ -129,7 +134,7 @@ struct cqx_ctx {
   DevBuf<double> d_psum;
   DevBuf<double> d_store;
   DevBuf<long long> d_store_off;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
 
   cqx_stats stats{};
 };
@@ -562,6 +567,8 @@ int cqx_ctx_create(const cqx_basis *b, double thresh, int device, cqx_ctx **out)
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));
+    HIPCHK(hipEventCreate(&c->ev_t0));
+    HIPCHK(hipEventCreate(&c->ev_t1));
     c->ns = (int)c->basis.shells.size();
     c->nb = c->basis.nbasis;
     c->nbc = c->basis.ncart_total;
@@ -586,6 +593,8 @@ int cqx_ctx_destroy(cqx_ctx *c) {
   (void)hipStreamSynchronize(c->stream);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->ev_t0) (void)hipEventDestroy(c->ev_t0);
+  if (c->ev_t1) (void)hipEventDestroy(c->ev_t1);
   hipStream_t s = c->stream;
   delete c;
   if (s) (void)hipStreamDestroy(s);
@@ -634,6 +643,14 @@ int cqx_set_option(cqx_ctx *c, const char *name, double value) {
     if (g != 0 && g != 1 && g != 2 && g != 4 && g != 8 && g != 16)
       return fail(c, CQX_ERR_ARG, "fast_lanes must be 0, 1, 2, 4, 8 or 16");
     c->fast_lanes = g;
+    return CQX_OK;
+  }
+  if (!std::strcmp(name, "autotune")) {
+    // 1: the next contraction times every lane count of every lane-group
+    // launch on scratch accumulators and keeps the fastest per class pair
+    // and precision; 0: forget the choices
+    c->autotune_pending = value != 0.0;
+    if (value == 0.0) c->tuned_lanes.clear();
     return CQX_OK;
   }
   if (!std::strcmp(name, "list_cap")) {
@@ -959,6 +976,8 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         }
       }
       c->d_quart.ensure((size_t)cap_need * (mixed ? 2 : 1));
+      bool tune_ready = false;
+      cqx::PlaneSet ps_tune = ps;
       for (const auto &batch : batches) {
         const size_t np = batch.size();
         if (!np) continue;
@@ -1035,6 +1054,47 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           E.planes = ps;
           E.store_mode = 0;
           const int r = route(E.la, E.lb, E.lc, E.ld);
+          if (r == 0) {
+            const int key = ((((E.la * 4 + E.lb) * 4 + E.lc) * 4 + E.ld) << 1) | (f32 ? 1 : 0);
+            if (c->autotune_pending && !c->fast_lanes && !c->tuned_lanes.count(key)) {
+              // time each lane count on scratch accumulators (results discarded)
+              if (!tune_ready) {
+                c->d_acc_tune.ensure(std::max<size_t>(poff[nmol], 1));
+                std::vector<cqx::MolPlanes> mt = mp;
+                for (int m = 0; m < nmol; ++m)
+                  for (int k = 0; k < 4; ++k) mt[m].ka[k] = c->d_acc_tune.p + (mp[m].ka[k] - c->d_acc.p);
+                c->d_molplanes_tune.upload(mt, s);
+                ps_tune = ps;
+                ps_tune.mols = c->d_molplanes_tune.p;
+                ps_tune.m0 = mt[0];
+                tune_ready = true;
+              }
+              cqx::EriLaunch T = E;
+              T.planes = ps_tune;
+              const int nc = cqx::ncart(E.la) * cqx::ncart(E.lb) * cqx::ncart(E.lc) * cqx::ncart(E.ld);
+              float best = 1e30f;
+              int best_g = 0;
+              for (int g : {1, 2, 4, 8, 16}) {
+                if (g == 1 && nc > 9) continue;  // LDS tile too large for one lane per quartet
+                T.force_g = g;
+                float ms = 0.0f;
+                for (int rep = 0; rep < 2; ++rep) {  // the second run is timed
+                  HIPCHK(hipEventRecord(c->ev_t0, s));
+                  HIPCHK(cqx::launch_eri_fast(T, s));
+                  HIPCHK(hipEventRecord(c->ev_t1, s));
+                  HIPCHK(hipEventSynchronize(c->ev_t1));
+                  HIPCHK(hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1));
+                }
+                if (ms < best) {
+                  best = ms;
+                  best_g = g;
+                }
+              }
+              c->tuned_lanes[key] = best_g;
+            }
+            auto it = c->tuned_lanes.find(key);
+            if (!c->fast_lanes && it != c->tuned_lanes.end()) E.force_g = it->second;
+          }
           hipError_t e;
           if (r == 0)
             e = cqx::launch_eri_fast(E, s);
@@ -1083,6 +1143,7 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
       start += nmat;
     }
     c->stats.ms_eri = eri_ms_total;
+    c->autotune_pending = false;
   } catch (const std::exception &e) {
     return fail(c, CQX_ERR_HIP, e.what());
   }

@@ -167,7 +167,10 @@ int cqx_last_stats(const cqx_ctx *ctx, cqx_stats *st);
  * the larger s/p/d classes to the generic kernel instead of the templated
  * wave-per-quartet ones), "fast_lanes" (lanes per shell quartet of the
  * lane-group kernels: 1, 2, 4, 8 or 16; 0, the default, picks it per launch
- * from the mean primitive count), "fp32_cut" (mixed precision: quartets whose
+ * from the mean primitive count), "autotune" (1: the next contraction
+ * times every lane count of every lane-group launch on scratch
+ * accumulators, keeps the fastest per class pair and precision and uses it
+ * from then on; 0 forgets the choices), "fp32_cut" (mixed precision: quartets whose
  * Schwarz product Q_ab Q_cd is below this value are evaluated in FP32 with
  * an FP32 Rys table, digested and accumulated in FP64; default 0 = every
  * quartet in FP64; classes with an f shell always stay FP64), "list_cap"

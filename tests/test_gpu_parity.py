@@ -247,3 +247,23 @@ def test_bounded_list_batches_give_the_same_result():
             assert np.abs(a - m64).max() < 1e-12
             assert np.abs(a - m).max() < 1e-9
 
+
+def test_autotune_leaves_results_unchanged():
+    """The autotuner's timing launches write scratch accumulators only: the
+    tuning call and the calls after it give the untuned J/K (1e-12)."""
+    mol, gbs = _cluster(2, 4.0, 5, "6-31G(d)")
+    say("case: autotune")
+    _, _, b, tpi, gto = make(mol, gbs)
+    nb = b.nBasis
+    rng = np.random.Generator(np.random.PCG64(17))
+    C_ = rng.normal(size=(nb, 5)) + 1j * rng.normal(size=(nb, 5))
+    D = C_ @ C_.conj().T
+    Xs, types, hers = [D, D, D], [COULOMB, EXCHANGE, EXCHANGE], [True, True, False]
+    plain = contract(gto, Xs, types, hers)
+    tpi.set_option("autotune", 1)
+    tuned = contract(gto, Xs, types, hers)   # tunes
+    after = contract(gto, Xs, types, hers)   # uses the choices
+    for p_, t_, a_ in zip(plain, tuned, after):
+        assert np.abs(p_ - t_).max() < 1e-12
+        assert np.abs(p_ - a_).max() < 1e-12
+
