@@ -24,10 +24,47 @@ template <class K, int AR, int AC, int XR, int XC, bool ONE>
 __device__ __forceinline__ void digest_term_items(const double *__restrict__ sI, const int (*__restrict__ sbf)[4],
                                                   const unsigned *__restrict__ smask, const int *__restrict__ smol,
                                                   int nq, const PlaneSet &ps, int kind, int p0, int np, int lgp,
-                                                  double base_coef, bool neg_ok, int lane) {
+                                                  double base_coef, bool neg_ok, bool same_bra, int lane) {
   constexpr int NAR = K::n(AR), NT = NAR * K::n(AC);
   constexpr int ST[4] = {K::NCB * K::NCC * K::NCD, K::NCC * K::NCD, K::NCD, 1};
   const int S = ps.kps[kind];
+  // a bra-only target (J(a, b)) of a tile whose quartets share the bra:
+  // one lane per (target, plane) sums over the quartets, one atomic each
+  if constexpr (AR < 2 && AC < 2) {
+    if (same_bra) {
+      const int bitems = NT << lgp;
+      const int *bf = sbf[0];
+      const MolPlanes &mp = ONE ? ps.m0 : ps.mols[smol[0]];
+      const int N = mp.nbc;
+      for (int it = lane; it < bitems; it += 64) {
+        const int p = it & ((1 << lgp) - 1);
+        if (p >= np) continue;
+        const int tgt = it >> lgp;
+        const int t1 = tgt / NAR, t0 = tgt - t1 * NAR;
+        const int slot = p0 + p;
+        const int mat = (int)((ps.mats >> (4 * slot)) & 15ull);
+        double s0 = 0.0, s1 = 0.0;
+        for (int qi = 0; qi < nq; ++qi) {
+          if (!((smask[qi] >> mat) & 1u)) continue;
+          const int *bq = sbf[qi];
+          const double *__restrict__ Iq = sI + qi * K::NC + t0 * ST[AR] + t1 * ST[AC];
+          const double *__restrict__ X = mp.kx[kind] + p + ((size_t)bq[XR] + (size_t)bq[XC] * N) * S;
+#pragma unroll
+          for (int u0 = 0; u0 < K::n(XR); ++u0)
+#pragma unroll
+            for (int u1 = 0; u1 < K::n(XC); ++u1) {
+              const double v = Iq[u0 * ST[XR] + u1 * ST[XC]] * X[((size_t)u0 + (size_t)u1 * N) * S];
+              if ((u0 * K::n(XC) + u1) & 1) s1 += v; else s0 += v;
+            }
+        }
+        const double coef = (neg_ok && ((ps.negs >> slot) & 1u)) ? -base_coef : base_coef;
+        const double v = coef * (s0 + s1);
+        if (v != 0.0)
+          unsafeAtomicAdd(mp.ka[kind] + p + ((size_t)(bf[AR] + t0) + (size_t)(bf[AC] + t1) * N) * S, v);
+      }
+      return;
+    }
+  }
   const int items = (nq * NT) << lgp;
   for (int it = lane; it < items; it += 64) {
     const int p = it & ((1 << lgp) - 1);
@@ -63,13 +100,13 @@ template <class K, int AR, int AC, int XR, int XC>
 __device__ __forceinline__ void digest_term_tile(const double *__restrict__ sI, const int (*__restrict__ sbf)[4],
                                                  const unsigned *__restrict__ smask, const int *__restrict__ smol,
                                                  int nq, const PlaneSet &ps, int kind, int p0, int np, int lgp,
-                                                 double base_coef, bool neg_ok, int lane) {
+                                                 double base_coef, bool neg_ok, bool same_bra, int lane) {
   if (ps.nmol == 1)
     digest_term_items<K, AR, AC, XR, XC, true>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, base_coef, neg_ok,
-                                               lane);
+                                               same_bra, lane);
   else
     digest_term_items<K, AR, AC, XR, XC, false>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, base_coef,
-                                                neg_ok, lane);
+                                                neg_ok, same_bra, lane);
 }
 
 // Every update of every plane kind for nq quartets: sI holds quartet q at
@@ -79,6 +116,9 @@ __device__ __forceinline__ void digest_term_tile(const double *__restrict__ sI, 
 template <class K>
 __device__ __forceinline__ void digest_tile(const double *sI, const int (*sbf)[4], const unsigned *smask,
                                             const int *smol, int nq, const PlaneSet &ps, int lane) {
+  // do all quartets of the tile share the bra pair (first functions and molecule)?
+  const bool differs = lane < nq && (sbf[lane][0] != sbf[0][0] || sbf[lane][1] != sbf[0][1] || smol[lane] != smol[0]);
+  const bool same_bra = nq > 1 && __ballot(differs) == 0ull;
   for (int kind = 0; kind < 4; ++kind) {
     const int np = ps.kcount[kind];
     if (np == 0) continue;
@@ -86,34 +126,34 @@ __device__ __forceinline__ void digest_tile(const double *sI, const int (*sbf)[4
     const int lgp = np <= 1 ? 0 : np <= 2 ? 1 : np <= 4 ? 2 : np <= 8 ? 3 : 4;
     switch (kind) {
       case kJHer:  // direct.hpp:1863-1883, real parts
-        digest_term_tile<K, 0, 1, 3, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 1.0, false, lane);
-        digest_term_tile<K, 3, 2, 0, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 1.0, false, lane);
+        digest_term_tile<K, 0, 1, 3, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 1.0, false, same_bra, lane);
+        digest_term_tile<K, 3, 2, 0, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 1.0, false, same_bra, lane);
         break;
       case kKHer:  // :1889-1917, 1/2 conj(X)
-        digest_term_tile<K, 0, 2, 3, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, true, lane);
-        digest_term_tile<K, 3, 1, 0, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, true, lane);
-        digest_term_tile<K, 3, 0, 1, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, true, lane);
-        digest_term_tile<K, 1, 2, 3, 0>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, true, lane);
+        digest_term_tile<K, 0, 2, 3, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, true, same_bra, lane);
+        digest_term_tile<K, 3, 1, 0, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, true, same_bra, lane);
+        digest_term_tile<K, 3, 0, 1, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, true, same_bra, lane);
+        digest_term_tile<K, 1, 2, 3, 0>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, true, same_bra, lane);
         break;
       case kJNonHer:  // :1925-1960
-        digest_term_tile<K, 0, 1, 3, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 0, 1, 2, 3>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 1, 0, 3, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 1, 0, 2, 3>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 2, 3, 1, 0>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 2, 3, 0, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 3, 2, 1, 0>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 3, 2, 0, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 0, 1, 3, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 0, 1, 2, 3>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 1, 0, 3, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 1, 0, 2, 3>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 2, 3, 1, 0>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 2, 3, 0, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 3, 2, 1, 0>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 3, 2, 0, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
         break;
       default:  // kKNonHer, :1961-1998
-        digest_term_tile<K, 2, 0, 3, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 3, 1, 2, 0>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 3, 0, 2, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 2, 1, 3, 0>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 0, 2, 1, 3>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 1, 3, 0, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 0, 3, 1, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
-        digest_term_tile<K, 1, 2, 0, 3>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, lane);
+        digest_term_tile<K, 2, 0, 3, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 3, 1, 2, 0>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 3, 0, 2, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 2, 1, 3, 0>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 0, 2, 1, 3>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 1, 3, 0, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 0, 3, 1, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 1, 2, 0, 3>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
         break;
     }
   }
