@@ -19,14 +19,20 @@
 
 namespace cqx {
 
-// ONE: a one-molecule launch, whose planes sit in the kernel arguments
-template <class K, int AR, int AC, int XR, int XC, bool ONE>
+// ONE: a one-molecule launch, whose planes sit in the kernel arguments.
+// HER: a Hermitian kind, finalised as 1/2 (A + A^H): an update of A(r, c)
+// may then land on A(c, r) instead (negated in an imaginary-part plane),
+// and the target is stored with its longer dimension down a column so that
+// neighbouring lanes add neighbouring doubles.
+template <class K, int AR, int AC, int XR, int XC, bool ONE, bool HER>
 __device__ __forceinline__ void digest_term_items(const double *__restrict__ sI, const int (*__restrict__ sbf)[4],
                                                   const unsigned *__restrict__ smask, const int *__restrict__ smol,
                                                   int nq, const PlaneSet &ps, int kind, int p0, int np, int lgp,
                                                   double base_coef, bool neg_ok, bool same_bra, int lane) {
   constexpr int NAR = K::n(AR), NT = NAR * K::n(AC);
   constexpr int ST[4] = {K::NCB * K::NCC * K::NCD, K::NCC * K::NCD, K::NCD, 1};
+  constexpr bool TR = HER && K::n(AC) > K::n(AR);  // store A(ac, ar) instead
+  constexpr int NFAST = TR ? K::n(AC) : NAR;       // target index along the column
   const int S = ps.kps[kind];
   // a bra-only target (J(a, b)) of a tile whose quartets share the bra:
   // one lane per (target, plane) sums over the quartets, one atomic each
@@ -71,8 +77,9 @@ __device__ __forceinline__ void digest_term_items(const double *__restrict__ sI,
     if (p >= np) continue;
     const int r = it >> lgp;
     const int qi = r / NT, tgt = r - qi * NT;
-    // target rows fastest: neighbouring lanes hit neighbouring elements
-    const int t1 = tgt / NAR, t0 = tgt - t1 * NAR;
+    // the stored row index fastest: neighbouring lanes hit neighbouring elements
+    const int tslow = tgt / NFAST, tfast = tgt - tslow * NFAST;
+    const int t0 = TR ? tslow : tfast, t1 = TR ? tfast : tslow;
     const int slot = p0 + p;
     const int mat = (int)((ps.mats >> (4 * slot)) & 15ull);
     if (!((smask[qi] >> mat) & 1u)) continue;
@@ -91,21 +98,25 @@ __device__ __forceinline__ void digest_term_items(const double *__restrict__ sI,
         const double v = Iq[u0 * ST[XR] + u1 * ST[XC]] * X[((size_t)u0 + (size_t)u1 * N) * S];
         if ((u0 * K::n(XC) + u1) & 1) s1 += v; else s0 += v;
       }
-    const double coef = (neg_ok && ((ps.negs >> slot) & 1u)) ? -base_coef : base_coef;
-    unsafeAtomicAdd(ak + p + ((size_t)(bf[AR] + t0) + (size_t)(bf[AC] + t1) * N) * S, coef * (s0 + s1));
+    const bool neg = (ps.negs >> slot) & 1u;
+    // conj of Im X for Hermitian K; a transposed update of an imaginary plane flips sign again
+    const double coef = ((neg_ok && neg) != (TR && neg)) ? -base_coef : base_coef;
+    const size_t e = TR ? ((size_t)(bf[AC] + t1) + (size_t)(bf[AR] + t0) * N)
+                        : ((size_t)(bf[AR] + t0) + (size_t)(bf[AC] + t1) * N);
+    unsafeAtomicAdd(ak + p + e * S, coef * (s0 + s1));
   }
 }
 
-template <class K, int AR, int AC, int XR, int XC>
+template <class K, int AR, int AC, int XR, int XC, bool HER>
 __device__ __forceinline__ void digest_term_tile(const double *__restrict__ sI, const int (*__restrict__ sbf)[4],
                                                  const unsigned *__restrict__ smask, const int *__restrict__ smol,
                                                  int nq, const PlaneSet &ps, int kind, int p0, int np, int lgp,
                                                  double base_coef, bool neg_ok, bool same_bra, int lane) {
   if (ps.nmol == 1)
-    digest_term_items<K, AR, AC, XR, XC, true>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, base_coef, neg_ok,
+    digest_term_items<K, AR, AC, XR, XC, true, HER>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, base_coef, neg_ok,
                                                same_bra, lane);
   else
-    digest_term_items<K, AR, AC, XR, XC, false>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, base_coef,
+    digest_term_items<K, AR, AC, XR, XC, false, HER>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, base_coef,
                                                 neg_ok, same_bra, lane);
 }
 
@@ -126,34 +137,34 @@ __device__ __forceinline__ void digest_tile(const double *sI, const int (*sbf)[4
     const int lgp = np <= 1 ? 0 : np <= 2 ? 1 : np <= 4 ? 2 : np <= 8 ? 3 : 4;
     switch (kind) {
       case kJHer:  // direct.hpp:1863-1883, real parts
-        digest_term_tile<K, 0, 1, 3, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 1.0, false, same_bra, lane);
-        digest_term_tile<K, 3, 2, 0, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 1.0, false, same_bra, lane);
+        digest_term_tile<K, 0, 1, 3, 2, true>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 1.0, false, same_bra, lane);
+        digest_term_tile<K, 3, 2, 0, 1, true>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 1.0, false, same_bra, lane);
         break;
       case kKHer:  // :1889-1917, 1/2 conj(X)
-        digest_term_tile<K, 0, 2, 3, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, true, same_bra, lane);
-        digest_term_tile<K, 3, 1, 0, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, true, same_bra, lane);
-        digest_term_tile<K, 3, 0, 1, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, true, same_bra, lane);
-        digest_term_tile<K, 1, 2, 3, 0>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, true, same_bra, lane);
+        digest_term_tile<K, 0, 2, 3, 1, true>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, true, same_bra, lane);
+        digest_term_tile<K, 3, 1, 0, 2, true>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, true, same_bra, lane);
+        digest_term_tile<K, 3, 0, 1, 2, true>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, true, same_bra, lane);
+        digest_term_tile<K, 1, 2, 3, 0, true>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, true, same_bra, lane);
         break;
       case kJNonHer:  // :1925-1960
-        digest_term_tile<K, 0, 1, 3, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
-        digest_term_tile<K, 0, 1, 2, 3>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
-        digest_term_tile<K, 1, 0, 3, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
-        digest_term_tile<K, 1, 0, 2, 3>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
-        digest_term_tile<K, 2, 3, 1, 0>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
-        digest_term_tile<K, 2, 3, 0, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
-        digest_term_tile<K, 3, 2, 1, 0>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
-        digest_term_tile<K, 3, 2, 0, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 0, 1, 3, 2, false>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 0, 1, 2, 3, false>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 1, 0, 3, 2, false>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 1, 0, 2, 3, false>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 2, 3, 1, 0, false>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 2, 3, 0, 1, false>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 3, 2, 1, 0, false>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 3, 2, 0, 1, false>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
         break;
       default:  // kKNonHer, :1961-1998
-        digest_term_tile<K, 2, 0, 3, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
-        digest_term_tile<K, 3, 1, 2, 0>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
-        digest_term_tile<K, 3, 0, 2, 1>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
-        digest_term_tile<K, 2, 1, 3, 0>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
-        digest_term_tile<K, 0, 2, 1, 3>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
-        digest_term_tile<K, 1, 3, 0, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
-        digest_term_tile<K, 0, 3, 1, 2>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
-        digest_term_tile<K, 1, 2, 0, 3>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 2, 0, 3, 1, false>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 3, 1, 2, 0, false>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 3, 0, 2, 1, false>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 2, 1, 3, 0, false>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 0, 2, 1, 3, false>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 1, 3, 0, 2, false>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 0, 3, 1, 2, false>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
+        digest_term_tile<K, 1, 2, 0, 3, false>(sI, sbf, smask, smol, nq, ps, kind, p0, np, lgp, 0.5, false, same_bra, lane);
         break;
     }
   }
