@@ -1,11 +1,11 @@
 // One-wave-per-quartet Rys ERI + J/K digestion kernels for the larger
 // s/p/d classes (Cartesian quartet size > 27), templated on the class.
 //
-// Per primitive quartet every lane evaluates the Rys nodes/weights (the
-// same T across the wave, so the table reads broadcast), lanes (root, dir)
-// < 3 NR build the 2D integrals of their root and direction in registers
-// and store them to the wave's LDS slice, and every lane accumulates its
-// own share of the contracted Cartesian quartet.  After the primitive loop
+// Each round takes up to PP primitive quartets: lane group g (3 n_r lanes)
+// builds the 2D integrals of primitive quartet g, one lane per (root,
+// direction), each lane evaluating only its own Rys node and weight, and
+// stores them to the wave's LDS slice; then every lane accumulates its own
+// share of the contracted Cartesian quartet over the round's tables.  After the primitive loop
 // the quartet goes to LDS and the lanes digest it (digest_tile.hpp).
 #include <hip/hip_runtime.h>
 
@@ -20,12 +20,23 @@ constexpr int kWavesPerBlock = 4;
 
 // R is the evaluation precision; the quartet handed to the digestion and
 // the accumulators are FP64 either way.
+// Primitive quartets whose 2D tables one round builds side by side: as many
+// groups of 3 n_r lanes as fit in a wave, at most 8, within 8 KB of LDS.
+template <class K, class R>
+constexpr int prims_per_round() {
+  constexpr int by_lanes = 64 / (3 * K::NR);
+  constexpr int by_lds = 8192 / (int)(3 * K::NR * K::NH * sizeof(R));
+  constexpr int p = by_lanes < 8 ? by_lanes : 8;
+  return p < by_lds ? (p < 1 ? 1 : p) : (by_lds < 1 ? 1 : by_lds);
+}
+
 template <int LA, int LB, int LC, int LD, class R>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch a) {
   using K = Cls<LA, LB, LC, LD>;
   constexpr int NOWN = (K::NC + 63) / 64;
   constexpr int HSZ = 3 * K::NR * K::NH;
-  __shared__ R s_H[kWavesPerBlock][HSZ];
+  constexpr int PP = prims_per_round<K, R>();
+  __shared__ R s_H[kWavesPerBlock][PP * HSZ];
   __shared__ double s_I[kWavesPerBlock][K::NC];
   __shared__ int s_bf[kWavesPerBlock][1][4];
   __shared__ unsigned s_mask[kWavesPerBlock][1];
@@ -57,10 +68,17 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
     oy[k] = K::NH + K::hidx(cexp(LA, ia, 1), cexp(LB, ib, 1), cexp(LC, ic, 1), cexp(LD, id, 1));
     oz[k] = 2 * K::NH + K::hidx(cexp(LA, ia, 2), cexp(LB, ib, 2), cexp(LC, ic, 2), cexp(LD, id, 2));
   }
-  const int my_r = lane / 3, my_d = lane % 3;
-  for (int i = 0; i < bp.npp; ++i) {
-    const PrimPair A = a.prims[bp.pp_off + i];
-    for (int j = 0; j < kp.npp; ++j) {
+  // lane group g = lane / (3 n_r) builds the 2D tables of primitive quartet
+  // k0 + g of the round; its lanes (root, direction) one table each
+  const int grp = lane / (3 * K::NR), sub = lane - grp * (3 * K::NR);
+  const int my_r = sub / 3, my_d = sub - my_r * 3;
+  const int nk = kp.npp, ntot = bp.npp * kp.npp;
+  for (int k0 = 0; k0 < ntot; k0 += PP) {
+    const int cnt = ntot - k0 < PP ? ntot - k0 : PP;
+    if (grp < cnt) {
+      const int kq = k0 + grp;
+      const int i = kq / nk, j = kq - i * nk;
+      const PrimPair A = a.prims[bp.pp_off + i];
       const PrimPair B = a.prims[kp.pp_off + j];
       const R ap = (R)A.p, bq = (R)B.p;
       const R pq = ap + bq, rpq = (R)1 / pq;
@@ -69,47 +87,40 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
       const R PQ[3] = {(R)(A.P[0] - B.P[0]), (R)(A.P[1] - B.P[1]), (R)(A.P[2] - B.P[2])};
       const R T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
       const R pref = (R)(A.K * B.K * kTwoPi52) / (ap * bq * sqrt(pq));
-      if (lane < 3 * K::NR) {
-        R x[K::NR], w[K::NR];
-        rys_roots<K::NR, R>(T, rys, x, w);
-        R t2 = x[0], wr = w[0];
+      R t2, wr;
+      rys_root<K::NR, R>(T, rys, my_r, t2, wr);
+      const R rp = (R)1 / ap, rq = (R)1 / bq;
+      const R B00 = (R)0.5 * t2 * rpq;
+      const R B10 = (R)0.5 * rp * ((R)1 - rho * rp * t2);
+      const R B01 = (R)0.5 * rq * ((R)1 - rho * rq * t2);
+      const R PQd = my_d == 0 ? PQ[0] : (my_d == 1 ? PQ[1] : PQ[2]);
+      const R PAd = (R)(my_d == 0 ? A.PA[0] : (my_d == 1 ? A.PA[1] : A.PA[2]));
+      const R QCd = (R)(my_d == 0 ? B.PA[0] : (my_d == 1 ? B.PA[1] : B.PA[2]));
+      const R ABd = (R)(my_d == 0 ? bp.AB[0] : (my_d == 1 ? bp.AB[1] : bp.AB[2]));
+      const R CDd = (R)(my_d == 0 ? kp.AB[0] : (my_d == 1 ? kp.AB[1] : kp.AB[2]));
+      const R C00 = PAd - bq * rpq * t2 * PQd;
+      const R D00 = QCd + ap * rpq * t2 * PQd;
+      R h[K::NH];
+      rys_2d_full<LA, LB, LC, LD, R>(C00, D00, B10, B01, B00, ABd, CDd, my_d == 2 ? pref * wr : (R)1, h);
+      R *dst = H + grp * HSZ + (my_r * 3 + my_d) * K::NH;
 #pragma unroll
-        for (int r = 1; r < K::NR; ++r)
-          if (my_r == r) {
-            t2 = x[r];
-            wr = w[r];
-          }
-        const R rp = (R)1 / ap, rq = (R)1 / bq;
-        const R B00 = (R)0.5 * t2 * rpq;
-        const R B10 = (R)0.5 * rp * ((R)1 - rho * rp * t2);
-        const R B01 = (R)0.5 * rq * ((R)1 - rho * rq * t2);
-        const R PQd = my_d == 0 ? PQ[0] : (my_d == 1 ? PQ[1] : PQ[2]);
-        const R PAd = (R)(my_d == 0 ? A.PA[0] : (my_d == 1 ? A.PA[1] : A.PA[2]));
-        const R QCd = (R)(my_d == 0 ? B.PA[0] : (my_d == 1 ? B.PA[1] : B.PA[2]));
-        const R ABd = (R)(my_d == 0 ? bp.AB[0] : (my_d == 1 ? bp.AB[1] : bp.AB[2]));
-        const R CDd = (R)(my_d == 0 ? kp.AB[0] : (my_d == 1 ? kp.AB[1] : kp.AB[2]));
-        const R C00 = PAd - bq * rpq * t2 * PQd;
-        const R D00 = QCd + ap * rpq * t2 * PQd;
-        R h[K::NH];
-        rys_2d_full<LA, LB, LC, LD, R>(C00, D00, B10, B01, B00, ABd, CDd, my_d == 2 ? pref * wr : (R)1, h);
-        R *dst = H + (my_r * 3 + my_d) * K::NH;
+      for (int e = 0; e < K::NH; ++e) dst[e] = h[e];
+    }
+    wave_sync();
 #pragma unroll
-        for (int e = 0; e < K::NH; ++e) dst[e] = h[e];
-      }
-      wave_sync();
-#pragma unroll
-      for (int k = 0; k < NOWN; ++k) {
-        if (lane + 64 * k >= K::NC) continue;
-        R s = 0;
+    for (int k = 0; k < NOWN; ++k) {
+      if (lane + 64 * k >= K::NC) continue;
+      R s = 0;
+      for (int g = 0; g < cnt; ++g) {
 #pragma unroll
         for (int r = 0; r < K::NR; ++r) {
-          const R *Hr = H + r * 3 * K::NH;
+          const R *Hr = H + g * HSZ + r * 3 * K::NH;
           s = fma(Hr[ox[k]] * Hr[oy[k]], Hr[oz[k]], s);
         }
-        acc[k] += s;
       }
-      wave_sync();
+      acc[k] += s;
     }
+    wave_sync();
   }
   // degeneracy (direct.hpp:1664-1667, 1793-1802) and the 1/2 of :1849-1850
   const bool same_pair = (bp.a == kp.a && bp.b == kp.b);
