@@ -127,6 +127,8 @@ struct cqx_ctx {
   DevBuf<long long> d_cand, d_mol_boff;
   DevBuf<int> d_mol_ns, d_ket_lo, d_seg;
   DevBuf<char> d_scan_tmp;
+  DevBuf<cqx::ListLaunch> d_lists;
+  DevBuf<long long> d_lstart;
   DevBuf<double> d_molmax;
   DevBuf<cqx::MolPlanes> d_molplanes;
   DevBuf<Quartet> d_quart;
@@ -987,10 +989,13 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         // counters cleared by a fill kernel (no memset nodes if the caller captures a graph)
         HIPCHK(cqx::launch_zero(reinterpret_cast<double *>(c->d_count.p), nslot, s));
         HIPCHK(cqx::launch_zero(c->d_psum.p, nslot, s));
+        std::vector<cqx::ListLaunch> lls(np);
+        std::vector<long long> lstart(np + 1, 0);
         for (size_t k = 0; k < np; ++k) {
           const Piece &pc = batch[k];
           const auto &cp = cps[pc.cp];
-          cqx::ListLaunch LL{};
+          cqx::ListLaunch &LL = lls[k];
+          LL = cqx::ListLaunch{};
           LL.bra = c->d_scr[cp.cb].p + pc.i0;
           LL.ket = c->d_scr[cp.ck].p;
           LL.bra_pn = c->d_pn[cp.cb].p + (size_t)pc.i0 * (nmat + 1);
@@ -1021,8 +1026,11 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
             LL.count32 = c->d_count.p + np + k;
             LL.prim_sum32 = c->d_psum.p + np + k;
           }
-          HIPCHK(cqx::launch_quartet_list(LL, s));
+          lstart[k + 1] = lstart[k] + (pc.ncand + cqx::kListChunk - 1) / cqx::kListChunk;
         }
+        c->d_lists.upload(lls, s);
+        c->d_lstart.upload(lstart, s);
+        HIPCHK(cqx::launch_quartet_lists(c->d_lists.p, c->d_lstart.p, (int)np, lstart[np], s));
         std::vector<unsigned long long> counts(nslot);
         std::vector<double> psum(nslot);
         HIPCHK(hipMemcpyAsync(counts.data(), c->d_count.p, sizeof(unsigned long long) * nslot,

@@ -185,146 +185,168 @@ __global__ __launch_bounds__(64) void eri_team_kernel(EriLaunch a) {
 // ---------------------------------------------------------------------------
 constexpr int kListThreads = 1024;
 
-// Survivors are appended block-wide: the waves' counts meet in LDS and one
-// thread reserves the block's slots with one atomic per list, so the
-// shared counters see one atomic per 1024 candidates; the Ka Kb Kc Kd sums
-// stay in registers until the end.
-__global__ __launch_bounds__(kListThreads) void quartet_list_kernel(ListLaunch a) {
+// Candidates [base, base + 1024) of one list: the block's bra range is found
+// once, each thread tests one candidate, survivors are appended block-wide
+// (the waves' counts meet in LDS and one thread reserves the block's slots
+// with one atomic per list), and the Ka Kb Kc Kd sums of the chunk are
+// reduced in LDS to one atomic per list.
+__device__ __forceinline__ void list_chunk(const ListLaunch &a, long long base) {
   constexpr int NW = kListThreads / 64;
-  const long long stride = (long long)gridDim.x * blockDim.x;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   __shared__ int s_lo, s_hi;
   __shared__ unsigned s_cnt[2][NW];
   __shared__ unsigned long long s_start[2][NW];
-  double psum64 = 0.0, psum32 = 0.0;
-  for (long long base = (long long)blockIdx.x * blockDim.x; base < a.ncand; base += stride) {
-    const long long t = base + threadIdx.x;
-    bool pass = false;
-    unsigned mask = 0;
-    int bi = 0, kj = 0;
-    double qq = 0.0, kk4 = 0.0;
-    // the block's candidates span bra indices [s_lo, s_hi]: two searches per
-    // block, then each thread searches that (usually short) range
-    if (threadIdx.x == 0 || threadIdx.x == blockDim.x - 1) {
-      const long long tt =
-          a.cand_base + (threadIdx.x == 0 ? base : min(base + (long long)blockDim.x, a.ncand) - 1);
-      int lo = 0, hi = a.nbra - 1;
-      while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (a.cand_off[mid] <= tt) lo = mid; else hi = mid - 1;
-      }
-      if (threadIdx.x == 0) s_lo = lo; else s_hi = lo;
+  __shared__ double s_ps[2][NW];
+  const long long t = base + threadIdx.x;
+  bool pass = false;
+  unsigned mask = 0;
+  int bi = 0, kj = 0;
+  double qq = 0.0, kk4 = 0.0;
+  // the block's candidates span bra indices [s_lo, s_hi]: two searches per
+  // block, then each thread searches that (usually short) range
+  if (threadIdx.x == 0 || threadIdx.x == blockDim.x - 1) {
+    const long long tt =
+        a.cand_base + (threadIdx.x == 0 ? base : min(base + (long long)blockDim.x, a.ncand) - 1);
+    int lo = 0, hi = a.nbra - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (a.cand_off[mid] <= tt) lo = mid; else hi = mid - 1;
     }
-    __syncthreads();
-    const int blo = s_lo, bhi = s_hi;
-    __syncthreads();
-    if (t < a.ncand) {
-      // bra index: last i with cand_off[i] <= t (absolute candidate index)
-      const long long ta = a.cand_base + t;
-      int lo = blo, hi = bhi;
-      while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (a.cand_off[mid] <= ta) lo = mid; else hi = mid - 1;
-      }
-      bi = lo;
-      kj = a.ket_lo[bi] + (int)(ta - a.cand_off[lo]);
-      const PairScr P = a.bra[bi], Q = a.ket[kj];
-      const int ps1 = (int)(P.s12 & 0xFFFFu), ps2 = (int)(P.s12 >> 16);
-      const int qs1 = (int)(Q.s12 & 0xFFFFu), qs2 = (int)(Q.s12 >> 16);
-      const bool p_first = (ps1 > qs1) || (ps1 == qs1 && ps2 >= qs2);
-      const int s1 = p_first ? ps1 : qs1, s2 = p_first ? ps2 : qs2;
-      const int s3 = p_first ? qs1 : ps1, s4 = p_first ? qs2 : ps2;
-      qq = P.Q * Q.Q;
-      kk4 = (double)P.kk * (double)Q.kk;
-      // lower bounds from the pairs' own block norms: enough to accept
-      const double *pb = a.bra_pn + (size_t)bi * (a.nmat + 1), *pk = a.ket_pn + (size_t)kj * (a.nmat + 1);
-      bool decided = false;
-      if (!a.screen) {
+    if (threadIdx.x == 0) s_lo = lo; else s_hi = lo;
+  }
+  __syncthreads();
+  const int blo = s_lo, bhi = s_hi;
+  if (t < a.ncand) {
+    // bra index: last i with cand_off[i] <= t (absolute candidate index)
+    const long long ta = a.cand_base + t;
+    int lo = blo, hi = bhi;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (a.cand_off[mid] <= ta) lo = mid; else hi = mid - 1;
+    }
+    bi = lo;
+    kj = a.ket_lo[bi] + (int)(ta - a.cand_off[lo]);
+    const PairScr P = a.bra[bi], Q = a.ket[kj];
+    const int ps1 = (int)(P.s12 & 0xFFFFu), ps2 = (int)(P.s12 >> 16);
+    const int qs1 = (int)(Q.s12 & 0xFFFFu), qs2 = (int)(Q.s12 >> 16);
+    const bool p_first = (ps1 > qs1) || (ps1 == qs1 && ps2 >= qs2);
+    const int s1 = p_first ? ps1 : qs1, s2 = p_first ? ps2 : qs2;
+    const int s3 = p_first ? qs1 : ps1, s4 = p_first ? qs2 : ps2;
+    qq = P.Q * Q.Q;
+    kk4 = (double)P.kk * (double)Q.kk;
+    // lower bounds from the pairs' own block norms: enough to accept
+    const double *pb = a.bra_pn + (size_t)bi * (a.nmat + 1), *pk = a.ket_pn + (size_t)kj * (a.nmat + 1);
+    bool decided = false;
+    if (!a.screen) {
+      pass = true;
+      mask = (1u << a.nmat) - 1u;
+      decided = true;
+    } else if (!(fmax(pb[0], pk[0]) * qq < a.thresh)) {
+      if (a.nmat == 1) {
         pass = true;
-        mask = (1u << a.nmat) - 1u;
+        mask = 1u;
         decided = true;
-      } else if (!(fmax(pb[0], pk[0]) * qq < a.thresh)) {
-        if (a.nmat == 1) {
+      } else {
+        unsigned lm = 0;
+        for (int m = 0; m < a.nmat; ++m)
+          if (!(fmax(pb[1 + m], pk[1 + m]) * qq < a.thresh)) lm |= 1u << m;
+        if (lm == (1u << a.nmat) - 1u) {
           pass = true;
-          mask = 1u;
+          mask = lm;
           decided = true;
+        }
+      }
+    }
+    if (!decided) {
+      const long long boff = a.mol_boff[P.mol];
+      const int ns = a.mol_ns[P.mol];
+      auto bmax = [&](const double *B, int ld, int m) {
+        auto at = [&](int x, int y) { return B[(boff + x + (size_t)y * ns) * ld + m]; };
+        double v = fmax(at(s1, s2), fmax(at(s1, s3), at(s2, s3)));
+        v = fmax(v, at(s3, s4));
+        v = fmax(v, fmax(at(s1, s4), at(s2, s4)));
+        return v;
+      };
+      if (!(bmax(a.Bg, 1, 0) * qq < a.thresh)) {
+        if (a.nmat == 1) {
+          mask = 1u;
         } else {
-          unsigned lm = 0;
+          // per-matrix norms interleaved: the nmat values of one block share a line
           for (int m = 0; m < a.nmat; ++m)
-            if (!(fmax(pb[1 + m], pk[1 + m]) * qq < a.thresh)) lm |= 1u << m;
-          if (lm == (1u << a.nmat) - 1u) {
-            pass = true;
-            mask = lm;
-            decided = true;
-          }
+            if (!(bmax(a.Bm, a.nmat, m) * qq < a.thresh)) mask |= 1u << m;
         }
-      }
-      if (!decided) {
-        const long long boff = a.mol_boff[P.mol];
-        const int ns = a.mol_ns[P.mol];
-        auto bmax = [&](const double *B, int ld, int m) {
-          auto at = [&](int x, int y) { return B[(boff + x + (size_t)y * ns) * ld + m]; };
-          double v = fmax(at(s1, s2), fmax(at(s1, s3), at(s2, s3)));
-          v = fmax(v, at(s3, s4));
-          v = fmax(v, fmax(at(s1, s4), at(s2, s4)));
-          return v;
-        };
-        if (!(bmax(a.Bg, 1, 0) * qq < a.thresh)) {
-          if (a.nmat == 1) {
-            mask = 1u;
-          } else {
-            // per-matrix norms interleaved: the nmat values of one block share a line
-            for (int m = 0; m < a.nmat; ++m)
-              if (!(bmax(a.Bm, a.nmat, m) * qq < a.thresh)) mask |= 1u << m;
-          }
-          pass = mask != 0u;
-        }
+        pass = mask != 0u;
       }
     }
-    // precision split on the Schwarz product (the FP32 list when below cut32)
-    const bool lo32 = pass && a.cut32 > 0.0 && qq < a.cut32;
-    const bool hi64 = pass && !lo32;
-    if (hi64) psum64 += kk4;
-    if (lo32) psum32 += kk4;
-    const unsigned long long bal[2] = {__ballot(hi64), __ballot(lo32)};
-    if (lane == 0) {
-      s_cnt[0][wv] = (unsigned)__popcll(bal[0]);
-      s_cnt[1][wv] = (unsigned)__popcll(bal[1]);
-    }
-    __syncthreads();
-    if (threadIdx.x < 2) {  // thread l reserves list l's slots for the whole block
-      const int l = threadIdx.x;
-      unsigned long long tot = 0;
-      for (int w = 0; w < NW; ++w) tot += s_cnt[l][w];
-      unsigned long long st = 0;
-      if (tot) st = atomicAdd(l == 0 ? a.count : a.count32, tot);
-      for (int w = 0; w < NW; ++w) {
-        s_start[l][w] = st;
-        st += s_cnt[l][w];
-      }
-    }
-    __syncthreads();
-    const bool take[2] = {hi64, lo32};
-    for (int l = 0; l < 2; ++l) {
-      if (!take[l]) continue;
-      const unsigned long long slot = s_start[l][wv] + __popcll(bal[l] & ((1ull << lane) - 1ull));
-      if ((long long)slot < a.cap) {
-        Quartet qv;
-        qv.bra = (uint32_t)bi;
-        qv.ket_mask = (uint32_t)kj | (mask << 24);
-        (l == 0 ? a.out : a.out32)[slot] = qv;
-      }
-    }
-    __syncthreads();  // s_cnt / s_start are rewritten next iteration
   }
-  // primitive-quartet sums: one atomic per wave and list at the end
+  // precision split on the Schwarz product (the FP32 list when below cut32)
+  const bool lo32 = pass && a.cut32 > 0.0 && qq < a.cut32;
+  const bool hi64 = pass && !lo32;
+  double p64 = hi64 ? kk4 : 0.0, p32 = lo32 ? kk4 : 0.0;
   for (int off = 32; off > 0; off >>= 1) {
-    psum64 += __shfl_xor(psum64, off);
-    psum32 += __shfl_xor(psum32, off);
+    p64 += __shfl_xor(p64, off);
+    p32 += __shfl_xor(p32, off);
   }
+  const unsigned long long bal[2] = {__ballot(hi64), __ballot(lo32)};
   if (lane == 0) {
-    if (psum64 != 0.0) atomic_add(a.prim_sum, psum64);
-    if (psum32 != 0.0) atomic_add(a.prim_sum32, psum32);
+    s_cnt[0][wv] = (unsigned)__popcll(bal[0]);
+    s_cnt[1][wv] = (unsigned)__popcll(bal[1]);
+    s_ps[0][wv] = p64;
+    s_ps[1][wv] = p32;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {  // thread l reserves list l's slots for the whole block
+    const int l = threadIdx.x;
+    unsigned long long tot = 0;
+    double ps = 0.0;
+    for (int w = 0; w < NW; ++w) {
+      tot += s_cnt[l][w];
+      ps += s_ps[l][w];
+    }
+    unsigned long long st = 0;
+    if (tot) {
+      st = atomicAdd(l == 0 ? a.count : a.count32, tot);
+      atomic_add(l == 0 ? a.prim_sum : a.prim_sum32, ps);
+    }
+    for (int w = 0; w < NW; ++w) {
+      s_start[l][w] = st;
+      st += s_cnt[l][w];
+    }
+  }
+  __syncthreads();
+  const bool take[2] = {hi64, lo32};
+  for (int l = 0; l < 2; ++l) {
+    if (!take[l]) continue;
+    const unsigned long long slot = s_start[l][wv] + __popcll(bal[l] & ((1ull << lane) - 1ull));
+    if ((long long)slot < a.cap) {
+      Quartet qv;
+      qv.bra = (uint32_t)bi;
+      qv.ket_mask = (uint32_t)kj | (mask << 24);
+      (l == 0 ? a.out : a.out32)[slot] = qv;
+    }
+  }
+  __syncthreads();  // the LDS above is rewritten by the next chunk
+}
+
+// Every list of a batch in one launch: list k covers the 1024-aligned
+// chunks [start[k], start[k + 1]) of a common chunk space, so a chunk never
+// spans two lists.
+__global__ __launch_bounds__(kListThreads) void quartet_lists_kernel(const ListLaunch *lists, const long long *start,
+                                                                     int nlist, long long nchunk) {
+  __shared__ int s_k;
+  for (long long c = blockIdx.x; c < nchunk; c += gridDim.x) {
+    if (threadIdx.x == 0) {
+      int lo = 0, hi = nlist - 1;  // last k with start[k] <= c
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= c) lo = mid; else hi = mid - 1;
+      }
+      s_k = lo;
+    }
+    __syncthreads();
+    const int k = s_k;
+    __syncthreads();
+    list_chunk(lists[k], (c - start[k]) * kListThreads);
   }
 }
 
@@ -522,11 +544,12 @@ hipError_t launch_eri_team(const EriLaunch &a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_quartet_list(const ListLaunch &a, hipStream_t s) {
-  if (a.ncand <= 0) return hipSuccess;
-  long long blocks = (a.ncand + kListThreads - 1) / kListThreads;
-  if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(quartet_list_kernel, dim3((unsigned)blocks), dim3(kListThreads), 0, s, a);
+hipError_t launch_quartet_lists(const ListLaunch *lists, const long long *start, int nlist, long long nchunk,
+                                hipStream_t s) {
+  if (nlist <= 0 || nchunk <= 0) return hipSuccess;
+  const long long blocks = nchunk < 65536 ? nchunk : 65536;
+  hipLaunchKernelGGL(quartet_lists_kernel, dim3((unsigned)blocks), dim3(kListThreads), 0, s, lists, start, nlist,
+                     nchunk);
   return hipGetLastError();
 }
 

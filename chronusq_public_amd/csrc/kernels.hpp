@@ -69,7 +69,12 @@ struct ListLaunch {
   unsigned long long *count32;
   double *prim_sum32;
 };
-hipError_t launch_quartet_list(const ListLaunch &a, hipStream_t s);
+// All lists of a batch in one launch: lists[k] (device memory) covers the
+// 1024-candidate chunks [start[k], start[k + 1]) of a common chunk space
+// (start: nlist + 1 entries on the device, nchunk = start[nlist]).
+constexpr int kListChunk = 1024;
+hipError_t launch_quartet_lists(const ListLaunch *lists, const long long *start, int nlist, long long nchunk,
+                                hipStream_t s);
 // pn[i (nmat + 1) + k] = block norm of pair i's own shells (k = 0 global,
 // k = 1 + m matrix m)
 hipError_t launch_pair_norms(const PairScr *pairs, int npair, const double *Bg, const double *Bm,
