@@ -331,6 +331,8 @@ def main():
                     help="SCF jobs of the serving measurement (0 skips it)")
     ap.add_argument("--mixed-steps", type=int, default=3,
                     help="timed builds per precision of the mixed-precision comparison (0 skips it)")
+    ap.add_argument("--opt", action="append", default=[], metavar="NAME=VALUE",
+                    help="engine option of the main workload's context (cqx_set_option), repeatable")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM "
                          "counter calibration for rocprofv3 --pmc runs)")
@@ -341,6 +343,9 @@ def main():
 
     from chronusq_public_amd import cqx
     bench = Build(args.workload)
+    for o in args.opt:
+        name, value = o.split("=", 1)
+        bench.tpi.set_option(name, float(value))
     mol, gbs, tpi = bench.mol, bench.gbs, bench.tpi
     mats, types, cplx = bench.mats, bench.types, bench.cplx
     nb = bench.basis.nBasis

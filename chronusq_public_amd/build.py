@@ -79,6 +79,29 @@ def build_cpp_driver():
     return out
 
 
+HDF5_PREFIX = os.environ.get("CQX_HDF5_PREFIX", "/opt/conda")
+H5LIB = os.path.join(HERE, "libcqx_h5.so")
+
+
+def build_h5():
+    """libcqx_h5.so: the ChronusQ binary-file (HDF5) interop, host C++ linked
+    against the HDF5 C library under HDF5_PREFIX; a library of its own so
+    that libcqx.so never depends on HDF5.  Returns None (and builds
+    nothing) where the HDF5 headers and library are absent."""
+    inc = os.path.join(HDF5_PREFIX, "include")
+    lib = os.path.join(HDF5_PREFIX, "lib")
+    if not (os.path.exists(os.path.join(inc, "hdf5.h")) and os.path.exists(os.path.join(lib, "libhdf5.so"))):
+        print(f"HDF5 not found under {HDF5_PREFIX}: libcqx_h5.so not built", file=sys.stderr)
+        return None
+    src = os.path.join(HERE, "h5io", "cqx_h5.cpp")
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-I", INCLUDE, "-I", inc, src, "-o", H5LIB + ".tmp",
+           "-L", lib, "-lhdf5", "-Wl,-rpath," + lib]
+    subprocess.run(cmd, check=True)
+    os.replace(H5LIB + ".tmp", H5LIB)
+    return H5LIB
+
+
 if __name__ == "__main__":
     build(verbose=True)
     build_cpp_driver()
+    build_h5()

@@ -24,6 +24,7 @@ the transpose; the transpose is undone here.
 import json
 import os
 import re
+import shutil
 import subprocess
 import sys
 
@@ -32,6 +33,8 @@ import numpy as np
 REF = "/root/reference"
 H5DUMP = "/opt/conda/bin/h5dump"
 HERE = os.path.dirname(os.path.abspath(__file__))
+
+BIN_FILES = ("water_6-31Gd.bin.ref", "B_swap_GHF_sto-3g.bin.ref")
 
 NUM = r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?|[-+]?nan|[-+]?inf"
 
@@ -170,6 +173,14 @@ def main():
             json.dump({"source": "tests/scf/reference/" + ref,
                        "deck": "tests/" + inp, **deck}, f, indent=1)
         print(name, {k: v.shape for k, v in arrs.items()})
+
+    # two ChronusQ binary files kept whole (data files of the reference's
+    # own tests) for the binary-file interop: a real RHF file and a complex
+    # GHF one, read there with this package's HDF5 reader and checked
+    # against the h5dump extractions above
+    os.makedirs(os.path.join(HERE, "chronusq_bin"), exist_ok=True)
+    for ref in BIN_FILES:
+        shutil.copyfile(os.path.join(REF, "tests/scf/reference", ref), os.path.join(HERE, "chronusq_bin", ref))
 
 
 if __name__ == "__main__":
