@@ -54,7 +54,7 @@ SYMBOLS = ["cqx_basis_create", "cqx_basis_from_g94", "cqx_basis_destroy",
            "cqx_memcpy", "cqx_device_zero", "cqx_one_body", "cqx_basis_batch", "cqx_scf_default_opts",
            "cqx_scf_create", "cqx_scf_iterate", "cqx_scf_run", "cqx_scf_result_get",
            "cqx_scf_matrix", "cqx_scf_orbital_energies", "cqx_scf_destroy",
-           "cqx_scf_last_error", "cqx_serve_default_opts", "cqx_serve"]
+           "cqx_scf_last_error", "cqx_serve_default_opts", "cqx_serve", "cqx_grad_two_body_contract"]
 
 OVERLAP, KINETIC, POTENTIAL = 0, 1, 2
 
@@ -89,6 +89,7 @@ def lib():
         L.cqx_device_zero.argtypes = [P, P, C.c_size_t]
         L.cqx_one_body.argtypes = [P, I, I, P, P, P]
         L.cqx_basis_batch.argtypes = [I, P, C.POINTER(P)]
+        L.cqx_grad_two_body_contract.argtypes = [P, I, I, P, I]
         _lib = L
     return _lib
 
@@ -300,6 +301,62 @@ class GTODirectTPIContraction:
         _check(lib().cqx_two_body_contract(self.tpi._h, int(screen), len(contractions),
                                            C.cast(arr, C.c_void_p), int(cplx), HOST),
                self.tpi._h)
+
+
+class DirectGradContraction:
+    """GradContractions implementation for the DIRECT algorithm
+    (particleintegrals/gradints/direct.hpp:31-76): gradTwoBodyContract takes
+    cList[g][m], g = 3 atom + xyz, with one density per matrix m shared by
+    every g, and adds the derivative-integral J / K of coordinate g to
+    cList[g][m].AX (direct.hpp:2198-2580).  Real Hermitian densities."""
+
+    def __init__(self, tpi):
+        if not isinstance(tpi, DirectTPI):
+            raise CErr("DirectGradContraction expects a DirectTPI")
+        self.tpi = tpi
+
+    def gradTwoBodyContract(self, cList, screen=True):
+        ngrad, nmat = len(cList), len(cList[0]) if cList else 0
+        if not ngrad or not nmat or any(len(x) != nmat for x in cList):
+            raise CErr("gradTwoBodyContract takes the same number of matrices per coordinate")
+        arr = (_Contraction * (ngrad * nmat))()
+        X = [np.asfortranarray(c.X, dtype=np.float64) for c in cList[0]]
+        for g, row in enumerate(cList):
+            for m, c in enumerate(row):
+                if not (isinstance(c.AX, np.ndarray) and c.AX.dtype == np.float64 and c.AX.flags.f_contiguous):
+                    raise CErr("AX must be a Fortran-ordered float64 array")
+                if c.X is not cList[0][m].X:
+                    raise CErr("every coordinate must contract the same density per matrix")
+                k = g * nmat + m
+                arr[k].X, arr[k].AX = X[m].ctypes.data, c.AX.ctypes.data
+                arr[k].her, arr[k].type = int(c.HER), c.contType
+        _check(lib().cqx_grad_two_body_contract(self.tpi._h, ngrad, nmat, C.cast(arr, C.c_void_p), HOST),
+               self.tpi._h)
+
+
+def getGDGrad(onePDM, tpi, natom, xHFX=1.0):
+    """FockBuilder::getGDGrad (fockbuilder/impl.hpp:371-488) for a real
+    closed-shell density: J^x[D_S] and K^x[D_S] per nuclear coordinate,
+    G^x = 2 J^x - xHFX K^x, and the two-electron gradient
+    1/4 Tr(D_S G^x) (computeOBProperty<SCALAR>)."""
+    D = np.asfortranarray(onePDM, dtype=np.float64)
+    nb = D.shape[0]
+    ngrad = 3 * natom
+    doK = abs(xHFX) > 1e-12
+    J = [np.zeros((nb, nb), order="F") for _ in range(ngrad)]
+    K = [np.zeros((nb, nb), order="F") for _ in range(ngrad)] if doK else None
+    cList = []
+    for g in range(ngrad):
+        row = [TwoBodyContraction(D, J[g], True, COULOMB)]
+        if doK:
+            row.append(TwoBodyContraction(D, K[g], True, EXCHANGE))
+        cList.append(row)
+    DirectGradContraction(tpi).gradTwoBodyContract(cList)
+    grad = np.zeros(ngrad)
+    for g in range(ngrad):
+        G = 2.0 * J[g] - (xHFX * K[g] if doK else 0.0)
+        grad[g] = 0.25 * float(np.sum(D * G))
+    return grad, J, K
 
 
 class SingleSlaterState:

@@ -92,6 +92,7 @@ struct cqx_ctx {
   int fast_max_nc = 27;   // classes up to this Cartesian size use the lane-group kernels
   int use_wave = 1;       // templated wave-per-quartet kernels for the other s/p/d classes
   int fast_lanes = 0;     // lanes per quartet of the lane-group kernels (0: per launch)
+  int lds_max = 65536;    // LDS bytes a block may use on the device
   bool autotune_pending = false;  // time the lane counts of each class on the next call
   std::map<int, int> tuned_lanes; // (class pair, precision) -> lanes per quartet
   DevBuf<double> d_acc_tune;      // scratch accumulators of the timing launches
@@ -109,12 +110,18 @@ struct cqx_ctx {
   std::vector<std::vector<int>> seg;           // per class: molecule m's pairs [seg[m], seg[m + 1])
   std::vector<long long> mol_boff;             // molecule m's NS_m^2 block-norm offset
   std::vector<PrimPair> prims;
+  std::vector<double> prim_alpha;  // exponent of the first shell of each primitive pair (gradients)
   std::vector<double> schwarz;                 // NS x NS
 
   DevBuf<PairRec> d_pairs[kNumClass];
   DevBuf<cqx::PairScr> d_scr[kNumClass];
   DevBuf<double> d_pn[kNumClass];  // per build: each pair's own block norms
   DevBuf<PrimPair> d_prims;
+  DevBuf<double> d_prim_alpha;
+  // nuclear-gradient contraction: shell atoms, Cartesian densities,
+  // accumulators and outputs
+  DevBuf<int> d_shell_atom;
+  DevBuf<double> d_gX, d_gacc, d_gAX;
   DevBuf<double> d_rys, d_sph;
   DevBuf<float> d_rys32;
   DevBuf<int> d_cart_shell, d_cart_comp, d_sh_bf, d_sh_bfc, d_sh_l, d_sh_pure, d_sh_nf,
@@ -202,7 +209,7 @@ double class_pair_flops(int la, int lb, int lc, int ld, bool pure_d, double prim
 
 // primitive pairs of shells (a, b) in kernel orientation
 void make_prims(const cqx::Shell &A, const cqx::Shell &B, std::vector<PrimPair> &out,
-                std::vector<double> *bound) {
+                std::vector<double> *bound, std::vector<double> *alpha_a = nullptr) {
   double AB2 = 0;
   for (int d = 0; d < 3; ++d) AB2 += (A.origin[d] - B.origin[d]) * (A.origin[d] - B.origin[d]);
   for (int i = 0; i < A.nprim(); ++i)
@@ -216,6 +223,7 @@ void make_prims(const cqx::Shell &A, const cqx::Shell &B, std::vector<PrimPair> 
       }
       pp.K = A.coef[i] * B.coef[j] * std::exp(-a * b / p * AB2);
       out.push_back(pp);
+      if (alpha_a) alpha_a->push_back(a);
       // |(pp|qq)| <= U_p U_q with U = sqrt(2) pi^(5/4) |K| p^(-5/4)
       if (bound) bound->push_back(std::sqrt(2.0) * std::pow(M_PI, 1.25) * std::fabs(pp.K) * std::pow(p, -1.25));
     }
@@ -337,7 +345,7 @@ void build_pairs(cqx_ctx *c) {
   c->ref_pairs = B.significant_pairs();
   // primitive-pair bound scale
   std::vector<PrimPair> tmp;
-  std::vector<double> bnd;
+  std::vector<double> bnd, alp;
   double umax = 0.0;
   for (auto &pr : c->ref_pairs) {
     tmp.clear();
@@ -347,17 +355,20 @@ void build_pairs(cqx_ctx *c) {
   }
   c->cls.assign(kNumClass, {});
   c->prims.clear();
+  c->prim_alpha.clear();
   for (auto &pr : c->ref_pairs) {
     int s1 = pr.first, s2 = pr.second;
     int a = s1, b = s2;
     if (B.shells[s1].l < B.shells[s2].l) std::swap(a, b);
     tmp.clear();
     bnd.clear();
-    make_prims(B.shells[a], B.shells[b], tmp, &bnd);
+    alp.clear();
+    make_prims(B.shells[a], B.shells[b], tmp, &bnd, &alp);
     int off = (int)c->prims.size(), n = 0;
     for (size_t k = 0; k < tmp.size(); ++k)
       if (bnd[k] * umax >= c->prim_thresh) {
         c->prims.push_back(tmp[k]);
+        c->prim_alpha.push_back(alp[k]);
         ++n;
       }
     c->cls[class_of(B.shells[a].l, B.shells[b].l)].push_back(make_pair_rec(B, a, b, s1, s2, off, n));
@@ -422,6 +433,7 @@ void build_pairs(cqx_ctx *c) {
   for (int k = 0; k < kNumClass; ++k) segs.insert(segs.end(), c->seg[k].begin(), c->seg[k].end());
   c->d_seg.upload(segs, c->stream);
   c->d_prims.upload(c->prims, c->stream);
+  c->d_prim_alpha.upload(c->prim_alpha, c->stream);
   for (int k = 0; k < kNumClass; ++k) {
     c->d_pairs[k].upload(c->cls[k], c->stream);
     std::vector<cqx::PairScr> scr;
@@ -571,6 +583,7 @@ int cqx_ctx_create(const cqx_basis *b, double thresh, int device, cqx_ctx **out)
     HIPCHK(hipEventCreate(&c->ev1));
     HIPCHK(hipEventCreate(&c->ev_t0));
     HIPCHK(hipEventCreate(&c->ev_t1));
+    HIPCHK(hipDeviceGetAttribute(&c->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
     c->ns = (int)c->basis.shells.size();
     c->nb = c->basis.nbasis;
     c->nbc = c->basis.ncart_total;
@@ -1152,6 +1165,106 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
     }
     c->stats.ms_eri = eri_ms_total;
     c->autotune_pending = false;
+  } catch (const std::exception &e) {
+    return fail(c, CQX_ERR_HIP, e.what());
+  }
+  c->stats.ms_total =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  return CQX_OK;
+}
+
+int cqx_grad_two_body_contract(cqx_ctx *c, int ngrad, int nmat, const cqx_contraction *list, int mem) {
+  if (!c || ngrad <= 0 || ngrad % 3 || nmat <= 0 || !list)
+    return fail(c, CQX_ERR_ARG, "cqx_grad_two_body_contract: bad arguments");
+  if (c->nmol != 1) return fail(c, CQX_ERR_UNSUPPORTED, "gradient contractions take a one-molecule context");
+  const cqx::Basis &BS = c->basis;
+  int natom = 0;
+  for (const auto &sh : BS.shells) {
+    natom = std::max(natom, sh.atom + 1);
+    if (sh.l > 2) return fail(c, CQX_ERR_UNSUPPORTED, "gradient contractions cover s, p and d shells");
+  }
+  if (ngrad < 3 * natom) return fail(c, CQX_ERR_ARG, "fewer gradient components than 3 x atoms of the basis");
+  unsigned kmask = 0;
+  for (int g = 0; g < ngrad; ++g)
+    for (int m = 0; m < nmat; ++m) {
+      const cqx_contraction &cc = list[(size_t)g * nmat + m];
+      if (!cc.X || !cc.AX || (cc.type != CQX_COULOMB && cc.type != CQX_EXCHANGE))
+        return fail(c, CQX_ERR_ARG, "invalid gradient contraction " + std::to_string(g * nmat + m));
+      // direct.hpp:2498-2500
+      if (!cc.her) return fail(c, CQX_ERR_UNSUPPORTED, "Nonhermetian NYI!");
+      // one density per matrix, shared by every coordinate (FockBuilder::getGDGrad)
+      if (cc.X != list[m].X || cc.type != list[m].type)
+        return fail(c, CQX_ERR_ARG, "matrix " + std::to_string(m) + " must use one density and type for every coordinate");
+      if (g == 0 && cc.type == CQX_EXCHANGE) kmask |= 1u << m;
+    }
+  if (nmat > 32) return fail(c, CQX_ERR_ARG, "at most 32 matrices per gradient contraction");
+  const auto t_start = std::chrono::steady_clock::now();
+  c->stats = cqx_stats{};
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int nb = c->nb, nbc = c->nbc;
+    const size_t nb2 = (size_t)nb * nb, nbc2 = (size_t)nbc * nbc;
+    std::vector<int> atom(BS.shells.size());
+    for (size_t k = 0; k < BS.shells.size(); ++k) atom[k] = BS.shells[k].atom;
+    c->d_shell_atom.upload(atom, s);
+    // densities to Cartesian planes
+    const hipMemcpyKind hk = mem == CQX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    c->d_X.ensure((size_t)nmat * nb2);
+    c->d_gX.ensure((size_t)nmat * nbc2);
+    for (int m = 0; m < nmat; ++m) {
+      HIPCHK(hipMemcpyAsync(c->d_X.p + (size_t)m * nb2, list[m].X, sizeof(double) * nb2, hk, s));
+      HIPCHK(cqx::launch_to_cart(c->cmap, c->d_X.p + (size_t)m * nb2, 0, c->d_gX.p + (size_t)m * nbc2, nullptr, 1, s));
+    }
+    const size_t nacc = (size_t)ngrad * nmat * nbc2;
+    c->d_gacc.ensure(nacc);
+    HIPCHK(cqx::launch_zero(c->d_gacc.p, nacc, s));
+    // every quartet of every class pair (bra class >= ket class)
+    HIPCHK(hipEventRecord(c->ev0, s));
+    for (int cb = 0; cb < kNumClass; ++cb)
+      for (int ck = 0; ck <= cb; ++ck) {
+        const long long nbra = (long long)c->cls[cb].size(), nket = (long long)c->cls[ck].size();
+        if (!nbra || !nket) continue;
+        cqx::GradLaunch G{};
+        class_ls(cb, G.la, G.lb);
+        class_ls(ck, G.lc, G.ld);
+        if (cqx::grad_lds_bytes(G.la, G.lb, G.lc, G.ld) > (size_t)c->lds_max)
+          throw std::runtime_error("gradient kernel LDS beyond the device limit");
+        G.bra = c->d_pairs[cb].p;
+        G.ket = c->d_pairs[ck].p;
+        G.nbra = (int)nbra;
+        G.nket = (int)nket;
+        G.same_class = cb == ck;
+        G.nq = cb == ck ? nbra * (nbra + 1) / 2 : nbra * nket;
+        G.prims = c->d_prims.p;
+        G.prim_alpha = c->d_prim_alpha.p;
+        G.rys = c->d_rys.p;
+        G.shell_atom = c->d_shell_atom.p;
+        G.X = c->d_gX.p;
+        G.acc = c->d_gacc.p;
+        G.nmat = nmat;
+        G.nbc = nbc;
+        G.kmask = kmask;
+        HIPCHK(cqx::launch_eri_grad(G, s));
+        c->stats.n_quartets += G.nq;
+        c->stats.n_launch_eri++;
+      }
+    HIPCHK(hipEventRecord(c->ev1, s));
+    // back to pure functions, Hermitian symmetrisation into AX
+    // (AX += 1/2 (A + A^T), direct.hpp:2548-2570), per coordinate and matrix
+    const size_t nout = (size_t)ngrad * nmat;
+    c->d_gAX.ensure(nout * nb2);
+    for (size_t k = 0; k < nout; ++k) {
+      HIPCHK(hipMemcpyAsync(c->d_gAX.p + k * nb2, list[k].AX, sizeof(double) * nb2, hk, s));
+      HIPCHK(cqx::launch_from_cart(c->cmap, c->d_gacc.p + k * nbc2, nullptr, 1, 1, 0, c->d_gAX.p + k * nb2, s));
+    }
+    const hipMemcpyKind dk = mem == CQX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    for (size_t k = 0; k < nout; ++k)
+      HIPCHK(hipMemcpyAsync(list[k].AX, c->d_gAX.p + k * nb2, sizeof(double) * nb2, dk, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->stats.ms_eri = ms;
   } catch (const std::exception &e) {
     return fail(c, CQX_ERR_HIP, e.what());
   }

@@ -36,6 +36,27 @@ bool eri_fast_available(int la, int lb, int lc, int ld);
 hipError_t launch_eri_wave(const EriLaunch &a, hipStream_t s);
 bool eri_wave_available(int la, int lb, int lc, int ld);
 
+// Nuclear-gradient J/K contraction over every quartet of one class pair
+// (no screening, as directScaffoldGrad): quartet q is (bra[i] | ket[j]) with
+// j <= i in row-major lower-triangle order when same_class, else
+// i = q / nket, j = q % nket.
+struct GradLaunch {
+  int la, lb, lc, ld;
+  const PairRec *bra, *ket;
+  int nbra, nket, same_class;
+  long long nq;
+  const PrimPair *prims;
+  const double *prim_alpha;   // exponent of the pair's first (kernel-orientation) shell, per primitive pair
+  const double *rys;
+  const int *shell_atom;      // global shell -> atom
+  const double *X;            // nmat Cartesian density planes, nbc x nbc each
+  double *acc;                // [3 natom][nmat][nbc x nbc] Cartesian accumulators
+  int nmat, nbc;
+  unsigned kmask;             // bit m: matrix m is an exchange contraction
+};
+hipError_t launch_eri_grad(const GradLaunch &a, hipStream_t s);
+size_t grad_lds_bytes(int la, int lb, int lc, int ld);
+
 struct ListLaunch {
   const PairScr *bra, *ket;   // screening records of the bra / ket classes
   int same_class;

@@ -161,6 +161,18 @@ typedef struct {
 } cqx_stats;
 int cqx_last_stats(const cqx_ctx *ctx, cqx_stats *st);
 
+/* DirectGradContraction::gradTwoBodyContract (direct.hpp:2198-2580;
+ * gradints/direct.hpp:31-76): derivative-integral J / K contractions for
+ * every nuclear coordinate.  list[g * nmat + m] (g = 3 atom + xyz) holds
+ * the density X of matrix m -- the same X and type for every g, as
+ * FockBuilder::getGDGrad builds the list (fockbuilder/impl.hpp:371-446) --
+ * and the NB x NB output AX of coordinate g, which receives
+ * 1/2 (A + A^T) (callers zero it).  Real, Hermitian contractions only
+ * (the reference instantiates <double, double> and stops on non-Hermitian
+ * ones), no screening (as the reference), s/p/d shells, one-molecule
+ * contexts; mem as for cqx_two_body_contract. */
+int cqx_grad_two_body_contract(cqx_ctx *ctx, int ngrad, int nmat, const cqx_contraction *list, int mem);
+
 /* Tuning knobs: "prim_screen" (primitive-pair drop threshold, default
  * 1e-18; rebuilds the pair data), "fast_max_nc" (largest Cartesian quartet
  * size routed to the lane-group kernels, default 27), "use_wave" (0 routes

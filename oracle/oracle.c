@@ -562,9 +562,8 @@ static void hrr(int l1, int l2, const double AB[3], int nspec, const double *in,
   memcpy(out, cur, sizeof(double) * NCART(l1) * NCART(l2) * nspec);
 }
 
-static int eri_shell(const oracle_basis *bs, int s1, int s2, int s3, int s4,
-                     double *out, eri_work *w) {
-  const oshell *A = &bs->sh[s1], *B = &bs->sh[s2], *Cc = &bs->sh[s3], *D = &bs->sh[s4];
+static int eri_shell_p(const oshell *A, const oshell *B, const oshell *Cc, const oshell *D,
+                       double *out, eri_work *w) {
   int la = A->l, lb = B->l, lc = Cc->l, ld = D->l;
   int Le = la + lb, Lf = lc + ld, Lt = Le + Lf;
   int nE = NOFF(Le + 1), nF = NOFF(Lf + 1), nM = Lt + 1;
@@ -671,6 +670,11 @@ static int eri_shell(const oracle_basis *bs, int s1, int s2, int s3, int s4,
   int n = n1 * n2 * n3 * n4;
   memcpy(out, src, sizeof(double) * n);
   return n;
+}
+
+static int eri_shell(const oracle_basis *bs, int s1, int s2, int s3, int s4,
+                     double *out, eri_work *w) {
+  return eri_shell_p(&bs->sh[s1], &bs->sh[s2], &bs->sh[s3], &bs->sh[s4], out, w);
 }
 
 int oracle_eri_quartet(const oracle_basis *b, int s1, int s2, int s3, int s4,
@@ -1021,6 +1025,137 @@ int oracle_two_body_contract(oracle_basis *b, double thresh, int screen,
   free(Braw);
   b->st_computed = n_comp;
   b->st_skipped = n_skip;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* DirectGradContraction::directScaffoldGrad (direct.hpp:2198-2580)     */
+/* ------------------------------------------------------------------ */
+/* A shell moved up or down one unit of l on the same primitives, Cartesian:
+ * the derivative of a primitive x_A^a e^{-alpha r_A^2} along x is
+ * 2 alpha x_A^{a+1} (...) - a x_A^{a-1} (...), so the up shell carries the
+ * coefficients 2 alpha c and the down shell c. */
+static void shifted_shell(const oshell *S, int up, oshell *V) {
+  *V = *S;
+  V->pure = 0;
+  V->l = S->l + (up ? 1 : -1);
+  V->nfunc = NCART(V->l);
+  if (up)
+    for (int i = 0; i < S->nprim; i++) V->coef[i] = 2.0 * S->alpha[i] * S->coef[i];
+}
+
+/* Derivatives of (s1 s2|s3 s4) with respect to centre k (0..3) along x, y
+ * and z: out[d] in eri_shell's layout (pure indices where the shell is
+ * pure).  scr holds three buffers of the largest quartet with one index
+ * raised by one unit of l. */
+static void eri_centre_deriv(const oracle_basis *bs, const int s[4], int k, double *out[3],
+                             eri_work *w, double *scr, size_t scr_n) {
+  const oshell *S[4] = {&bs->sh[s[0]], &bs->sh[s[1]], &bs->sh[s[2]], &bs->sh[s[3]]};
+  const oshell *K = S[k];
+  oshell up, dn;
+  shifted_shell(K, 1, &up);
+  const oshell *Q[4] = {S[0], S[1], S[2], S[3]};
+  Q[k] = &up;
+  double *P = scr, *M = scr + scr_n, *T = scr + 2 * scr_n;
+  eri_shell_p(Q[0], Q[1], Q[2], Q[3], P, w);
+  if (K->l > 0) {
+    shifted_shell(K, 0, &dn);
+    Q[k] = &dn;
+    eri_shell_p(Q[0], Q[1], Q[2], Q[3], M, w);
+  }
+  int pre = 1, post = 1;
+  for (int j = 0; j < k; j++) pre *= S[j]->nfunc;
+  for (int j = k + 1; j < 4; j++) post *= S[j]->nfunc;
+  const int l = K->l, nc = NCART(l), nu = NCART(l + 1), nd = l > 0 ? NCART(l - 1) : 0;
+  for (int d = 0; d < 3; d++) {
+    /* Cartesian derivative along d in T: [pre][nc][post] */
+    for (int c = 0; c < nc; c++) {
+      ijk_t e = CART[NOFF(l) + c];
+      int iu = cidx_inc(e, d) - NOFF(l + 1);
+      int n = comp(e, d);
+      int id = n > 0 ? cidx_dec(e, d, 1) - NOFF(l - 1) : -1;
+      for (int a = 0; a < pre; a++)
+        for (int z = 0; z < post; z++) {
+          double v = P[((size_t)a * nu + iu) * post + z];
+          if (n > 0) v -= n * M[((size_t)a * nd + id) * post + z];
+          T[((size_t)a * nc + c) * post + z] = v;
+        }
+    }
+    if (K->pure)
+      sph_index(l, pre, post, T, out[d]);
+    else
+      memcpy(out[d], T, sizeof(double) * (size_t)pre * nc * post);
+  }
+}
+
+int oracle_grad_contract(oracle_basis *b, int nmat, const oracle_contraction *cl, int nthreads) {
+  build_tables();
+  if (nmat <= 0) { set_err("empty contraction list"); return 1; }
+  const int ngrad = 3 * b->natom, NB = b->nbf, ns = b->nshell;
+  for (int i = 0; i < ngrad * nmat; i++)
+    if (!cl[i].her) { set_err("Nonhermetian NYI (directScaffoldGrad)"); return 1; }
+  const size_t NB2 = (size_t)NB * NB;
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+  double *acc = calloc((size_t)nthreads * ngrad * nmat * NB2, sizeof(double));
+  if (!acc) { set_err("out of memory"); return 1; }
+  const int ml = b->maxl + 1;
+  size_t scr_n = 1;
+  for (int i = 0; i < 4; i++) scr_n *= NCART(ml);
+
+#pragma omp parallel num_threads(nthreads)
+  {
+    const int tid = omp_get_thread_num();
+    eri_work w;
+    work_alloc(&w, ml);
+    double *scr = malloc(sizeof(double) * 3 * scr_n);
+    double *dbuf = malloc(sizeof(double) * 3 * scr_n);
+    double *dI[3] = {dbuf, dbuf + scr_n, dbuf + 2 * scr_n};
+    double *A = acc + (size_t)tid * ngrad * nmat * NB2;
+    size_t s12 = 0;
+    for (int s1 = 0; s1 < ns; s1++)
+      for (int k2 = b->sig_off[s1]; k2 < b->sig_off[s1 + 1]; k2++) {
+        const int s2 = b->sig_s2[k2];
+        if ((s12++) % (size_t)nthreads != (size_t)tid) continue;
+        for (int s3 = 0; s3 <= s1; s3++) {
+          const int s4max = (s1 == s3) ? s2 : s3;
+          for (int k4 = b->sig_off[s3]; k4 < b->sig_off[s3 + 1]; k4++) {
+            const int s4 = b->sig_s2[k4];
+            if (s4 > s4max) break;
+            const int s[4] = {s1, s2, s3, s4};
+            const oshell *S[4] = {&b->sh[s1], &b->sh[s2], &b->sh[s3], &b->sh[s4]};
+            const double deg = (s1 == s2 ? 1.0 : 2.0) * (s3 == s4 ? 1.0 : 2.0) *
+                               ((s1 == s3) ? (s2 == s4 ? 1.0 : 2.0) : 2.0);
+            const size_t n = (size_t)S[0]->nfunc * S[1]->nfunc * S[2]->nfunc * S[3]->nfunc;
+            /* the twelve derivative buffers, centre by centre (:2390-2405) */
+            for (int k = 0; k < 4; k++) {
+              eri_centre_deriv(b, s, k, dI, &w, scr, scr_n);
+              for (int d = 0; d < 3; d++) {
+                for (size_t i = 0; i < n; i++) dI[d][i] *= 0.5 * deg;
+                const int g = S[k]->atom * 3 + d;
+                for (int m = 0; m < nmat; m++)
+                  digest_real(cl[(size_t)g * nmat + m].X, A + ((size_t)g * nmat + m) * NB2, 1,
+                              cl[(size_t)g * nmat + m].type, NB, S[0]->bf, S[1]->bf, S[2]->bf, S[3]->bf,
+                              S[0]->nfunc, S[1]->nfunc, S[2]->nfunc, S[3]->nfunc, dI[d]);
+              }
+            }
+          }
+        }
+      }
+    free(dbuf);
+    free(scr);
+    work_free(&w);
+  }
+  /* Hermitian symmetrisation per thread, added to AX (:2548-2570) */
+  for (int gm = 0; gm < ngrad * nmat; gm++) {
+    double *O = cl[gm].AX;
+    for (int t = 0; t < nthreads; t++) {
+      const double *At = acc + ((size_t)t * ngrad * nmat + gm) * NB2;
+      for (int j = 0; j < NB; j++)
+        for (int i = 0; i < NB; i++)
+          O[i + (size_t)j * NB] += 0.5 * At[i + (size_t)j * NB] + 0.5 * At[j + (size_t)i * NB];
+    }
+  }
+  free(acc);
   return 0;
 }
 

@@ -100,6 +100,13 @@ int oracle_two_body_contract(oracle_basis *b, double thresh_schwarz,
                              const oracle_contraction *c, int is_complex,
                              int nthreads);
 
+/* DirectGradContraction::directScaffoldGrad semantics (real, Hermitian
+ * only, no screening): c[g * nmat + m] for g = 3 * atom + xyz holds the
+ * density X and the output AX of matrix m for nuclear coordinate g; every
+ * AX receives 1/2 (A + A^T) of the derivative-integral digestion (callers
+ * zero it). */
+int oracle_grad_contract(oracle_basis *b, int nmat, const oracle_contraction *c, int nthreads);
+
 /* compute and cache the Schwarz bounds (otherwise done by the first
  * contract call, as DirectTPI::computeSchwarz is in the reference) */
 void oracle_prepare(oracle_basis *b);

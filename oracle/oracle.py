@@ -50,6 +50,8 @@ def lib():
         L.oracle_schwarz.argtypes = [P, P]
         L.oracle_two_body_contract.argtypes = [P, D, I, I, P, I, I]
         L.oracle_two_body_contract.restype = I
+        L.oracle_grad_contract.argtypes = [P, I, P, I]
+        L.oracle_grad_contract.restype = I
         L.oracle_last_stats.argtypes = [P, P, P]
         L.oracle_prepare.argtypes = [P]
         _lib = L
@@ -148,6 +150,24 @@ class OracleBasis:
         rc = lib().oracle_two_body_contract(self._h, float(thresh), int(screen),
                                             len(Xs), C.cast(arr, C.c_void_p),
                                             int(cplx), int(nthreads))
+        if rc:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        return AX
+
+    def grad_contract(self, Xs, types, nthreads=0):
+        """directScaffoldGrad (direct.hpp:2198-2580) over real Hermitian
+        NB x NB densities: returns AX[g][m] for every nuclear coordinate
+        g = 3 atom + xyz and matrix m (derivative-integral J / K)."""
+        natom = len(self.Z)
+        Xf = [np.asfortranarray(x, dtype=np.float64) for x in Xs]
+        ngrad, nmat = 3 * natom, len(Xs)
+        AX = [[np.zeros((self.nbf, self.nbf), order="F") for _ in Xs] for _ in range(ngrad)]
+        arr = (_Contraction * (ngrad * nmat))()
+        for g in range(ngrad):
+            for m, (x, t) in enumerate(zip(Xf, types)):
+                arr[g * nmat + m].X, arr[g * nmat + m].AX = x.ctypes.data, AX[g][m].ctypes.data
+                arr[g * nmat + m].her, arr[g * nmat + m].type = 1, int(t)
+        rc = lib().oracle_grad_contract(self._h, nmat, C.cast(arr, C.c_void_p), int(nthreads))
         if rc:
             raise RuntimeError(lib().oracle_last_error().decode())
         return AX

@@ -1,0 +1,63 @@
+"""The oracle's ERI-gradient contraction (directScaffoldGrad restated) is
+pinned by the mathematics, not by restating it: with the AO density fixed,
+J[D] and K[D] depend on the nuclei only through the integrals, so the
+derivative-integral contraction for coordinate g must equal the central
+finite difference of the forward contraction (itself pinned by ChronusQ's
+contract.hdf5 and SCF files) with atom g moved by +-h; and the contractions
+of the coordinates of all atoms along one axis must sum to zero
+(translational invariance)."""
+import numpy as np
+import pytest
+
+from conftest import fixture_molecule, load_fixture
+from oracle.oracle import COULOMB, EXCHANGE, OracleBasis
+
+
+def _water(basis_fixture="scf_water_631gd_rhf"):
+    meta, d = load_fixture(basis_fixture)
+    mol, gbs = fixture_molecule(meta)
+    return mol, gbs, d
+
+
+@pytest.mark.parametrize("cart", [False, True])
+def test_grad_contraction_is_the_derivative_of_the_contraction(cart):
+    mol, gbs, d = _water()
+    Z = np.asarray(mol.Z, dtype=np.int32)
+    xyz = np.asarray(mol.xyz, dtype=np.float64)
+    ob = OracleBasis(gbs, Z, xyz, force_cart=cart)
+    nb = ob.nbf
+    rng = np.random.Generator(np.random.PCG64(41))
+    C_ = rng.normal(0, 1 / np.sqrt(nb), size=(nb, 5))
+    D = C_ @ C_.T
+    G = ob.grad_contract([D, D], [COULOMB, EXCHANGE])
+    assert len(G) == 3 * len(Z)
+    h = 1e-4
+    for g in range(3 * len(Z)):
+        a, x = divmod(g, 3)
+        plus, minus = xyz.copy(), xyz.copy()
+        plus[a, x] += h
+        minus[a, x] -= h
+        Fp = OracleBasis(gbs, Z, plus, force_cart=cart).contract([D, D], [COULOMB, EXCHANGE], [True, True], screen=False)
+        Fm = OracleBasis(gbs, Z, minus, force_cart=cart).contract([D, D], [COULOMB, EXCHANGE], [True, True], screen=False)
+        for m in range(2):
+            fd = (Fp[m] - Fm[m]) / (2 * h)
+            assert np.abs(G[g][m] - fd).max() < 2e-7 * max(1.0, np.abs(fd).max()), (g, m)
+            assert np.abs(G[g][m] - G[g][m].T).max() < 1e-14
+    # translational invariance
+    for m in range(2):
+        for x in range(3):
+            tot = sum(G[3 * a + x][m] for a in range(len(Z)))
+            assert np.abs(tot).max() < 1e-11
+
+
+def test_grad_contraction_rejects_non_hermitian():
+    mol, gbs, _ = _water()
+    ob = OracleBasis(gbs, np.asarray(mol.Z, dtype=np.int32), np.asarray(mol.xyz))
+    import ctypes as C
+    from oracle import oracle as O
+    nb = ob.nbf
+    X = np.zeros((nb, nb), order="F")
+    arr = (O._Contraction * (3 * len(mol.Z)))()
+    for i in range(len(arr)):
+        arr[i].X, arr[i].AX, arr[i].her, arr[i].type = X.ctypes.data, X.ctypes.data, 0, 0
+    assert O.lib().oracle_grad_contract(ob._h, 1, C.cast(arr, C.c_void_p), 1) != 0
