@@ -115,6 +115,29 @@ class GpuDirectTPIContraction : public TPIContractions<MatsT> {
   std::shared_ptr<DirectTPI> tpi_;
 };
 
+// GradContractions<double, double> for the DIRECT algorithm
+// (particleintegrals/gradints/direct.hpp:31-76): cList[g][m], g = 3 atom +
+// xyz, one density per matrix m shared by every g (FockBuilder::getGDGrad).
+class GpuDirectGradContraction {
+ public:
+  explicit GpuDirectGradContraction(std::shared_ptr<DirectTPI> tpi) : tpi_(std::move(tpi)) {}
+  void gradTwoBodyContract(std::vector<std::vector<TwoBodyContraction<double>>> &cList) const {
+    if (cList.empty() || cList[0].empty()) throw std::runtime_error("FATAL: Nothing specified to compute");
+    const size_t nmat = cList[0].size();
+    std::vector<cqx_contraction> c;
+    c.reserve(cList.size() * nmat);
+    for (auto &row : cList) {
+      if (row.size() != nmat) throw std::runtime_error("FATAL: ragged gradient contraction list");
+      for (auto &x : row) c.push_back({x.X, x.AX, x.HER ? 1 : 0, (int)x.contType});
+    }
+    check(cqx_grad_two_body_contract(tpi_->handle(), (int)cList.size(), (int)nmat, c.data(), CQX_HOST),
+          tpi_->handle());
+  }
+
+ private:
+  std::shared_ptr<DirectTPI> tpi_;
+};
+
 // The parts of SingleSlater that formGD reads and writes: Pauli components
 // of the density (S; or S, Z, Y, X) and the J / K / G outputs.
 template <typename MatsT>
