@@ -43,7 +43,7 @@ class Stats(C.Structure):
                 ("n_prim_quartets_max", C.c_longlong), ("flops_model", C.c_double),
                 ("ms_total", C.c_double), ("ms_eri", C.c_double),
                 ("n_launch_eri", C.c_int), ("n_quartets_f32", C.c_longlong),
-                ("flops_model_f32", C.c_double)]
+                ("flops_model_f32", C.c_double), ("graph", C.c_int)]
 
 
 SYMBOLS = ["cqx_basis_create", "cqx_basis_from_g94", "cqx_basis_destroy",
@@ -508,7 +508,7 @@ class _Job(C.Structure):
 class ServeOpts(C.Structure):
     _fields_ = [("max_inflight", C.c_int), ("hbm_budget_bytes", C.c_double), ("device", C.c_int),
                 ("thresh", C.c_double), ("list_cap", C.c_longlong), ("scf", ScfOpts),
-                ("verbose", C.c_int)]
+                ("verbose", C.c_int), ("graph", C.c_int)]
 
 
 class JobResult(C.Structure):
@@ -518,7 +518,7 @@ class JobResult(C.Structure):
 
 
 def serve(jobs, max_inflight=8, hbm_budget_bytes=200e9, fp32_cut=0.0, guess="SAD",
-          list_cap=1 << 25, device=0, rmsd_tol=1e-8, verbose=0):
+          list_cap=1 << 25, device=0, rmsd_tol=1e-8, verbose=0, graph=0):
     """Run SCF jobs [(basis name or G94 text, Molecule, arrival seconds)]
     through the serving loop; returns one dict per job (latency =
     t_done - t_arrival)."""
@@ -531,7 +531,7 @@ def serve(jobs, max_inflight=8, hbm_budget_bytes=200e9, fp32_cut=0.0, guess="SAD
     o = ServeOpts()
     L.cqx_serve_default_opts(C.byref(o))
     o.max_inflight, o.hbm_budget_bytes, o.device, o.list_cap = int(max_inflight), float(hbm_budget_bytes), int(device), int(list_cap)
-    o.verbose = int(verbose)
+    o.verbose, o.graph = int(verbose), int(graph)
     o.scf.fp32_cut, o.scf.guess, o.scf.rmsd_tol = fp32_cut, GUESS[guess.upper()], rmsd_tol
     arr = (_Job * len(jobs))()
     keep = []

@@ -145,6 +145,26 @@ struct cqx_ctx {
   DevBuf<long long> d_store_off;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
 
+  // Captured build (option "graph"): the launches of one build, from the
+  // shell-block norms to the accumulation into AX, recorded once into a HIP
+  // graph and replayed by the next builds with the same contraction
+  // signature.  Every list and ERI launch reads its count from the device;
+  // the capacities come from the build that recorded the graph.
+  struct GraphSlot { int la, lb, lc, ld, f32; };
+  int use_graph = 0;
+  double graph_margin = 1.25;      // candidate capacity / candidates of the recording build
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::string graph_key;           // contraction signature of the recording build
+  int g_nmat = 0, g_nJ = 0, g_nK = 0;
+  bool g_pure_d = false;
+  std::vector<size_t> g_xoff, g_mnb2;  // per molecule: offset and size of its matrices
+  std::vector<long long> g_cap;        // per class pair: candidate capacity
+  std::vector<long long> g_lcap;       // per ERI slot: list capacity
+  std::vector<GraphSlot> g_slots;      // per ERI slot (class pair, precision); la < 0: not launched
+  DevBuf<cqx::ListLaunch> d_glists;
+  DevBuf<long long> d_glstart, d_gidx, d_gchk;
+
   cqx_stats stats{};
 };
 
@@ -154,6 +174,24 @@ int fail(cqx_ctx *c, int code, const std::string &msg) {
   if (c) c->err = msg;
   g_error = msg;
   return code;
+}
+
+// A captured build is dropped whenever the workspace it refers to may be
+// reallocated (any build launched eagerly, options, gradient contractions).
+void drop_graph(cqx_ctx *c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  c->graph_key.clear();
+}
+
+// contraction signature: what the recorded launches depend on besides the
+// densities themselves
+std::string contraction_key(int screen, int is_complex, int ncon, const cqx_contraction *list) {
+  std::string k = std::to_string(screen) + ":" + std::to_string(is_complex) + ":";
+  for (int i = 0; i < ncon; ++i) k += (char)('0' + 2 * list[i].type + (list[i].her ? 1 : 0));
+  return k;
 }
 
 // Cartesian -> pure transform of a contracted Cartesian quartet (host side,
@@ -452,6 +490,68 @@ void build_pairs(cqx_ctx *c) {
   HIPCHK(hipStreamSynchronize(c->stream));
 }
 
+// Replay of the captured build for the densities of `list`: stage X and AX,
+// launch the graph, read back the candidate totals and list counts in one
+// synchronisation.  Returns false, with the caller's AX untouched, when a
+// class pair had more candidates, or a list more quartets, than its
+// capacity (the lists or the evaluation would be incomplete); the caller
+// then builds eagerly.
+bool replay_graph(cqx_ctx *c, int ncon, const cqx_contraction *list, int is_complex, int mem) {
+  hipStream_t s = c->stream;
+  const int nmol = c->nmol, nper = ncon / nmol, nmat = c->g_nmat;
+  const size_t ew = is_complex ? 2 : 1;
+  const hipMemcpyKind hk = mem == CQX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  for (int m = 0; m < nmol; ++m)
+    for (int j = 0; j < nmat; ++j) {
+      const cqx_contraction &cc = list[m * nper + j];
+      const size_t o = c->g_xoff[m] + (size_t)j * c->g_mnb2[m] * ew, n = sizeof(double) * c->g_mnb2[m] * ew;
+      HIPCHK(hipMemcpyAsync(c->d_X.p + o, cc.X, n, hk, s));
+      HIPCHK(hipMemcpyAsync(c->d_AX.p + o, cc.AX, n, hk, s));
+    }
+  HIPCHK(hipEventRecord(c->ev0, s));
+  HIPCHK(hipGraphLaunch(c->graph_exec, s));
+  HIPCHK(hipEventRecord(c->ev1, s));
+  const size_t ncp = c->g_cap.size();
+  std::vector<long long> tot(ncp);
+  std::vector<unsigned long long> counts(2 * ncp);
+  std::vector<double> psum(2 * ncp);
+  HIPCHK(hipMemcpyAsync(tot.data(), c->d_gchk.p, sizeof(long long) * ncp, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(counts.data(), c->d_count.p, sizeof(unsigned long long) * 2 * ncp, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(psum.data(), c->d_psum.p, sizeof(double) * 2 * ncp, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (size_t k = 0; k < ncp; ++k)
+    if (tot[k] > c->g_cap[k]) return false;
+  for (size_t kk = 0; kk < 2 * ncp; ++kk)
+    if ((long long)counts[kk] > c->g_lcap[kk]) return false;
+  const hipMemcpyKind dk = mem == CQX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  for (int m = 0; m < nmol; ++m)
+    for (int j = 0; j < nmat; ++j)
+      HIPCHK(hipMemcpyAsync(list[m * nper + j].AX, c->d_AX.p + c->g_xoff[m] + (size_t)j * c->g_mnb2[m] * ew,
+                            sizeof(double) * c->g_mnb2[m] * ew, dk, s));
+  HIPCHK(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->stats.ms_eri = ms;  // the whole recorded build
+  for (size_t k = 0; k < ncp; ++k) c->stats.n_candidates += tot[k];
+  for (size_t kk = 0; kk < 2 * ncp; ++kk) {
+    const auto &G = c->g_slots[kk];
+    if (G.la < 0) continue;
+    c->stats.n_launch_eri++;
+    if (!counts[kk]) continue;
+    const double fl = class_pair_flops(G.la, G.lb, G.lc, G.ld, c->g_pure_d, psum[kk], (double)counts[kk],
+                                       is_complex ? 2 : 1, c->g_nJ, c->g_nK);
+    c->stats.n_quartets += (long long)counts[kk];
+    c->stats.n_prim_quartets_max += (long long)psum[kk];
+    c->stats.flops_model += fl;
+    if (G.f32) {
+      c->stats.n_quartets_f32 += (long long)counts[kk];
+      c->stats.flops_model_f32 += fl;
+    }
+  }
+  c->stats.graph = 2;
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -606,6 +706,7 @@ int cqx_ctx_destroy(cqx_ctx *c) {
   if (!c) return CQX_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->ev_t0) (void)hipEventDestroy(c->ev_t0);
@@ -636,6 +737,17 @@ int cqx_ctx_shell_pairs(cqx_ctx *c, int *npairs, int *s1, int *s2) {
 
 int cqx_set_option(cqx_ctx *c, const char *name, double value) {
   if (!c || !name) return fail(c, CQX_ERR_ARG, "cqx_set_option: bad arguments");
+  (void)hipSetDevice(c->device);
+  drop_graph(c);  // every option changes what a recorded build would launch
+  if (!std::strcmp(name, "graph")) {
+    c->use_graph = value != 0.0;
+    return CQX_OK;
+  }
+  if (!std::strcmp(name, "graph_margin")) {
+    if (!(value >= 1.0)) return fail(c, CQX_ERR_ARG, "graph_margin must be >= 1");
+    c->graph_margin = value;
+    return CQX_OK;
+  }
   if (!std::strcmp(name, "prim_screen")) {
     c->prim_thresh = value;
     try {
@@ -736,8 +848,19 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         return fail(c, CQX_ERR_ARG, "batched molecules must share the contraction types and HER flags");
   const auto t_start = std::chrono::steady_clock::now();
   c->stats = cqx_stats{};
+  const std::string key = contraction_key(screen, is_complex, ncon, list);
   try {
     HIPCHK(hipSetDevice(c->device));
+    if (c->use_graph && c->graph_exec && c->graph_key == key) {
+      if (replay_graph(c, ncon, list, is_complex, mem)) {
+        c->stats.ms_total =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+        return CQX_OK;
+      }
+      c->stats = cqx_stats{};  // capacities exceeded: build eagerly and record again
+    }
+    // an eager build may reallocate the workspace a recorded graph refers to
+    drop_graph(c);
     hipStream_t s = c->stream;
     const cqx::Basis &BS = c->basis;
     const size_t ew = is_complex ? 2 : 1;  // doubles per element
@@ -780,26 +903,17 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           HIPCHK(hipMemcpyAsync(c->d_X.p + o, cc.X, n, hk, s));
           HIPCHK(hipMemcpyAsync(c->d_AX.p + o, cc.AX, n, hk, s));
         }
+      // --- plan: every upload and allocation of the build comes first, so
+      // that the launches below can also be recorded into a graph ---
       std::vector<int> her(nmat);
       for (int j = 0; j < nmat; ++j) her[j] = L[j].her ? 1 : 0;
       c->d_her.upload(her, s);
-      // shell-block norms, per molecule, and each molecule's largest one
       const long long nbl = c->mol_boff[nmol];
       c->d_Bm.ensure((size_t)nmat * nbl);
       c->d_Bg.ensure((size_t)nbl);
-      for (int m = 0; m < nmol; ++m)
-        HIPCHK(cqx::launch_block_norms(c->d_X.p + xoff[m], nmat, is_complex, c->d_her.p, mnb[m], mns[m],
-                                       c->d_sh_bf.p + BS.mol_sh0[m], BS.mol_bf0[m], c->d_sh_nf.p + BS.mol_sh0[m],
-                                       c->d_Bm.p + (size_t)c->mol_boff[m] * nmat, c->d_Bg.p + c->mol_boff[m], s));
       c->d_molmax.ensure(nmol);
-      HIPCHK(cqx::launch_mol_max(c->d_Bg.p, c->d_mol_boff.p, c->d_mol_ns.p, nmol, c->d_molmax.p, s));
-      for (int k = 0; k < kNumClass; ++k) {
-        const int np_k = (int)c->cls[k].size();
-        if (!np_k) continue;
-        c->d_pn[k].ensure((size_t)np_k * (nmat + 1));
-        HIPCHK(cqx::launch_pair_norms(c->d_scr[k].p, np_k, c->d_Bg.p, c->d_Bm.p, c->d_mol_boff.p, c->d_mol_ns.p,
-                                      nmat, c->d_pn[k].p, s));
-      }
+      for (int k = 0; k < kNumClass; ++k)
+        if (!c->cls[k].empty()) c->d_pn[k].ensure(c->cls[k].size() * (size_t)(nmat + 1));
       cqx::PlaneSet ps{};
       // planes (one per real part), ordered by kind; slot = position, so the
       // planes that one digestion term visits sit in consecutive slots
@@ -837,7 +951,6 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
       for (int m = 0; m < nmol; ++m) poff[m + 1] = poff[m] + (size_t)kpsum * mnbc2[m];
       c->d_planes.ensure(std::max<size_t>(poff[nmol], 1));
       c->d_acc.ensure(std::max<size_t>(poff[nmol], 1));
-      HIPCHK(cqx::launch_zero(c->d_acc.p, poff[nmol], s));
       std::vector<cqx::MolPlanes> mp(nmol);
       for (int m = 0; m < nmol; ++m) {
         size_t o = poff[m];
@@ -882,16 +995,6 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         M.bfc0 = BS.mol_bfc0[m];
         return M;
       };
-      for (int m = 0; m < nmol; ++m) {
-        const cqx::CartMap M = mol_map(m);
-        for (int j = 0; j < nmat; ++j) {
-          const int pr = slot_re[j], pi = slot_im[j];
-          double *xre = const_cast<double *>(mp[m].kx[pd[pr].kind]) + pidx[pr];
-          double *xim = pi >= 0 ? const_cast<double *>(mp[m].kx[pd[pi].kind]) + pidx[pi] : nullptr;
-          HIPCHK(cqx::launch_to_cart(M, c->d_X.p + xoff[m] + (size_t)j * mnb2[m] * ew, is_complex, xre, xim,
-                                     ps.kps[pd[pr].kind], s));
-        }
-      }
       int nJ = 0, nK = 0;
       for (int j = 0; j < nmat; ++j) (L[j].type == CQX_COULOMB ? nJ : nK)++;
       // candidate lists per class pair (bra class >= ket class): bra i of
@@ -909,20 +1012,12 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         }
       c->d_cand.ensure(std::max<size_t>(cand_total, 1));
       c->d_ket_lo.ensure(std::max<size_t>(cand_total, 1));
+      std::vector<cqx::CandLaunch> cls_launch(cps.size());
       size_t scan_bytes = 0;
-      for (auto &cp : cps) {
-        cqx::CandLaunch CL{};
-        CL.nbra = cp.nbra;
-        CL.off = c->d_cand.p + cp.coff;
-        size_t b = 0;
-        HIPCHK(cqx::launch_candidates(CL, nullptr, b, s));
-        scan_bytes = std::max(scan_bytes, b);
-      }
-      c->d_scan_tmp.ensure(std::max<size_t>(scan_bytes, 1));
-      std::vector<long long> totals(cps.size());
       for (size_t k = 0; k < cps.size(); ++k) {
-        auto &cp = cps[k];
-        cqx::CandLaunch CL{};
+        const auto &cp = cps[k];
+        cqx::CandLaunch &CL = cls_launch[k];
+        CL = cqx::CandLaunch{};
         CL.bra = c->d_pairs[cp.cb].p;
         CL.ket = c->d_pairs[cp.ck].p;
         CL.nbra = cp.nbra;
@@ -934,11 +1029,59 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         CL.same_class = cp.cb == cp.ck;
         CL.off = c->d_cand.p + cp.coff;
         CL.klo = c->d_ket_lo.p + cp.coff;
-        size_t b = c->d_scan_tmp.n;
-        HIPCHK(cqx::launch_candidates(CL, c->d_scan_tmp.p, b, s));
-        HIPCHK(hipMemcpyAsync(&totals[k], c->d_cand.p + cp.coff + cp.nbra, sizeof(long long),
-                              hipMemcpyDeviceToHost, s));
+        size_t b = 0;
+        HIPCHK(cqx::launch_candidates(CL, nullptr, b, s));  // size query only
+        scan_bytes = std::max(scan_bytes, b);
       }
+      c->d_scan_tmp.ensure(std::max<size_t>(scan_bytes, 1));
+      // --- launches up to the candidate counts: shell-block norms, per
+      // molecule, and each molecule's largest one; the pairs' own norms;
+      // cleared accumulators; Cartesian density planes; candidate offsets ---
+      auto front = [&]() {
+        for (int m = 0; m < nmol; ++m)
+          HIPCHK(cqx::launch_block_norms(c->d_X.p + xoff[m], nmat, is_complex, c->d_her.p, mnb[m], mns[m],
+                                         c->d_sh_bf.p + BS.mol_sh0[m], BS.mol_bf0[m], c->d_sh_nf.p + BS.mol_sh0[m],
+                                         c->d_Bm.p + (size_t)c->mol_boff[m] * nmat, c->d_Bg.p + c->mol_boff[m], s));
+        HIPCHK(cqx::launch_mol_max(c->d_Bg.p, c->d_mol_boff.p, c->d_mol_ns.p, nmol, c->d_molmax.p, s));
+        for (int k = 0; k < kNumClass; ++k)
+          if (!c->cls[k].empty())
+            HIPCHK(cqx::launch_pair_norms(c->d_scr[k].p, (int)c->cls[k].size(), c->d_Bg.p, c->d_Bm.p,
+                                          c->d_mol_boff.p, c->d_mol_ns.p, nmat, c->d_pn[k].p, s));
+        HIPCHK(cqx::launch_zero(c->d_acc.p, poff[nmol], s));
+        for (int m = 0; m < nmol; ++m) {
+          const cqx::CartMap M = mol_map(m);
+          for (int j = 0; j < nmat; ++j) {
+            const int pr = slot_re[j], pi = slot_im[j];
+            double *xre = const_cast<double *>(mp[m].kx[pd[pr].kind]) + pidx[pr];
+            double *xim = pi >= 0 ? const_cast<double *>(mp[m].kx[pd[pi].kind]) + pidx[pi] : nullptr;
+            HIPCHK(cqx::launch_to_cart(M, c->d_X.p + xoff[m] + (size_t)j * mnb2[m] * ew, is_complex, xre, xim,
+                                       ps.kps[pd[pr].kind], s));
+          }
+        }
+        for (auto &CL : cls_launch) {
+          size_t b = c->d_scan_tmp.n;
+          HIPCHK(cqx::launch_candidates(CL, c->d_scan_tmp.p, b, s));
+        }
+      };
+      // --- back to pure functions, symmetrise, accumulate into AX ---
+      auto back = [&]() {
+        for (int m = 0; m < nmol; ++m) {
+          const cqx::CartMap M = mol_map(m);
+          for (int j = 0; j < nmat; ++j) {
+            const bool two = is_complex && !(L[j].her && L[j].type == CQX_COULOMB);
+            const int pr = slot_re[j], pi = slot_im[j];
+            const double *are = mp[m].ka[pd[pr].kind] + pidx[pr];
+            const double *aim = two ? mp[m].ka[pd[pi].kind] + pidx[pi] : nullptr;
+            HIPCHK(cqx::launch_from_cart(M, are, aim, ps.kps[pd[pr].kind], L[j].her, is_complex,
+                                         c->d_AX.p + xoff[m] + (size_t)j * mnb2[m] * ew, s));
+          }
+        }
+      };
+      front();
+      std::vector<long long> totals(cps.size());
+      for (size_t k = 0; k < cps.size(); ++k)
+        HIPCHK(hipMemcpyAsync(&totals[k], c->d_cand.p + cps[k].coff + cps[k].nbra, sizeof(long long),
+                              hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
       long long total = 0;
       for (size_t k = 0; k < cps.size(); ++k) {
@@ -959,6 +1102,40 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         return 2;
       };
       const bool mixed = c->fp32_cut > 0.0;
+      // the ERI launch of class pair k's bras from i0 on, in one precision
+      // (the count, list and lane choice are set by the caller)
+      auto eri_base = [&](const CP &cp, int i0, bool f32) {
+        cqx::EriLaunch E{};
+        class_ls(cp.cb, E.la, E.lb);
+        class_ls(cp.ck, E.lc, E.ld);
+        E.bra = c->d_pairs[cp.cb].p + i0;
+        E.ket = c->d_pairs[cp.ck].p;
+        E.prims = c->d_prims.p;
+        E.rys = c->d_rys.p;
+        E.rys32 = c->d_rys32.p;
+        E.fp32 = f32 ? 1 : 0;
+        E.force_g = c->fast_lanes;
+        E.planes = ps;
+        E.store_mode = 0;
+        return E;
+      };
+      auto lane_key = [](const cqx::EriLaunch &E) {
+        return ((((E.la * 4 + E.lb) * 4 + E.lc) * 4 + E.ld) << 1) | (E.fp32 ? 1 : 0);
+      };
+      // one list in launches of at most 2^25 quartets, so that no grid
+      // exceeds 2^31 work-items (the wave kernels take 4 quartets per block)
+      auto launch_eri = [&](const cqx::EriLaunch &E) {
+        constexpr long long kPiece = 1ll << 25;
+        const int r = route(E.la, E.lb, E.lc, E.ld);
+        if (r == 2 && E.fp32) throw std::runtime_error("FP32 quartet list for a class without an FP32 kernel");
+        for (long long b = 0; b < E.nq; b += kPiece) {
+          cqx::EriLaunch P = E;
+          P.quartets = E.quartets + b;
+          P.nq = std::min(kPiece, E.nq - b);
+          P.nq_base = b;
+          HIPCHK(r == 0 ? cqx::launch_eri_fast(P, s) : r == 1 ? cqx::launch_eri_wave(P, s) : cqx::launch_eri_team(P, s));
+        }
+      };
       // bra-range pieces of at most list_cap candidates, grouped into batches
       // of at most list_cap candidates: the quartet lists of one batch are
       // built, counted (one host synchronisation) and evaluated before the
@@ -991,6 +1168,9 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         }
       }
       c->d_quart.ensure((size_t)cap_need * (mixed ? 2 : 1));
+      // per class pair and precision: quartets and primitive quartets of this
+      // build (they size the launches of a recorded graph)
+      std::vector<double> cp_count(2 * cps.size(), 0.0), cp_psum(2 * cps.size(), 0.0);
       bool tune_ready = false;
       cqx::PlaneSet ps_tune = ps;
       for (const auto &batch : batches) {
@@ -1030,6 +1210,7 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           LL.count = c->d_count.p + k;
           LL.prim_sum = c->d_psum.p + k;
           LL.cap = pc.ncand;
+          LL.cap32 = pc.ncand;
           int la_, lb_, lc_, ld_;
           class_ls(cp.cb, la_, lb_);
           class_ls(cp.ck, lc_, ld_);
@@ -1059,24 +1240,14 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           const bool f32 = kk >= np;
           const Piece &pc = batch[k];
           const auto &cp = cps[pc.cp];
-          cqx::EriLaunch E{};
-          class_ls(cp.cb, E.la, E.lb);
-          class_ls(cp.ck, E.lc, E.ld);
+          cp_count[pc.cp + (f32 ? cps.size() : 0)] += (double)counts[kk];
+          cp_psum[pc.cp + (f32 ? cps.size() : 0)] += psum[kk];
+          cqx::EriLaunch E = eri_base(cp, pc.i0, f32);
           E.quartets = c->d_quart.p + (f32 ? cap_need : 0) + pc.seg;
           E.nq = (long long)counts[kk];
-          E.bra = c->d_pairs[cp.cb].p + pc.i0;
-          E.ket = c->d_pairs[cp.ck].p;
-          E.prims = c->d_prims.p;
-          E.rys = c->d_rys.p;
-          E.rys32 = c->d_rys32.p;
-          E.fp32 = f32 ? 1 : 0;
           E.mean_prims = psum[kk] / (double)counts[kk];
-          E.force_g = c->fast_lanes;
-          E.planes = ps;
-          E.store_mode = 0;
-          const int r = route(E.la, E.lb, E.lc, E.ld);
-          if (r == 0) {
-            const int key = ((((E.la * 4 + E.lb) * 4 + E.lc) * 4 + E.ld) << 1) | (f32 ? 1 : 0);
+          if (route(E.la, E.lb, E.lc, E.ld) == 0) {
+            const int key = lane_key(E);
             if (c->autotune_pending && !c->fast_lanes && !c->tuned_lanes.count(key)) {
               // time each lane count on scratch accumulators (results discarded)
               if (!tune_ready) {
@@ -1101,7 +1272,7 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
                 float ms = 0.0f;
                 for (int rep = 0; rep < 2; ++rep) {  // the second run is timed
                   HIPCHK(hipEventRecord(c->ev_t0, s));
-                  HIPCHK(cqx::launch_eri_fast(T, s));
+                  launch_eri(T);
                   HIPCHK(hipEventRecord(c->ev_t1, s));
                   HIPCHK(hipEventSynchronize(c->ev_t1));
                   HIPCHK(hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1));
@@ -1116,16 +1287,7 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
             auto it = c->tuned_lanes.find(key);
             if (!c->fast_lanes && it != c->tuned_lanes.end()) E.force_g = it->second;
           }
-          hipError_t e;
-          if (r == 0)
-            e = cqx::launch_eri_fast(E, s);
-          else if (r == 1)
-            e = cqx::launch_eri_wave(E, s);
-          else if (!f32)
-            e = cqx::launch_eri_team(E, s);
-          else
-            throw std::runtime_error("FP32 quartet list for a class without an FP32 kernel");
-          HIPCHK(e);
+          launch_eri(E);
           const double fl = class_pair_flops(E.la, E.lb, E.lc, E.ld, pure_d, psum[kk], (double)counts[kk],
                                              is_complex ? 2 : 1, nJ, nK);
           c->stats.n_quartets += (long long)counts[kk];
@@ -1143,24 +1305,138 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
         eri_ms_total += ms;
       }
-      // back to pure functions, symmetrise, accumulate into AX
-      for (int m = 0; m < nmol; ++m) {
-        const cqx::CartMap M = mol_map(m);
-        for (int j = 0; j < nmat; ++j) {
-          const bool two = is_complex && !(L[j].her && L[j].type == CQX_COULOMB);
-          const int pr = slot_re[j], pi = slot_im[j];
-          const double *are = mp[m].ka[pd[pr].kind] + pidx[pr];
-          const double *aim = two ? mp[m].ka[pd[pi].kind] + pidx[pi] : nullptr;
-          HIPCHK(cqx::launch_from_cart(M, are, aim, ps.kps[pd[pr].kind], L[j].her, is_complex,
-                                       c->d_AX.p + xoff[m] + (size_t)j * mnb2[m] * ew, s));
-        }
-      }
+      back();
       const hipMemcpyKind dk = mem == CQX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
       for (int m = 0; m < nmol; ++m)
         for (int j = 0; j < nmat; ++j)
           HIPCHK(hipMemcpyAsync(list[m * nper + start + j].AX, c->d_AX.p + xoff[m] + (size_t)j * mnb2[m] * ew,
                                 sizeof(double) * mnb2[m] * ew, dk, s));
       HIPCHK(hipStreamSynchronize(s));
+      // Record the same build into a graph for the next calls (option
+      // graph; one contraction group and one list batch).  Capacities are
+      // graph_margin times this build's own sizes: each class pair's
+      // candidates (the list kernel's range) and each list (class pair and
+      // precision; the range its ERI launch covers).  The list kernel reads
+      // the candidate totals and the ERI kernels the list counts from the
+      // device, and a replay that exceeds a capacity is discarded.
+      const size_t ncp = cps.size();
+      std::vector<long long> gcap(ncp), glst(ncp + 1, 0), lcap(2 * ncp, 0), lseg(2 * ncp + 1, 0);
+      for (size_t k = 0; k < ncp; ++k) {
+        const long long want = (long long)std::ceil((double)cps[k].ncand * c->graph_margin) + cqx::kListChunk;
+        gcap[k] = (want + cqx::kListChunk - 1) / cqx::kListChunk * cqx::kListChunk;
+        glst[k + 1] = glst[k] + gcap[k] / cqx::kListChunk;
+        int la_, lb_, lc_, ld_;
+        class_ls(cps[k].cb, la_, lb_);
+        class_ls(cps[k].ck, lc_, ld_);
+        lcap[k] = (long long)std::ceil(cp_count[k] * c->graph_margin) + 1024;
+        if (mixed && route(la_, lb_, lc_, ld_) < 2)
+          lcap[ncp + k] = (long long)std::ceil(cp_count[ncp + k] * c->graph_margin) + 1024;
+      }
+      for (size_t kk = 0; kk < 2 * ncp; ++kk) lseg[kk + 1] = lseg[kk] + lcap[kk];
+      if (c->use_graph && start == 0 && nmat == nper && batches.size() == 1 && ncp > 0 &&
+          lseg[2 * ncp] <= c->list_cap * (mixed ? 2 : 1)) {
+        c->d_quart.ensure((size_t)lseg[2 * ncp]);
+        c->d_count.ensure(2 * ncp);
+        c->d_psum.ensure(2 * ncp);
+        std::vector<cqx::ListLaunch> gl(ncp);
+        std::vector<long long> gidx(ncp);
+        std::vector<cqx_ctx::GraphSlot> slots(2 * ncp, cqx_ctx::GraphSlot{-1, 0, 0, 0, 0});
+        for (size_t k = 0; k < ncp; ++k) {
+          const auto &cp = cps[k];
+          cqx::ListLaunch &LL = gl[k];
+          LL = cqx::ListLaunch{};
+          LL.bra = c->d_scr[cp.cb].p;
+          LL.ket = c->d_scr[cp.ck].p;
+          LL.bra_pn = c->d_pn[cp.cb].p;
+          LL.ket_pn = c->d_pn[cp.ck].p;
+          LL.same_class = cp.cb == cp.ck;
+          LL.cand_off = c->d_cand.p + cp.coff;
+          LL.cand_base = 0;
+          LL.ket_lo = c->d_ket_lo.p + cp.coff;
+          LL.nbra = cp.nbra;
+          LL.ncand = gcap[k];
+          LL.ncand_dev = c->d_cand.p + cp.coff + cp.nbra;
+          LL.Bg = c->d_Bg.p;
+          LL.Bm = c->d_Bm.p;
+          LL.mol_boff = c->d_mol_boff.p;
+          LL.mol_ns = c->d_mol_ns.p;
+          LL.nmat = nmat;
+          LL.thresh = c->thresh;
+          LL.screen = screen;
+          LL.out = c->d_quart.p + lseg[k];
+          LL.count = c->d_count.p + k;
+          LL.prim_sum = c->d_psum.p + k;
+          LL.cap = lcap[k];
+          int la_, lb_, lc_, ld_;
+          class_ls(cp.cb, la_, lb_);
+          class_ls(cp.ck, lc_, ld_);
+          slots[k] = {la_, lb_, lc_, ld_, 0};
+          if (mixed && route(la_, lb_, lc_, ld_) < 2) {
+            LL.cut32 = c->fp32_cut;
+            LL.out32 = c->d_quart.p + lseg[ncp + k];
+            LL.cap32 = lcap[ncp + k];
+            LL.count32 = c->d_count.p + ncp + k;
+            LL.prim_sum32 = c->d_psum.p + ncp + k;
+            slots[ncp + k] = {la_, lb_, lc_, ld_, 1};
+          }
+          gidx[k] = (long long)(cp.coff + cp.nbra);
+        }
+        c->d_glists.upload(gl, s);
+        c->d_glstart.upload(glst, s);
+        c->d_gidx.upload(gidx, s);
+        c->d_gchk.ensure(ncp);
+        HIPCHK(hipStreamSynchronize(s));  // every upload done before recording
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+          front();
+          HIPCHK(cqx::launch_zero(reinterpret_cast<double *>(c->d_count.p), 2 * ncp, s));
+          HIPCHK(cqx::launch_zero(c->d_psum.p, 2 * ncp, s));
+          HIPCHK(cqx::launch_quartet_lists(c->d_glists.p, c->d_glstart.p, (int)ncp, glst[ncp], s));
+          for (size_t kk = 0; kk < 2 * ncp; ++kk) {
+            if (slots[kk].la < 0) continue;
+            const size_t k = kk % ncp;
+            const bool f32 = kk >= ncp;
+            cqx::EriLaunch E = eri_base(cps[k], 0, f32);
+            E.quartets = c->d_quart.p + lseg[kk];
+            E.nq = lcap[kk];
+            E.nq_dev = c->d_count.p + kk;
+            const double n0 = cp_count[kk];
+            E.mean_prims = n0 > 0.0 ? cp_psum[kk] / n0 : 1.0;
+            if (route(E.la, E.lb, E.lc, E.ld) == 0) {
+              auto it = c->tuned_lanes.find(lane_key(E));
+              if (!c->fast_lanes && it != c->tuned_lanes.end()) E.force_g = it->second;
+            }
+            launch_eri(E);
+          }
+          back();
+          HIPCHK(cqx::launch_gather(c->d_cand.p, c->d_gidx.p, (int)ncp, c->d_gchk.p, s));
+        } catch (...) {
+          (void)hipStreamEndCapture(s, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        HIPCHK(hipStreamEndCapture(s, &g));
+        hipGraphExec_t ge = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+          (void)hipGraphDestroy(g);
+          HIPCHK(ie);
+        }
+        c->graph = g;
+        c->graph_exec = ge;
+        c->graph_key = key;
+        c->g_nmat = nmat;
+        c->g_nJ = nJ;
+        c->g_nK = nK;
+        c->g_pure_d = pure_d;
+        c->g_xoff.assign(xoff.begin(), xoff.end());
+        c->g_mnb2 = mnb2;
+        c->g_cap = gcap;
+        c->g_lcap = lcap;
+        c->g_slots = slots;
+        c->stats.graph = 1;
+      }
       start += nmat;
     }
     c->stats.ms_eri = eri_ms_total;
@@ -1202,6 +1478,7 @@ int cqx_grad_two_body_contract(cqx_ctx *c, int ngrad, int nmat, const cqx_contra
   c->stats = cqx_stats{};
   try {
     HIPCHK(hipSetDevice(c->device));
+    drop_graph(c);  // the workspace below may be reallocated
     hipStream_t s = c->stream;
     const int nb = c->nb, nbc = c->nbc;
     const size_t nb2 = (size_t)nb * nb, nbc2 = (size_t)nbc * nbc;

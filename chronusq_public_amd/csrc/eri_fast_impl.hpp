@@ -129,6 +129,24 @@ __device__ __forceinline__ void rys_2d_full(R C00, R D00, R B10, R B01, R B00, R
     }
 }
 
+// The factors of a primitive quartet (bra exponent sum p, ket q) that need
+// a division, from one reciprocal d = 1 / (p q (p + q)):
+// 1 / (p + q), 1 / (p q), and the prefactor K_ab K_cd 2 pi^(5/2) / (p q sqrt(p + q)).
+template <class R>
+struct PrimRecip {
+  R rpq, rab, pref;
+};
+template <class R>
+__device__ __forceinline__ PrimRecip<R> prim_recip(R ap, R bq, double kk) {
+  const R pq = ap + bq, ab = ap * bq;
+  const R d = eval_rcp<R>(ab * pq);
+  PrimRecip<R> r;
+  r.rpq = ab * d;
+  r.rab = pq * d;
+  r.pref = (R)(kk * kTwoPi52) * r.rab * eval_sqrt<R>(r.rpq);
+  return r;
+}
+
 // Rys table of the evaluation precision
 template <class R> __device__ __forceinline__ const R *rys_table(const EriLaunch &a);
 template <> __device__ __forceinline__ const double *rys_table<double>(const EriLaunch &a) { return a.rys; }
@@ -151,10 +169,13 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
   __shared__ int s_mol[4][QPW];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane & (G - 1), qw = lane / G;
+  // quartet count: the list's device counter for a launch sized to the
+  // list's capacity before the list was built
+  const long long nq = a.nq_dev ? min((long long)*a.nq_dev - a.nq_base, a.nq) : a.nq;
   const long long q0 = ((long long)blockIdx.x * 4 + wv) * QPW;
-  if (q0 >= a.nq) return;  // uniform across the wave; no block barriers below
+  if (q0 >= nq) return;  // uniform across the wave; no block barriers below
   const long long q = q0 + qw;
-  const bool valid = q < a.nq;
+  const bool valid = q < nq;
   Quartet qt{0u, 0u};
   PairRec bp{}, kp{};
   if (valid) {
@@ -181,12 +202,12 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
       ++i;
     }
     const R ap = (R)A.p, bq = (R)B.p;
-    const R pq = ap + bq, rpq = (R)1 / pq;
+    const PrimRecip<R> pr = prim_recip<R>(ap, bq, A.K * B.K);
+    const R rpq = pr.rpq, pref = pr.pref;
     const R rho = ap * bq * rpq;
     // centre differences in FP64 before any rounding to R
     const R PQ[3] = {(R)(A.P[0] - B.P[0]), (R)(A.P[1] - B.P[1]), (R)(A.P[2] - B.P[2])};
     const R T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
-    const R pref = (R)(A.K * B.K * kTwoPi52) / (ap * bq * sqrt(pq));
     R x[K::NR], w[K::NR];
     rys_roots<K::NR, R>(T, rys, x, w);
     if constexpr (K::NC == 1) {
@@ -195,7 +216,7 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
       for (int r = 0; r < K::NR; ++r) s += w[r];
       I[0] = fma(pref, s, I[0]);
     } else {
-      const R rp = (R)1 / ap, rq = (R)1 / bq;
+      const R rp = bq * pr.rab, rq = ap * pr.rab;
 #pragma unroll
       for (int r = 0; r < K::NR; ++r) {
         const R t2 = x[r];
@@ -238,7 +259,7 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
     s_mol[wv][qw] = bp.mol;
   }
   wave_sync();
-  const long long left = a.nq - q0;
+  const long long left = nq - q0;
   digest_tile<K>(s_I[wv], s_bf[wv], s_mask[wv], s_mol[wv], left < QPW ? (int)left : QPW, a.planes, lane);
 }
 

@@ -42,8 +42,11 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
   __shared__ unsigned s_mask[kWavesPerBlock][1];
   __shared__ int s_mol[kWavesPerBlock][1];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // quartet count: the list's device counter for a launch sized to the
+  // list's capacity before the list was built
+  const long long nq = a.nq_dev ? min((long long)*a.nq_dev - a.nq_base, a.nq) : a.nq;
   const long long q = (long long)blockIdx.x * kWavesPerBlock + wv;
-  if (q >= a.nq) return;  // uniform across the wave; no block barriers below
+  if (q >= nq) return;  // uniform across the wave; no block barriers below
   const Quartet qt = a.quartets[q];
   const PairRec bp = a.bra[qt.bra];
   const PairRec kp = a.ket[qt.ket_mask & 0xFFFFFFu];
@@ -81,15 +84,15 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
       const PrimPair A = a.prims[bp.pp_off + i];
       const PrimPair B = a.prims[kp.pp_off + j];
       const R ap = (R)A.p, bq = (R)B.p;
-      const R pq = ap + bq, rpq = (R)1 / pq;
+      const PrimRecip<R> pr = prim_recip<R>(ap, bq, A.K * B.K);
+      const R rpq = pr.rpq, pref = pr.pref;
       const R rho = ap * bq * rpq;
       // centre differences in FP64 before any rounding to R
       const R PQ[3] = {(R)(A.P[0] - B.P[0]), (R)(A.P[1] - B.P[1]), (R)(A.P[2] - B.P[2])};
       const R T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
-      const R pref = (R)(A.K * B.K * kTwoPi52) / (ap * bq * sqrt(pq));
       R t2, wr;
       rys_root<K::NR, R>(T, rys, my_r, t2, wr);
-      const R rp = (R)1 / ap, rq = (R)1 / bq;
+      const R rp = bq * pr.rab, rq = ap * pr.rab;
       const R B00 = (R)0.5 * t2 * rpq;
       const R B10 = (R)0.5 * rp * ((R)1 - rho * rp * t2);
       const R B01 = (R)0.5 * rq * ((R)1 - rho * rq * t2);

@@ -107,7 +107,10 @@ __global__ __launch_bounds__(64) void eri_team_kernel(EriLaunch a) {
   double *s_w = lds + 8;
   double *s_H = lds + 16;
   double *s_I = s_H + D.nr * 3 * D.nh;
-  for (long long q = blockIdx.x; q < a.nq; q += gridDim.x) {
+  // quartet count: the list's device counter for a launch sized to the
+  // list's capacity before the list was built
+  const long long nq = a.nq_dev ? min((long long)*a.nq_dev - a.nq_base, a.nq) : a.nq;
+  for (long long q = blockIdx.x; q < nq; q += gridDim.x) {
     const Quartet qt = a.quartets[q];
     const PairRec bp = a.bra[qt.bra];
     const PairRec kp = a.ket[qt.ket_mask & 0xFFFFFFu];
@@ -197,6 +200,10 @@ __device__ __forceinline__ void list_chunk(const ListLaunch &a, long long base) 
   __shared__ unsigned s_cnt[2][NW];
   __shared__ unsigned long long s_start[2][NW];
   __shared__ double s_ps[2][NW];
+  // candidates of the list: counted on the device for a list sized in
+  // advance (ncand is then its capacity)
+  const long long ncand = a.ncand_dev ? min(*a.ncand_dev - a.cand_base, a.ncand) : a.ncand;
+  if (base >= ncand) return;  // uniform across the block
   const long long t = base + threadIdx.x;
   bool pass = false;
   unsigned mask = 0;
@@ -206,7 +213,7 @@ __device__ __forceinline__ void list_chunk(const ListLaunch &a, long long base) 
   // block, then each thread searches that (usually short) range
   if (threadIdx.x == 0 || threadIdx.x == blockDim.x - 1) {
     const long long tt =
-        a.cand_base + (threadIdx.x == 0 ? base : min(base + (long long)blockDim.x, a.ncand) - 1);
+        a.cand_base + (threadIdx.x == 0 ? base : min(base + (long long)blockDim.x, ncand) - 1);
     int lo = 0, hi = a.nbra - 1;
     while (lo < hi) {
       int mid = (lo + hi + 1) >> 1;
@@ -216,7 +223,7 @@ __device__ __forceinline__ void list_chunk(const ListLaunch &a, long long base) 
   }
   __syncthreads();
   const int blo = s_lo, bhi = s_hi;
-  if (t < a.ncand) {
+  if (t < ncand) {
     // bra index: last i with cand_off[i] <= t (absolute candidate index)
     const long long ta = a.cand_base + t;
     int lo = blo, hi = bhi;
@@ -318,7 +325,7 @@ __device__ __forceinline__ void list_chunk(const ListLaunch &a, long long base) 
   for (int l = 0; l < 2; ++l) {
     if (!take[l]) continue;
     const unsigned long long slot = s_start[l][wv] + __popcll(bal[l] & ((1ull << lane) - 1ull));
-    if ((long long)slot < a.cap) {
+    if ((long long)slot < (l == 0 ? a.cap : a.cap32)) {
       Quartet qv;
       qv.bra = (uint32_t)bi;
       qv.ket_mask = (uint32_t)kj | (mask << 24);
@@ -515,6 +522,11 @@ __global__ void mol_max_kernel(const double *Bg, const long long *mol_boff, cons
   if (threadIdx.x == 0) out[m] = red[0];
 }
 
+__global__ void gather_kernel(const long long *src, const long long *idx, int n, long long *dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
 __global__ void zero_kernel(double *p, size_t n) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = 0.0;
@@ -593,6 +605,12 @@ hipError_t launch_mol_max(const double *Bg, const long long *mol_boff, const int
                           double *out, hipStream_t s) {
   if (nmol <= 0) return hipSuccess;
   hipLaunchKernelGGL(mol_max_kernel, dim3(nmol), dim3(256), 0, s, Bg, mol_boff, mol_ns, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const long long *src, const long long *idx, int n, long long *dst, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, src, idx, n, dst);
   return hipGetLastError();
 }
 

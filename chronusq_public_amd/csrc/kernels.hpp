@@ -13,6 +13,12 @@ struct EriLaunch {
   int la, lb, lc, ld;
   const Quartet *quartets;
   long long nq;
+  // a launch sized before its list is built (a recorded build): nq is the
+  // capacity of the launch's part of the list, which the grid covers; the
+  // list's device counter holds the quartet count of the whole list, whose
+  // entries from nq_base on the launch takes
+  const unsigned long long *nq_dev;
+  long long nq_base;
   const PairRec *bra, *ket;
   const PrimPair *prims;
   const double *rys;
@@ -65,6 +71,9 @@ struct ListLaunch {
   const int *ket_lo;          // per bra: first ket of its molecule in the ket class
   int nbra;
   long long ncand;
+  // a list sized before the candidates are counted: ncand is its capacity
+  // and the candidate total is read here (the scan's last entry, absolute)
+  const long long *ncand_dev;
   // block norms of molecule m: Bg[boff + s1 + s2 ns], per matrix
   // Bm[(boff + s1 + s2 ns) nmat + k], boff = mol_boff[m], ns = mol_ns[m]
   const double *Bg;
@@ -82,7 +91,8 @@ struct ListLaunch {
   Quartet *out;
   unsigned long long *count;
   double *prim_sum;           // sum over survivors of Ka Kb Kc Kd
-  long long cap;
+  long long cap;              // entries out (out32) holds; the counters keep
+  long long cap32;            // counting past them
   // survivors with Schwarz product Q_ab Q_cd < cut32 go to the FP32 list
   // (out32 / count32 / prim_sum32) instead; cut32 = 0 keeps one list
   double cut32;
@@ -94,6 +104,8 @@ struct ListLaunch {
 // 1024-candidate chunks [start[k], start[k + 1]) of a common chunk space
 // (start: nlist + 1 entries on the device, nchunk = start[nlist]).
 constexpr int kListChunk = 1024;
+// dst[i] = src[idx[i]], i < n (one launch; reads back counters in one copy)
+hipError_t launch_gather(const long long *src, const long long *idx, int n, long long *dst, hipStream_t s);
 hipError_t launch_quartet_lists(const ListLaunch *lists, const long long *start, int nlist, long long nchunk,
                                 hipStream_t s);
 // pn[i (nmat + 1) + k] = block norm of pair i's own shells (k = 0 global,

@@ -54,6 +54,7 @@ void cqx_serve_default_opts(cqx_serve_opts *o) {
   o->list_cap = 1ll << 25;
   cqx_scf_default_opts(&o->scf);
   o->verbose = 0;
+  o->graph = 0;
 }
 
 int cqx_serve(const cqx_job *jobs, int njobs, const cqx_serve_opts *opts, cqx_job_result *out) {
@@ -84,6 +85,7 @@ int cqx_serve(const cqx_job *jobs, int njobs, const cqx_serve_opts *opts, cqx_jo
     cqx_scf *scf = nullptr;
     int rc = cqx_ctx_create(basis[j], opts->thresh, opts->device, &ctx);
     if (rc == CQX_OK) rc = cqx_set_option(ctx, "list_cap", (double)opts->list_cap);
+    if (rc == CQX_OK && opts->graph) rc = cqx_set_option(ctx, "graph", 1.0);
     if (rc == CQX_OK) {
       std::vector<double> Z(J.Z, J.Z + J.natom);
       cqx_scf_opts so = opts->scf;

@@ -158,6 +158,9 @@ typedef struct {
   int n_launch_eri;       /* ERI kernel launches */
   long long n_quartets_f32; /* of n_quartets, those evaluated in FP32 */
   double flops_model_f32;   /* of flops_model, the part on FP32 quartets */
+  int graph;                /* 0 launched eagerly, 1 eagerly and recorded into a graph,
+                               2 a replay of the recorded graph (ms_eri then covers
+                               the whole recorded build) */
 } cqx_stats;
 int cqx_last_stats(const cqx_ctx *ctx, cqx_stats *st);
 
@@ -188,7 +191,17 @@ int cqx_grad_two_body_contract(cqx_ctx *ctx, int ngrad, int nmat, const cqx_cont
  * quartet in FP64; classes with an f shell always stay FP64), "list_cap"
  * (screened quartet-list entries built per batch, default 2^28: the list
  * workspace is 8 B per entry, twice that with fp32_cut, whatever the
- * molecule size). */
+ * molecule size), "graph" (1: a build records its launches, from the
+ * shell-block norms to the accumulation into AX, into a HIP graph, and the
+ * next builds with the same contraction signature replay it with one host
+ * synchronisation; the list and ERI kernels read their counts from the
+ * device; a replay that finds more candidates than the recording build's
+ * capacities is discarded and built again eagerly; one contraction group
+ * (at most 8 contractions per molecule) and one list batch only, otherwise
+ * every build runs eagerly), "graph_margin" (capacity of each class pair's
+ * candidates and lists in a recorded graph, as a multiple of the recording
+ * build's candidates; default 1.25).  Setting any option drops a recorded
+ * graph. */
 int cqx_set_option(cqx_ctx *ctx, const char *name, double value);
 
 /* ---------------------------------------------------------------------
@@ -288,6 +301,9 @@ typedef struct {
   long long list_cap;      /* per-job quartet-list batch (see "list_cap"), default 2^25 */
   cqx_scf_opts scf;        /* per-job SCF options (nocc taken from the job) */
   int verbose;             /* 1: one line per finished job on stderr */
+  int graph;               /* 1: each job's context records its Fock build into a
+                              HIP graph (option "graph") and its later SCF
+                              iterations replay it; default 0 */
 } cqx_serve_opts;
 void cqx_serve_default_opts(cqx_serve_opts *o);
 

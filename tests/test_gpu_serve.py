@@ -47,3 +47,17 @@ def test_hbm_budget_serialises_admission():
     spans = sorted((r["t_admit"], r["t_done"]) for r in res)
     for (a0, a1), (b0, b1) in zip(spans, spans[1:]):
         assert b0 >= a1 - 1e-6  # one at a time
+
+
+def test_recorded_fock_builds_give_the_same_energies():
+    """graph=1: every job's later SCF iterations replay the Fock build its
+    context recorded; energies and iteration counts as without it."""
+    from chronusq_public_amd import cqx
+    jobs = _jobs([3, 5, 4], 0.0)
+    say("case: serve with recorded Fock builds")
+    plain = cqx.serve(jobs, max_inflight=3)
+    rec = cqx.serve(jobs, max_inflight=3, graph=1)
+    for a, b in zip(plain, rec):
+        assert a["status"] == 0 and b["status"] == 0 and a["converged"] and b["converged"]
+        assert abs(a["e_total"] - b["e_total"]) < 1e-8
+        assert abs(a["iterations"] - b["iterations"]) <= 1
