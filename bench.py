@@ -450,6 +450,23 @@ def main():
             "eri_ms_per_build": eri_avg,
         },
     }
+    # the same builds recorded once into a HIP graph and replayed (option
+    # graph): the first build records, the timed ones replay; the outputs
+    # are checked against the eager ones
+    if "graph" not in [o.split("=", 1)[0] for o in args.opt]:
+        progress(f"{args.workload}: recorded build, 1 warmup + {args.steps} timed replays")
+        tpi.set_option("graph", 1)
+        gdt, gms, _, _ = bench.timed(args.steps, 1)
+        gst = tpi.last_stats()
+        gout = bench.outputs()
+        line["graph"] = {
+            "builds_per_s": args.steps / gdt,
+            "ms_per_step": 1e3 * gdt / args.steps,
+            "ms_graph_per_build": gms,
+            "replayed": gst["graph"] == 2,
+            "max_abs_d_vs_eager": float(max(np.abs(a - b).max() for a, b in zip(gout, out))),
+        }
+        tpi.set_option("graph", 0)
     if args.mixed_steps > 0:
         line["mixed"] = mixed_comparison(args.mixed_steps, 1)
     if args.batch_mols > 0:
