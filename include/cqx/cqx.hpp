@@ -74,6 +74,14 @@ class DirectTPI {
   cqx_ctx *handle() const { return ctx_; }
   int nBasis() const { return nBasis_; }
   double threshSchwarz() const { return thresh_; }
+  // engine options (cqx_set_option: "graph", "fp32_cut", "list_cap", ...)
+  void setOption(const std::string &name, double value) { check(cqx_set_option(ctx_, name.c_str(), value), ctx_); }
+  // statistics of the last contraction (quartets, timings, graph replay)
+  cqx_stats lastStats() const {
+    cqx_stats st{};
+    check(cqx_last_stats(ctx_, &st), ctx_);
+    return st;
+  }
 
  private:
   cqx_ctx *ctx_ = nullptr;

@@ -22,13 +22,13 @@ def say(*a):
     print(*a, flush=True, file=sys.stderr)
 
 
-def _run(gto, Xs, types, hers):
+def _run(gto, Xs, types, hers, screen=True):
     from chronusq_public_amd import cqx
     cplx = any(np.iscomplexobj(x) for x in Xs)
     dt = np.complex128 if cplx else np.float64
     AX = [np.zeros(x.shape, dtype=dt, order="F") for x in Xs]
     gto.twoBodyContract([cqx.TwoBodyContraction(np.asarray(x, dtype=dt), a, h, t)
-                         for x, a, t, h in zip(Xs, AX, types, hers)])
+                         for x, a, t, h in zip(Xs, AX, types, hers)], screen=screen)
     return AX
 
 
@@ -158,3 +158,28 @@ def test_signature_change_and_options_build_eagerly():
     assert tg.last_stats()["graph"] == 1
     ref = _run(ge, [D], [COULOMB], [True])
     assert _maxdiff(got, ref) < TOL and _maxdiff(got2, ref) < TOL
+
+
+@pytest.mark.parametrize("case", ["nonhermitian", "unscreened"])
+def test_replays_of_nonhermitian_and_unscreened_builds(case):
+    from chronusq_public_amd import cqx
+    from chronusq_public_amd.chem import water_cluster
+    say("case: graph replays,", case)
+    b = cqx.BasisSet("6-31G(d)", water_cluster(3, 6.0, seed=8))
+    nb = b.nBasis
+    (tg, gg), (te, ge) = _pair(b)
+    rng = np.random.Generator(np.random.PCG64(15))
+    her = case != "nonhermitian"
+    screen = case != "unscreened"
+    for it in range(3):
+        if her:
+            Xs = [_density(rng, nb, 12)] * 2
+        else:
+            Xs = [np.asfortranarray(rng.normal(0, 1 / nb, size=(nb, nb))) for _ in range(2)]
+        got = _run(gg, Xs, [COULOMB, EXCHANGE], [her, her], screen)
+        mode = tg.last_stats()["graph"]
+        ref = _run(ge, Xs, [COULOMB, EXCHANGE], [her, her], screen)
+        err = _maxdiff(got, ref)
+        say(f"  build {it}: graph mode {mode} max|d| {err:.2e}")
+        assert mode == (1 if it == 0 else 2)
+        assert err < TOL
