@@ -737,7 +737,6 @@ int cqx_ctx_shell_pairs(cqx_ctx *c, int *npairs, int *s1, int *s2) {
 
 int cqx_set_option(cqx_ctx *c, const char *name, double value) {
   if (!c || !name) return fail(c, CQX_ERR_ARG, "cqx_set_option: bad arguments");
-  (void)hipSetDevice(c->device);
   drop_graph(c);  // every option changes what a recorded build would launch
   if (!std::strcmp(name, "graph")) {
     c->use_graph = value != 0.0;
