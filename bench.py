@@ -145,6 +145,14 @@ def traffic_per_build(workload):
         return json.load(f)["traffic_bytes_per_build"]
 
 
+def executed_flops_per_build(workload):
+    """Executed FP64 FLOPs per build of the ERI launches from the committed
+    rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 run of this workload, calibrated
+    against a kernel of known FMA count (profiles/eri_flops_counters.md);
+    None when that workload has no such measurement."""
+    return {"ghf16": 9.714e10}.get(workload)
+
+
 def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -467,6 +475,13 @@ def main():
             "max_abs_d_vs_eager": float(max(np.abs(a - b).max() for a, b in zip(gout, out))),
         }
         tpi.set_option("graph", 0)
+    xf = executed_flops_per_build(args.workload)
+    if xf:
+        # the same time priced with the FP64 FLOPs the kernels execute
+        line["roofline"]["executed_flops_per_build"] = xf
+        line["roofline"]["executed_frac"] = xf / (eri_avg * 1e-3) / (FP64_VECTOR_PEAK * 1e12)
+        line["roofline"]["executed_note"] = ("rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes per build, "
+                                             "calibrated on a known-FMA kernel (profiles/eri_flops_counters.md)")
     if args.mixed_steps > 0:
         line["mixed"] = mixed_comparison(args.mixed_steps, 1)
     if args.batch_mols > 0:
