@@ -94,7 +94,7 @@ struct cqx_ctx {
   int fast_lanes = 0;     // lanes per quartet of the lane-group kernels (0: per launch)
   int lds_max = 65536;    // LDS bytes a block may use on the device
   bool autotune_pending = false;  // time the lane counts of each class on the next call
-  std::map<int, int> tuned_lanes; // (class pair, precision) -> lanes per quartet
+  std::map<int, int> tuned_lanes; // (class pair, precision) -> lanes per quartet, or quartets per wave
   DevBuf<double> d_acc_tune;      // scratch accumulators of the timing launches
   DevBuf<cqx::MolPlanes> d_molplanes_tune;
   double fp32_cut = 0.0;  // quartets with Q_ab Q_cd below this are evaluated in FP32
@@ -773,8 +773,9 @@ int cqx_set_option(cqx_ctx *c, const char *name, double value) {
   }
   if (!std::strcmp(name, "autotune")) {
     // 1: the next contraction times every lane count of every lane-group
-    // launch on scratch accumulators and keeps the fastest per class pair
-    // and precision; 0: forget the choices
+    // launch, and every quartets-per-wave count of every wave launch, on
+    // scratch accumulators and keeps the fastest per class pair and
+    // precision; 0: forget the choices
     c->autotune_pending = value != 0.0;
     if (value == 0.0) c->tuned_lanes.clear();
     return CQX_OK;
@@ -1118,8 +1119,19 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         E.store_mode = 0;
         return E;
       };
-      auto lane_key = [](const cqx::EriLaunch &E) {
-        return ((((E.la * 4 + E.lb) * 4 + E.lc) * 4 + E.ld) << 1) | (E.fp32 ? 1 : 0);
+      // autotuned shape of a launch, per class pair and precision: the lanes
+      // per quartet of a lane-group kernel, the quartets per wave of a wave
+      // kernel
+      auto tune_key = [&](const cqx::EriLaunch &E) {
+        const int k = ((((E.la * 4 + E.lb) * 4 + E.lc) * 4 + E.ld) << 1) | (E.fp32 ? 1 : 0);
+        return route(E.la, E.lb, E.lc, E.ld) == 1 ? k + 4096 : k;
+      };
+      auto apply_tuned = [&](cqx::EriLaunch &E) {
+        const int r = route(E.la, E.lb, E.lc, E.ld);
+        auto it = c->tuned_lanes.find(tune_key(E));
+        if (it == c->tuned_lanes.end()) return;
+        if (r == 0 && !c->fast_lanes) E.force_g = it->second;
+        if (r == 1) E.force_qt = it->second;
       };
       // one list in launches of at most 2^25 quartets, so that no grid
       // exceeds 2^31 work-items (the wave kernels take 4 quartets per block)
@@ -1245,10 +1257,11 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           E.quartets = c->d_quart.p + (f32 ? cap_need : 0) + pc.seg;
           E.nq = (long long)counts[kk];
           E.mean_prims = psum[kk] / (double)counts[kk];
-          if (route(E.la, E.lb, E.lc, E.ld) == 0) {
-            const int key = lane_key(E);
-            if (c->autotune_pending && !c->fast_lanes && !c->tuned_lanes.count(key)) {
-              // time each lane count on scratch accumulators (results discarded)
+          const int rt = route(E.la, E.lb, E.lc, E.ld);
+          if (rt <= 1) {
+            const int key = tune_key(E);
+            if (c->autotune_pending && !(rt == 0 && c->fast_lanes) && !c->tuned_lanes.count(key)) {
+              // time each shape on scratch accumulators (results discarded)
               if (!tune_ready) {
                 c->d_acc_tune.ensure(std::max<size_t>(poff[nmol], 1));
                 std::vector<cqx::MolPlanes> mt = mp;
@@ -1263,11 +1276,18 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
               cqx::EriLaunch T = E;
               T.planes = ps_tune;
               const int nc = cqx::ncart(E.la) * cqx::ncart(E.lb) * cqx::ncart(E.lc) * cqx::ncart(E.ld);
+              std::vector<int> shapes;
+              if (rt == 0) {
+                for (int g : {1, 2, 4, 8, 16})
+                  if (!(g == 1 && nc > 9)) shapes.push_back(g);  // one lane per quartet: LDS tile too large
+              } else {
+                for (int qt : {1, 2, 4, 8})
+                  if (cqx::eri_wave_qt_built(E.la, E.lb, E.lc, E.ld, E.fp32, qt)) shapes.push_back(qt);
+              }
               float best = 1e30f;
               int best_g = 0;
-              for (int g : {1, 2, 4, 8, 16}) {
-                if (g == 1 && nc > 9) continue;  // LDS tile too large for one lane per quartet
-                T.force_g = g;
+              for (int g : shapes) {
+                (rt == 0 ? T.force_g : T.force_qt) = g;
                 float ms = 0.0f;
                 for (int rep = 0; rep < 2; ++rep) {  // the second run is timed
                   HIPCHK(hipEventRecord(c->ev_t0, s));
@@ -1283,8 +1303,7 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
               }
               c->tuned_lanes[key] = best_g;
             }
-            auto it = c->tuned_lanes.find(key);
-            if (!c->fast_lanes && it != c->tuned_lanes.end()) E.force_g = it->second;
+            apply_tuned(E);
           }
           launch_eri(E);
           const double fl = class_pair_flops(E.la, E.lb, E.lc, E.ld, pure_d, psum[kk], (double)counts[kk],
@@ -1402,10 +1421,7 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
             E.nq_dev = c->d_count.p + kk;
             const double n0 = cp_count[kk];
             E.mean_prims = n0 > 0.0 ? cp_psum[kk] / n0 : 1.0;
-            if (route(E.la, E.lb, E.lc, E.ld) == 0) {
-              auto it = c->tuned_lanes.find(lane_key(E));
-              if (!c->fast_lanes && it != c->tuned_lanes.end()) E.force_g = it->second;
-            }
+            apply_tuned(E);
             launch_eri(E);
           }
           back();

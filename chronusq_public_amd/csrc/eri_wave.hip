@@ -1,212 +1,50 @@
-// Wave-per-quartet Rys ERI + J/K digestion kernels for the larger s/p/d
-// classes (Cartesian quartet size > 27), templated on the class: the whole
-// wave evaluates one quartet at a time, up to 4 in turn, then digests them.
-//
-// Each round takes up to PP primitive quartets: lane group g (3 n_r lanes)
-// builds the 2D integrals of primitive quartet g, one lane per (root,
-// direction), each lane evaluating only its own Rys node and weight, and
-// stores them to the wave's LDS slice; then every lane accumulates its own
-// share of the contracted Cartesian quartet over the round's tables.  After the primitive loop
-// the quartet goes to LDS; after the wave's last quartet the lanes digest
-// them together (digest_tile.hpp).
+// Dispatch of the wave-per-quartet ERI kernels (eri_wave_impl.hpp): the
+// class, and the number of quartets a wave evaluates before digesting them
+// (forced, e.g. by the autotuner, or the class default).
 #include <hip/hip_runtime.h>
 
-#include "digest.hpp"
-#include "eri_fast_impl.hpp"
+#include "kernels.hpp"
 
 namespace cqx {
 
-namespace {
-
-constexpr int kWavesPerBlock = 4;
-
-// R is the evaluation precision; the quartet handed to the digestion and
-// the accumulators are FP64 either way.
-// Primitive quartets whose 2D tables one round builds side by side: as many
-// groups of 3 n_r lanes as fit in a wave, at most 8, within 8 KB of LDS.
-template <class K, class R>
-constexpr int prims_per_round() {
-  constexpr int by_lanes = 64 / (3 * K::NR);
-  constexpr int by_lds = 8192 / (int)(3 * K::NR * K::NH * sizeof(R));
-  constexpr int p = by_lanes < 8 ? by_lanes : 8;
-  return p < by_lds ? (p < 1 ? 1 : p) : (by_lds < 1 ? 1 : by_lds);
-}
-
-// Quartets a wave evaluates one after the other before digesting them
-// together: a single quartet gives an update term fewer items than lanes
-// (e.g. 18 for (dp|ps) with one plane), so most lanes of each digestion
-// step idle while it waits on memory; 4 quartets fill them.  The largest
-// classes keep one, since their LDS already limits the blocks per CU.
-template <class K>
-constexpr int quartets_per_wave() {
-  return K::NC <= 108 ? 4 : K::NC <= 216 ? 2 : 1;
-}
-
-template <int LA, int LB, int LC, int LD, class R>
-__global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch a) {
-  using K = Cls<LA, LB, LC, LD>;
-  constexpr int NOWN = (K::NC + 63) / 64;
-  constexpr int HSZ = 3 * K::NR * K::NH;
-  constexpr int PP = prims_per_round<K, R>();
-  constexpr int QT = quartets_per_wave<K>();
-  __shared__ R s_H[kWavesPerBlock][PP * HSZ];
-  __shared__ double s_I[kWavesPerBlock][QT * K::NC];
-  __shared__ int s_bf[kWavesPerBlock][QT][4];
-  __shared__ unsigned s_mask[kWavesPerBlock][QT];
-  __shared__ int s_mol[kWavesPerBlock][QT];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // quartet count: the list's device counter for a launch sized to the
-  // list's capacity before the list was built
-  const long long nq = a.nq_dev ? min((long long)*a.nq_dev - a.nq_base, a.nq) : a.nq;
-  const long long q0 = ((long long)blockIdx.x * kWavesPerBlock + wv) * QT;
-  if (q0 >= nq) return;  // uniform across the wave; no block barriers below
-  const int ntile = nq - q0 < QT ? (int)(nq - q0) : QT;
-  for (int t = 0; t < ntile; ++t) {
-    const long long q = q0 + t;
-    const Quartet qt = a.quartets[q];
-    const PairRec bp = a.bra[qt.bra];
-    const PairRec kp = a.ket[qt.ket_mask & 0xFFFFFFu];
-    R *H = s_H[wv];
-    double *sI = s_I[wv] + t * K::NC;
-    const R *__restrict__ rys = rys_table<R>(a);
-
-    // the lane's outputs o = lane + 64 k and their 2D-table offsets
-    int ox[NOWN], oy[NOWN], oz[NOWN];
-    R acc[NOWN];
-#pragma unroll
-    for (int k = 0; k < NOWN; ++k) {
-      acc[k] = 0;
-      int o = lane + 64 * k;
-      if (o >= K::NC) o = 0;
-      int rem = o;
-      const int id = rem % K::NCD; rem /= K::NCD;
-      const int ic = rem % K::NCC; rem /= K::NCC;
-      const int ib = rem % K::NCB;
-      const int ia = rem / K::NCB;
-      ox[k] = K::hidx(cexp(LA, ia, 0), cexp(LB, ib, 0), cexp(LC, ic, 0), cexp(LD, id, 0));
-      oy[k] = K::NH + K::hidx(cexp(LA, ia, 1), cexp(LB, ib, 1), cexp(LC, ic, 1), cexp(LD, id, 1));
-      oz[k] = 2 * K::NH + K::hidx(cexp(LA, ia, 2), cexp(LB, ib, 2), cexp(LC, ic, 2), cexp(LD, id, 2));
-    }
-    // lane group g = lane / (3 n_r) builds the 2D tables of primitive quartet
-    // k0 + g of the round; its lanes (root, direction) one table each
-    const int grp = lane / (3 * K::NR), sub = lane - grp * (3 * K::NR);
-    const int my_r = sub / 3, my_d = sub - my_r * 3;
-    const int nk = kp.npp, ntot = bp.npp * kp.npp;
-    for (int k0 = 0; k0 < ntot; k0 += PP) {
-      const int cnt = ntot - k0 < PP ? ntot - k0 : PP;
-      if (grp < cnt) {
-        const int kq = k0 + grp;
-        const int i = kq / nk, j = kq - i * nk;
-        const PrimPair A = a.prims[bp.pp_off + i];
-        const PrimPair B = a.prims[kp.pp_off + j];
-        const R ap = (R)A.p, bq = (R)B.p;
-        const PrimRecip<R> pr = prim_recip<R>(ap, bq, A.K * B.K);
-        const R rpq = pr.rpq, pref = pr.pref;
-        const R rho = ap * bq * rpq;
-        // centre differences in FP64 before any rounding to R
-        const R PQ[3] = {(R)(A.P[0] - B.P[0]), (R)(A.P[1] - B.P[1]), (R)(A.P[2] - B.P[2])};
-        const R T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
-        R t2, wr;
-        rys_root<K::NR, R>(T, rys, my_r, t2, wr);
-        const R rp = bq * pr.rab, rq = ap * pr.rab;
-        const R B00 = (R)0.5 * t2 * rpq;
-        const R B10 = (R)0.5 * rp * ((R)1 - rho * rp * t2);
-        const R B01 = (R)0.5 * rq * ((R)1 - rho * rq * t2);
-        const R PQd = my_d == 0 ? PQ[0] : (my_d == 1 ? PQ[1] : PQ[2]);
-        const R PAd = (R)(my_d == 0 ? A.PA[0] : (my_d == 1 ? A.PA[1] : A.PA[2]));
-        const R QCd = (R)(my_d == 0 ? B.PA[0] : (my_d == 1 ? B.PA[1] : B.PA[2]));
-        const R ABd = (R)(my_d == 0 ? bp.AB[0] : (my_d == 1 ? bp.AB[1] : bp.AB[2]));
-        const R CDd = (R)(my_d == 0 ? kp.AB[0] : (my_d == 1 ? kp.AB[1] : kp.AB[2]));
-        const R C00 = PAd - bq * rpq * t2 * PQd;
-        const R D00 = QCd + ap * rpq * t2 * PQd;
-        R h[K::NH];
-        rys_2d_full<LA, LB, LC, LD, R>(C00, D00, B10, B01, B00, ABd, CDd, my_d == 2 ? pref * wr : (R)1, h);
-        R *dst = H + grp * HSZ + (my_r * 3 + my_d) * K::NH;
-#pragma unroll
-        for (int e = 0; e < K::NH; ++e) dst[e] = h[e];
-      }
-      wave_sync();
-#pragma unroll
-      for (int k = 0; k < NOWN; ++k) {
-        if (lane + 64 * k >= K::NC) continue;
-        R s = 0;
-        for (int g = 0; g < cnt; ++g) {
-#pragma unroll
-          for (int r = 0; r < K::NR; ++r) {
-            const R *Hr = H + g * HSZ + r * 3 * K::NH;
-            s = fma(Hr[ox[k]] * Hr[oy[k]], Hr[oz[k]], s);
-          }
-        }
-        acc[k] += s;
-      }
-      wave_sync();
-    }
-    // degeneracy (direct.hpp:1664-1667, 1793-1802) and the 1/2 of :1849-1850
-    const bool same_pair = (bp.a == kp.a && bp.b == kp.b);
-    const double scale = 0.5 * (bp.a == bp.b ? 1.0 : 2.0) * (kp.a == kp.b ? 1.0 : 2.0) * (same_pair ? 1.0 : 2.0);
-#pragma unroll
-    for (int k = 0; k < NOWN; ++k)
-      if (lane + 64 * k < K::NC) sI[lane + 64 * k] = (double)acc[k] * scale;
-    if (lane == 0) {
-      s_bf[wv][t][0] = bp.bfa;
-      s_bf[wv][t][1] = bp.bfb;
-      s_bf[wv][t][2] = kp.bfa;
-      s_bf[wv][t][3] = kp.bfb;
-      s_mask[wv][t] = qt.ket_mask >> 24;
-      s_mol[wv][t] = bp.mol;
-    }
-  }
-  wave_sync();
-  digest_tile<K>(s_I[wv], s_bf[wv], s_mask[wv], s_mol[wv], ntile, a.planes, lane);
-}
-
-template <int LA, int LB, int LC, int LD>
-hipError_t launch_wave_cls(const EriLaunch &a, hipStream_t s) {
-  constexpr long long per_block = (long long)kWavesPerBlock * quartets_per_wave<Cls<LA, LB, LC, LD>>();
-  const long long blocks = (a.nq + per_block - 1) / per_block;
-  if (a.fp32)
-    hipLaunchKernelGGL((eri_wave_kernel<LA, LB, LC, LD, float>), dim3((unsigned)blocks),
-                       dim3(64 * kWavesPerBlock), 0, s, a);
-  else
-    hipLaunchKernelGGL((eri_wave_kernel<LA, LB, LC, LD, double>), dim3((unsigned)blocks),
-                       dim3(64 * kWavesPerBlock), 0, s, a);
-  return hipGetLastError();
-}
-
-}  // namespace
-
-// s/p/d classes with more than 27 Cartesian components
-#define CQX_WAVE_CLASSES(X) \
-  X(1, 1, 1, 1)             \
-  X(2, 0, 1, 1)             \
-  X(2, 0, 2, 0)             \
-  X(2, 1, 1, 0)             \
-  X(2, 1, 1, 1)             \
-  X(2, 1, 2, 0)             \
-  X(2, 1, 2, 1)             \
-  X(2, 2, 0, 0)             \
-  X(2, 2, 1, 0)             \
-  X(2, 2, 1, 1)             \
-  X(2, 2, 2, 0)             \
-  X(2, 2, 2, 1)             \
-  X(2, 2, 2, 2)
+template <int QT>
+hipError_t launch_wave_q(const EriLaunch &a, hipStream_t s);
+template <int QT>
+bool wave_q_built(int la, int lb, int lc, int ld, bool fp32);
 
 bool eri_wave_available(int la, int lb, int lc, int ld) {
-#define CQX_CASE(A, B, C, D) \
-  if (la == A && lb == B && lc == C && ld == D) return true;
-  CQX_WAVE_CLASSES(CQX_CASE)
-#undef CQX_CASE
-  return false;
+  return wave_q_built<1>(la, lb, lc, ld, false);
+}
+
+bool eri_wave_qt_built(int la, int lb, int lc, int ld, int fp32, int qt) {
+  switch (qt) {
+    case 1: return wave_q_built<1>(la, lb, lc, ld, fp32 != 0);
+    case 2: return wave_q_built<2>(la, lb, lc, ld, fp32 != 0);
+    case 4: return wave_q_built<4>(la, lb, lc, ld, fp32 != 0);
+    case 8: return wave_q_built<8>(la, lb, lc, ld, fp32 != 0);
+    default: return false;
+  }
+}
+
+// 4 quartets per wave for Cartesian quartets of at most 108 components, 2
+// up to 216, else 1 (measured on the GHF and C40H82 builds)
+int eri_wave_default_qt(int la, int lb, int lc, int ld) {
+  auto nc = [](int l) { return (l + 1) * (l + 2) / 2; };
+  const int n = nc(la) * nc(lb) * nc(lc) * nc(ld);
+  return n <= 108 ? 4 : n <= 216 ? 2 : 1;
 }
 
 hipError_t launch_eri_wave(const EriLaunch &a, hipStream_t s) {
   if (a.nq <= 0) return hipSuccess;
   if (a.store_mode) return hipErrorNotSupported;
-#define CQX_CASE(A, B, C, D) \
-  if (a.la == A && a.lb == B && a.lc == C && a.ld == D) return launch_wave_cls<A, B, C, D>(a, s);
-  CQX_WAVE_CLASSES(CQX_CASE)
-#undef CQX_CASE
-  return hipErrorNotSupported;
+  int qt = a.force_qt;
+  if (!eri_wave_qt_built(a.la, a.lb, a.lc, a.ld, a.fp32, qt)) qt = eri_wave_default_qt(a.la, a.lb, a.lc, a.ld);
+  switch (qt) {
+    case 1: return launch_wave_q<1>(a, s);
+    case 2: return launch_wave_q<2>(a, s);
+    case 4: return launch_wave_q<4>(a, s);
+    default: return launch_wave_q<8>(a, s);
+  }
 }
 
 }  // namespace cqx

@@ -26,6 +26,7 @@ struct EriLaunch {
   int fp32;                      // evaluate the quartets in FP32 (digestion stays FP64)
   double mean_prims;             // mean primitive quartets per listed shell quartet
   int force_g;                   // lanes per quartet of the lane-group kernels (0: chosen per launch)
+  int force_qt;                  // quartets per wave of the wave kernels (0: the class default)
   PlaneSet planes;
   double *store;                 // STORE mode: contracted Cartesian quartets
   const long long *store_off;    // per quartet offset into store
@@ -41,6 +42,9 @@ bool eri_fast_available(int la, int lb, int lc, int ld);
 // One-wave-per-quartet templated kernels for the larger s/p/d classes.
 hipError_t launch_eri_wave(const EriLaunch &a, hipStream_t s);
 bool eri_wave_available(int la, int lb, int lc, int ld);
+// whether the wave kernel of a class is built with qt (1, 2, 4, 8) quartets
+// per wave for the precision (its LDS per block within the limit)
+bool eri_wave_qt_built(int la, int lb, int lc, int ld, int fp32, int qt);
 
 // Nuclear-gradient J/K contraction over every quartet of one class pair
 // (no screening, as directScaffoldGrad): quartet q is (bra[i] | ket[j]) with

@@ -183,9 +183,10 @@ int cqx_grad_two_body_contract(cqx_ctx *ctx, int ngrad, int nmat, const cqx_cont
  * wave-per-quartet ones), "fast_lanes" (lanes per shell quartet of the
  * lane-group kernels: 1, 2, 4, 8 or 16; 0, the default, picks it per launch
  * from the mean primitive count), "autotune" (1: the next contraction
- * times every lane count of every lane-group launch on scratch
- * accumulators, keeps the fastest per class pair and precision and uses it
- * from then on; 0 forgets the choices), "fp32_cut" (mixed precision: quartets whose
+ * times every lane count of every lane-group launch and every number of
+ * quartets a wave kernel evaluates before digesting them (1, 2, 4, 8) on
+ * scratch accumulators, keeps the fastest per class pair and precision and
+ * uses it from then on; 0 forgets the choices), "fp32_cut" (mixed precision: quartets whose
  * Schwarz product Q_ab Q_cd is below this value are evaluated in FP32 with
  * an FP32 Rys table, digested and accumulated in FP64; default 0 = every
  * quartet in FP64; classes with an f shell always stay FP64), "list_cap"

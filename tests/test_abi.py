@@ -67,7 +67,7 @@ def test_every_spd_class_has_a_specialised_kernel():
         for k in pairs[:bi + 1]:
             cls = f"{b[0]}, {b[1]}, {k[0]}, {k[1]}"
             for r in ("double", "float"):
-                if (f"eri_wave_kernel<{cls}, {r}>" not in syms
+                if (f"eri_wave_kernel<{cls}, {r}, 1>" not in syms
                         and f"eri_fast_kernel<{cls}, 16, {r}>" not in syms):
                     missing.append((cls, r))
     assert not missing, missing
