@@ -1,15 +1,14 @@
-// Wave-per-quartet Rys ERI + J/K digestion kernels for the larger s/p/d
-// classes (Cartesian quartet size > 27), templated on the class: the whole
-// wave evaluates one quartet at a time, up to 4 in turn, then digests them.
+// Wave kernels for the larger s/p/d classes (Cartesian quartet size > 27),
+// templated on the class and on QT, the quartets a wave takes together.
 //
-// Each round takes up to PP primitive quartets: lane group g (3 n_r lanes)
-// builds the 2D integrals of primitive quartet g, one lane per (root,
+// The primitive quartets of the wave's QT quartets form one list.  Each
+// round takes up to PP of them: lane group g (3 n_r lanes) builds the 2D
+// integrals of the round's g-th primitive quartet, one lane per (root,
 // direction), each lane evaluating only its own Rys node and weight, and
-// stores them to the wave's LDS slice; then every lane accumulates its own
-// share of the contracted Cartesian quartet over the round's tables.  After the primitive loop
-// the quartet goes to LDS; after the wave's last quartet the lanes digest
-// them together (digest_tile.hpp).  Instantiated per number of quartets a
-// wave evaluates before digesting (eri_wave_q*.hip).
+// stores them to the wave's LDS slice; then every lane adds the round's
+// tables to its share of the contracted Cartesian quartet they belong to.
+// After the last round the QT quartets go to LDS and the lanes digest them
+// together (digest_tile.hpp).  Instantiated per QT (eri_wave_q*.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -38,13 +37,17 @@ constexpr int prims_per_round() {
 // LDS of a block of the kernel below with QT quartets per wave
 template <class K, class R, int QT>
 constexpr int wave_lds_bytes() {
-  return kWavesPerBlock * (prims_per_round<K, R>() * 3 * K::NR * K::NH * (int)sizeof(R) + QT * K::NC * 8 + QT * 24);
+  return kWavesPerBlock * (prims_per_round<K, R>() * 3 * K::NR * K::NH * (int)sizeof(R) + QT * K::NC * 8 + QT * 24 +
+                          QT * 2 * (int)sizeof(PairRec));
 }
 
-// QT: quartets a wave evaluates one after the other before digesting them
-// together.  A single quartet gives an update term fewer items than lanes
-// (e.g. 18 for (dp|ps) with one plane), so most lanes of each digestion
-// step idle while it waits on memory; several quartets fill them.
+// QT: quartets a wave takes together.  Their primitive quartets form one
+// list that the rounds walk through, so the lane groups of a round stay busy
+// when a quartet has only a few primitive quartets, and each lane keeps its
+// outputs of every quartet.  The QT quartets are then digested together: a
+// single quartet gives an update term fewer items than lanes (e.g. 18 for
+// (dp|ps) with one plane), so most lanes of a digestion step would idle
+// while it waits on memory.
 template <int LA, int LB, int LC, int LD, class R, int QT>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch a) {
   using K = Cls<LA, LB, LC, LD>;
@@ -56,6 +59,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
   __shared__ int s_bf[kWavesPerBlock][QT][4];
   __shared__ unsigned s_mask[kWavesPerBlock][QT];
   __shared__ int s_mol[kWavesPerBlock][QT];
+  __shared__ PairRec s_pr[kWavesPerBlock][QT][2];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // quartet count: the list's device counter for a launch sized to the
   // list's capacity before the list was built
@@ -63,100 +67,127 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
   const long long q0 = ((long long)blockIdx.x * kWavesPerBlock + wv) * QT;
   if (q0 >= nq) return;  // uniform across the wave; no block barriers below
   const int ntile = nq - q0 < QT ? (int)(nq - q0) : QT;
-  for (int t = 0; t < ntile; ++t) {
-    const long long q = q0 + t;
-    const Quartet qt = a.quartets[q];
+  // lane t < ntile stages quartet t's pair records
+  if (lane < ntile) {
+    const Quartet qt = a.quartets[q0 + lane];
     const PairRec bp = a.bra[qt.bra];
     const PairRec kp = a.ket[qt.ket_mask & 0xFFFFFFu];
-    R *H = s_H[wv];
-    double *sI = s_I[wv] + t * K::NC;
-    const R *__restrict__ rys = rys_table<R>(a);
+    s_pr[wv][lane][0] = bp;
+    s_pr[wv][lane][1] = kp;
+    s_bf[wv][lane][0] = bp.bfa;
+    s_bf[wv][lane][1] = bp.bfb;
+    s_bf[wv][lane][2] = kp.bfa;
+    s_bf[wv][lane][3] = kp.bfb;
+    s_mask[wv][lane] = qt.ket_mask >> 24;
+    s_mol[wv][lane] = bp.mol;
+  }
+  wave_sync();
+  // offsets of the quartets' primitive quartets in the tile's list
+  int off[QT + 1];
+  off[0] = 0;
+#pragma unroll
+  for (int t = 0; t < QT; ++t)
+    off[t + 1] = off[t] + (t < ntile ? s_pr[wv][t][0].npp * s_pr[wv][t][1].npp : 0);
+  const int ntot = off[QT];
+  auto quartet_of = [&](int kq) {
+    int t = 0;
+#pragma unroll
+    for (int u = 1; u < QT; ++u)
+      if (kq >= off[u]) t = u;
+    return t;
+  };
+  R *H = s_H[wv];
+  const R *__restrict__ rys = rys_table<R>(a);
 
-    // the lane's outputs o = lane + 64 k and their 2D-table offsets
-    int ox[NOWN], oy[NOWN], oz[NOWN];
-    R acc[NOWN];
+  // the lane's outputs o = lane + 64 k and their 2D-table offsets
+  int ox[NOWN], oy[NOWN], oz[NOWN];
+  R acc[QT][NOWN];
 #pragma unroll
-    for (int k = 0; k < NOWN; ++k) {
-      acc[k] = 0;
-      int o = lane + 64 * k;
-      if (o >= K::NC) o = 0;
-      int rem = o;
-      const int id = rem % K::NCD; rem /= K::NCD;
-      const int ic = rem % K::NCC; rem /= K::NCC;
-      const int ib = rem % K::NCB;
-      const int ia = rem / K::NCB;
-      ox[k] = K::hidx(cexp(LA, ia, 0), cexp(LB, ib, 0), cexp(LC, ic, 0), cexp(LD, id, 0));
-      oy[k] = K::NH + K::hidx(cexp(LA, ia, 1), cexp(LB, ib, 1), cexp(LC, ic, 1), cexp(LD, id, 1));
-      oz[k] = 2 * K::NH + K::hidx(cexp(LA, ia, 2), cexp(LB, ib, 2), cexp(LC, ic, 2), cexp(LD, id, 2));
+  for (int k = 0; k < NOWN; ++k) {
+#pragma unroll
+    for (int t = 0; t < QT; ++t) acc[t][k] = 0;
+    int o = lane + 64 * k;
+    if (o >= K::NC) o = 0;
+    int rem = o;
+    const int id = rem % K::NCD; rem /= K::NCD;
+    const int ic = rem % K::NCC; rem /= K::NCC;
+    const int ib = rem % K::NCB;
+    const int ia = rem / K::NCB;
+    ox[k] = K::hidx(cexp(LA, ia, 0), cexp(LB, ib, 0), cexp(LC, ic, 0), cexp(LD, id, 0));
+    oy[k] = K::NH + K::hidx(cexp(LA, ia, 1), cexp(LB, ib, 1), cexp(LC, ic, 1), cexp(LD, id, 1));
+    oz[k] = 2 * K::NH + K::hidx(cexp(LA, ia, 2), cexp(LB, ib, 2), cexp(LC, ic, 2), cexp(LD, id, 2));
+  }
+  // lane group g = lane / (3 n_r) builds the 2D tables of primitive quartet
+  // k0 + g of the tile's list; its lanes (root, direction) one table each
+  const int grp = lane / (3 * K::NR), sub = lane - grp * (3 * K::NR);
+  const int my_r = sub / 3, my_d = sub - my_r * 3;
+  for (int k0 = 0; k0 < ntot; k0 += PP) {
+    const int cnt = ntot - k0 < PP ? ntot - k0 : PP;
+    if (grp < cnt) {
+      const int kq = k0 + grp;
+      const int t = quartet_of(kq);
+      const PairRec &bp = s_pr[wv][t][0];
+      const PairRec &kp = s_pr[wv][t][1];
+      const int loc = kq - off[t], nk = kp.npp;
+      const int i = loc / nk, j = loc - i * nk;
+      const PrimPair A = a.prims[bp.pp_off + i];
+      const PrimPair B = a.prims[kp.pp_off + j];
+      const R ap = (R)A.p, bq = (R)B.p;
+      const PrimRecip<R> pr = prim_recip<R>(ap, bq, A.K * B.K);
+      const R rpq = pr.rpq, pref = pr.pref;
+      const R rho = ap * bq * rpq;
+      // centre differences in FP64 before any rounding to R
+      const R PQ[3] = {(R)(A.P[0] - B.P[0]), (R)(A.P[1] - B.P[1]), (R)(A.P[2] - B.P[2])};
+      const R T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+      R t2, wr;
+      rys_root<K::NR, R>(T, rys, my_r, t2, wr);
+      const R rp = bq * pr.rab, rq = ap * pr.rab;
+      const R B00 = (R)0.5 * t2 * rpq;
+      const R B10 = (R)0.5 * rp * ((R)1 - rho * rp * t2);
+      const R B01 = (R)0.5 * rq * ((R)1 - rho * rq * t2);
+      const R PQd = my_d == 0 ? PQ[0] : (my_d == 1 ? PQ[1] : PQ[2]);
+      const R PAd = (R)(my_d == 0 ? A.PA[0] : (my_d == 1 ? A.PA[1] : A.PA[2]));
+      const R QCd = (R)(my_d == 0 ? B.PA[0] : (my_d == 1 ? B.PA[1] : B.PA[2]));
+      const R ABd = (R)(my_d == 0 ? bp.AB[0] : (my_d == 1 ? bp.AB[1] : bp.AB[2]));
+      const R CDd = (R)(my_d == 0 ? kp.AB[0] : (my_d == 1 ? kp.AB[1] : kp.AB[2]));
+      const R C00 = PAd - bq * rpq * t2 * PQd;
+      const R D00 = QCd + ap * rpq * t2 * PQd;
+      R h[K::NH];
+      rys_2d_full<LA, LB, LC, LD, R>(C00, D00, B10, B01, B00, ABd, CDd, my_d == 2 ? pref * wr : (R)1, h);
+      R *dst = H + grp * HSZ + (my_r * 3 + my_d) * K::NH;
+#pragma unroll
+      for (int e = 0; e < K::NH; ++e) dst[e] = h[e];
     }
-    // lane group g = lane / (3 n_r) builds the 2D tables of primitive quartet
-    // k0 + g of the round; its lanes (root, direction) one table each
-    const int grp = lane / (3 * K::NR), sub = lane - grp * (3 * K::NR);
-    const int my_r = sub / 3, my_d = sub - my_r * 3;
-    const int nk = kp.npp, ntot = bp.npp * kp.npp;
-    for (int k0 = 0; k0 < ntot; k0 += PP) {
-      const int cnt = ntot - k0 < PP ? ntot - k0 : PP;
-      if (grp < cnt) {
-        const int kq = k0 + grp;
-        const int i = kq / nk, j = kq - i * nk;
-        const PrimPair A = a.prims[bp.pp_off + i];
-        const PrimPair B = a.prims[kp.pp_off + j];
-        const R ap = (R)A.p, bq = (R)B.p;
-        const PrimRecip<R> pr = prim_recip<R>(ap, bq, A.K * B.K);
-        const R rpq = pr.rpq, pref = pr.pref;
-        const R rho = ap * bq * rpq;
-        // centre differences in FP64 before any rounding to R
-        const R PQ[3] = {(R)(A.P[0] - B.P[0]), (R)(A.P[1] - B.P[1]), (R)(A.P[2] - B.P[2])};
-        const R T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
-        R t2, wr;
-        rys_root<K::NR, R>(T, rys, my_r, t2, wr);
-        const R rp = bq * pr.rab, rq = ap * pr.rab;
-        const R B00 = (R)0.5 * t2 * rpq;
-        const R B10 = (R)0.5 * rp * ((R)1 - rho * rp * t2);
-        const R B01 = (R)0.5 * rq * ((R)1 - rho * rq * t2);
-        const R PQd = my_d == 0 ? PQ[0] : (my_d == 1 ? PQ[1] : PQ[2]);
-        const R PAd = (R)(my_d == 0 ? A.PA[0] : (my_d == 1 ? A.PA[1] : A.PA[2]));
-        const R QCd = (R)(my_d == 0 ? B.PA[0] : (my_d == 1 ? B.PA[1] : B.PA[2]));
-        const R ABd = (R)(my_d == 0 ? bp.AB[0] : (my_d == 1 ? bp.AB[1] : bp.AB[2]));
-        const R CDd = (R)(my_d == 0 ? kp.AB[0] : (my_d == 1 ? kp.AB[1] : kp.AB[2]));
-        const R C00 = PAd - bq * rpq * t2 * PQd;
-        const R D00 = QCd + ap * rpq * t2 * PQd;
-        R h[K::NH];
-        rys_2d_full<LA, LB, LC, LD, R>(C00, D00, B10, B01, B00, ABd, CDd, my_d == 2 ? pref * wr : (R)1, h);
-        R *dst = H + grp * HSZ + (my_r * 3 + my_d) * K::NH;
-#pragma unroll
-        for (int e = 0; e < K::NH; ++e) dst[e] = h[e];
-      }
-      wave_sync();
+    wave_sync();
+    for (int g = 0; g < cnt; ++g) {
+      const int t = quartet_of(k0 + g);  // uniform across the wave
 #pragma unroll
       for (int k = 0; k < NOWN; ++k) {
         if (lane + 64 * k >= K::NC) continue;
         R s = 0;
-        for (int g = 0; g < cnt; ++g) {
 #pragma unroll
-          for (int r = 0; r < K::NR; ++r) {
-            const R *Hr = H + g * HSZ + r * 3 * K::NH;
-            s = fma(Hr[ox[k]] * Hr[oy[k]], Hr[oz[k]], s);
-          }
+        for (int r = 0; r < K::NR; ++r) {
+          const R *Hr = H + g * HSZ + r * 3 * K::NH;
+          s = fma(Hr[ox[k]] * Hr[oy[k]], Hr[oz[k]], s);
         }
-        acc[k] += s;
+#pragma unroll
+        for (int u = 0; u < QT; ++u)
+          if (u == t) acc[u][k] += s;
       }
-      wave_sync();
     }
-    // degeneracy (direct.hpp:1664-1667, 1793-1802) and the 1/2 of :1849-1850
+    wave_sync();
+  }
+  // degeneracy (direct.hpp:1664-1667, 1793-1802) and the 1/2 of :1849-1850
+#pragma unroll
+  for (int t = 0; t < QT; ++t) {
+    if (t >= ntile) break;
+    const PairRec &bp = s_pr[wv][t][0];
+    const PairRec &kp = s_pr[wv][t][1];
     const bool same_pair = (bp.a == kp.a && bp.b == kp.b);
     const double scale = 0.5 * (bp.a == bp.b ? 1.0 : 2.0) * (kp.a == kp.b ? 1.0 : 2.0) * (same_pair ? 1.0 : 2.0);
 #pragma unroll
     for (int k = 0; k < NOWN; ++k)
-      if (lane + 64 * k < K::NC) sI[lane + 64 * k] = (double)acc[k] * scale;
-    if (lane == 0) {
-      s_bf[wv][t][0] = bp.bfa;
-      s_bf[wv][t][1] = bp.bfb;
-      s_bf[wv][t][2] = kp.bfa;
-      s_bf[wv][t][3] = kp.bfb;
-      s_mask[wv][t] = qt.ket_mask >> 24;
-      s_mol[wv][t] = bp.mol;
-    }
+      if (lane + 64 * k < K::NC) s_I[wv][t * K::NC + lane + 64 * k] = (double)acc[t][k] * scale;
   }
   wave_sync();
   digest_tile<K>(s_I[wv], s_bf[wv], s_mask[wv], s_mol[wv], ntile, a.planes, lane);
