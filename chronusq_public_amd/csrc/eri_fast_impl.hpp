@@ -85,17 +85,19 @@ __device__ __forceinline__ void rys_2d_full(R C00, R D00, R B10, R B01, R B00, R
   if constexpr (K::NI > 1) G[1][0] = C00 * scale;
 #pragma unroll
   for (int i = 1; i + 1 < K::NI; ++i) G[i + 1][0] = C00 * G[i][0] + i * B10 * G[i - 1][0];
+  if constexpr (K::NK > 1) {
 #pragma unroll
-  for (int k = 0; k + 1 < K::NK; ++k) {
-    if (k == 0) {
-      G[0][1] = D00 * G[0][0];
+    for (int k = 0; k + 1 < K::NK; ++k) {
+      if (k == 0) {
+        G[0][1] = D00 * G[0][0];
 #pragma unroll
-      for (int i = 1; i < K::NI; ++i) G[i][1] = D00 * G[i][0] + i * B00 * G[i - 1][0];
-    } else {
-      G[0][k + 1] = D00 * G[0][k] + k * B01 * G[0][k - 1];
+        for (int i = 1; i < K::NI; ++i) G[i][1] = D00 * G[i][0] + i * B00 * G[i - 1][0];
+      } else {
+        G[0][k + 1] = D00 * G[0][k] + k * B01 * G[0][k - 1];
 #pragma unroll
-      for (int i = 1; i < K::NI; ++i)
-        G[i][k + 1] = D00 * G[i][k] + k * B01 * G[i][k - 1] + i * B00 * G[i - 1][k];
+        for (int i = 1; i < K::NI; ++i)
+          G[i][k + 1] = D00 * G[i][k] + k * B01 * G[i][k - 1] + i * B00 * G[i - 1][k];
+      }
     }
   }
   // bra transfer: E[i][j][k], (i, j+1) = (i+1, j) + AB (i, j)

@@ -395,7 +395,7 @@ __global__ void pair_norm_kernel(const PairScr *pairs, int npair, const double *
   for (int m = 0; m < nmat; ++m) pn[(size_t)i * (nmat + 1) + 1 + m] = Bm[b * nmat + m];
 }
 
-__global__ void block_norm_finish_kernel(int nmat, int ns, const int *her, double *Bm, double *Bg) {
+__global__ void block_norm_finish_kernel(int nmat, int ns, const double *Bm, double *Bg) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)ns * ns) return;
   const int k = (int)(t % ns), l = (int)(t / ns);
@@ -583,7 +583,7 @@ hipError_t launch_block_norms(const double *X, int nmat, int is_complex, const i
   hipLaunchKernelGGL(block_norm_symm_kernel, dim3(nblocks(n1, 256)), dim3(256), 0, s, nmat, ns,
                      her, Bm);
   hipLaunchKernelGGL(block_norm_finish_kernel, dim3(nblocks((long long)ns * ns, 256)), dim3(256),
-                     0, s, nmat, ns, her, Bm, Bg);
+                     0, s, nmat, ns, Bm, Bg);
   return hipGetLastError();
 }
 
