@@ -186,3 +186,20 @@ def test_translation_invariance_and_positivity():
     assert np.abs(Ja - Jb).max() < 1e-10 and np.abs(Ka - Kb).max() < 1e-10
     assert np.sum(D * Ja) > 0  # Coulomb metric is positive definite
     assert np.sum(D * Ka) > 0
+
+
+def test_bilinear_forms_are_symmetric():
+    """sum(X * J[Y]) = sum(Y * J[X]) and sum(X * K[Y]) = sum(Y * K[X]) for
+    general complex non-Hermitian X, Y (real orbitals: (mn|kl) = (kl|mn)),
+    unscreened: the non-Hermitian digestion of every permutational update
+    must feed both forms alike."""
+    from chronusq_public_amd.chem import basis_text, water_cluster
+    mol = water_cluster(2, 4.0, seed=0)
+    ob = OracleBasis(basis_text("cc-pVDZ"), mol.Z, mol.xyz)
+    n = ob.nbf
+    rng = np.random.Generator(np.random.PCG64(5))
+    X = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    Y = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    JX, KX, JY, KY = ob.contract([X, X, Y, Y], [COULOMB, EXCHANGE] * 2, [False] * 4, screen=False)
+    for a, c in ((np.sum(X * JY), np.sum(Y * JX)), (np.sum(X * KY), np.sum(Y * KX))):
+        assert abs(a - c) <= 1e-12 * abs(a)
