@@ -144,6 +144,15 @@ struct cqx_ctx {
   DevBuf<double> d_store;
   DevBuf<long long> d_store_off;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
+  // Concurrent class-pair launches (option "streams"): a build, eager or
+  // recorded, deals its ERI launches round-robin over the context stream and
+  // nstreams - 1 side streams, forked from and joined back into the context
+  // stream, so that one launch's last waves overlap the next launch's first
+  // (the launches only add into the accumulators, with atomics)
+  static constexpr int kMaxSide = 3;
+  int nstreams = 4;
+  hipStream_t side[kMaxSide] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[kMaxSide] = {};
 
   // Captured build (option "graph"): the launches of one build, from the
   // shell-block norms to the accumulation into AX, recorded once into a HIP
@@ -711,6 +720,12 @@ int cqx_ctx_destroy(cqx_ctx *c) {
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->ev_t0) (void)hipEventDestroy(c->ev_t0);
   if (c->ev_t1) (void)hipEventDestroy(c->ev_t1);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  for (int i = 0; i < cqx_ctx::kMaxSide; ++i) {
+    if (c->side[i]) (void)hipStreamSynchronize(c->side[i]);
+    if (c->side[i]) (void)hipStreamDestroy(c->side[i]);
+    if (c->ev_join[i]) (void)hipEventDestroy(c->ev_join[i]);
+  }
   hipStream_t s = c->stream;
   delete c;
   if (s) (void)hipStreamDestroy(s);
@@ -778,6 +793,13 @@ int cqx_set_option(cqx_ctx *c, const char *name, double value) {
     // precision; 0: forget the choices
     c->autotune_pending = value != 0.0;
     if (value == 0.0) c->tuned_lanes.clear();
+    return CQX_OK;
+  }
+  if (!std::strcmp(name, "streams")) {
+    const int n = (int)value;
+    if (!(value == (double)n && n >= 1 && n <= cqx_ctx::kMaxSide + 1))
+      return fail(c, CQX_ERR_ARG, "streams must be 1, 2, 3 or 4");
+    c->nstreams = n;
     return CQX_OK;
   }
   if (!std::strcmp(name, "list_cap")) {
@@ -1135,7 +1157,7 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
       };
       // one list in launches of at most 2^25 quartets, so that no grid
       // exceeds 2^31 work-items (the wave kernels take 4 quartets per block)
-      auto launch_eri = [&](const cqx::EriLaunch &E) {
+      auto launch_eri = [&](const cqx::EriLaunch &E, hipStream_t st) {
         constexpr long long kPiece = 1ll << 25;
         const int r = route(E.la, E.lb, E.lc, E.ld);
         if (r == 2 && E.fp32) throw std::runtime_error("FP32 quartet list for a class without an FP32 kernel");
@@ -1144,9 +1166,32 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           P.quartets = E.quartets + b;
           P.nq = std::min(kPiece, E.nq - b);
           P.nq_base = b;
-          HIPCHK(r == 0 ? cqx::launch_eri_fast(P, s) : r == 1 ? cqx::launch_eri_wave(P, s) : cqx::launch_eri_team(P, s));
+          HIPCHK(r == 0 ? cqx::launch_eri_fast(P, st) : r == 1 ? cqx::launch_eri_wave(P, st) : cqx::launch_eri_team(P, st));
         }
       };
+      // side streams for concurrent class-pair launches: created outside any
+      // capture, forked from the context stream before the launches and
+      // joined back into it after them (also inside a recorded build)
+      auto ensure_side = [&](int nst) {
+        for (int i = 0; i < nst - 1; ++i)
+          if (!c->side[i]) {
+            HIPCHK(hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&c->ev_join[i], hipEventDisableTiming));
+          }
+        if (nst > 1 && !c->ev_fork) HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+      };
+      auto fork_side = [&](int nst) {
+        if (nst <= 1) return;
+        HIPCHK(hipEventRecord(c->ev_fork, s));
+        for (int i = 0; i < nst - 1; ++i) HIPCHK(hipStreamWaitEvent(c->side[i], c->ev_fork, 0));
+      };
+      auto join_side = [&](int nst) {
+        for (int i = 0; i < nst - 1; ++i) {
+          HIPCHK(hipEventRecord(c->ev_join[i], c->side[i]));
+          HIPCHK(hipStreamWaitEvent(s, c->ev_join[i], 0));
+        }
+      };
+      auto stream_of = [&](int launch, int nst) { return launch % nst == 0 ? s : c->side[launch % nst - 1]; };
       // bra-range pieces of at most list_cap candidates, grouped into batches
       // of at most list_cap candidates: the quartet lists of one batch are
       // built, counted (one host synchronisation) and evaluated before the
@@ -1245,6 +1290,11 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         // ERI + digestion: one launch per class pair and precision for every
         // molecule of the batch
         HIPCHK(hipEventRecord(c->ev0, s));
+        // autotuning times launches one by one on the context stream
+        const int nst = c->autotune_pending ? 1 : c->nstreams;
+        ensure_side(nst);
+        fork_side(nst);
+        int nlaunched = 0;
         for (size_t kk = 0; kk < nslot; ++kk) {
           if (!counts[kk]) continue;
           const size_t k = kk % np;
@@ -1291,7 +1341,7 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
                 float ms = 0.0f;
                 for (int rep = 0; rep < 2; ++rep) {  // the second run is timed
                   HIPCHK(hipEventRecord(c->ev_t0, s));
-                  launch_eri(T);
+                  launch_eri(T, s);
                   HIPCHK(hipEventRecord(c->ev_t1, s));
                   HIPCHK(hipEventSynchronize(c->ev_t1));
                   HIPCHK(hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1));
@@ -1305,7 +1355,7 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
             }
             apply_tuned(E);
           }
-          launch_eri(E);
+          launch_eri(E, stream_of(nlaunched++, nst));
           const double fl = class_pair_flops(E.la, E.lb, E.lc, E.ld, pure_d, psum[kk], (double)counts[kk],
                                              is_complex ? 2 : 1, nJ, nK);
           c->stats.n_quartets += (long long)counts[kk];
@@ -1317,6 +1367,7 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           }
           c->stats.n_launch_eri++;
         }
+        join_side(nst);
         HIPCHK(hipEventRecord(c->ev1, s));
         HIPCHK(hipEventSynchronize(c->ev1));
         float ms = 0.0f;
@@ -1403,14 +1454,20 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         c->d_glstart.upload(glst, s);
         c->d_gidx.upload(gidx, s);
         c->d_gchk.ensure(ncp);
+        const int nst = c->nstreams;
+        ensure_side(nst);
         HIPCHK(hipStreamSynchronize(s));  // every upload done before recording
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        bool forked = false;
         try {
           front();
           HIPCHK(cqx::launch_zero(reinterpret_cast<double *>(c->d_count.p), 2 * ncp, s));
           HIPCHK(cqx::launch_zero(c->d_psum.p, 2 * ncp, s));
           HIPCHK(cqx::launch_quartet_lists(c->d_glists.p, c->d_glstart.p, (int)ncp, glst[ncp], s));
+          fork_side(nst);
+          forked = true;
+          int nlaunched = 0;
           for (size_t kk = 0; kk < 2 * ncp; ++kk) {
             if (slots[kk].la < 0) continue;
             const size_t k = kk % ncp;
@@ -1422,11 +1479,18 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
             const double n0 = cp_count[kk];
             E.mean_prims = n0 > 0.0 ? cp_psum[kk] / n0 : 1.0;
             apply_tuned(E);
-            launch_eri(E);
+            launch_eri(E, stream_of(nlaunched++, nst));
           }
+          forked = false;
+          join_side(nst);
           back();
           HIPCHK(cqx::launch_gather(c->d_cand.p, c->d_gidx.p, (int)ncp, c->d_gchk.p, s));
         } catch (...) {
+          if (forked)  // an unjoined side stream would leave the capture open
+            for (int i = 0; i < nst - 1; ++i) {
+              (void)hipEventRecord(c->ev_join[i], c->side[i]);
+              (void)hipStreamWaitEvent(s, c->ev_join[i], 0);
+            }
           (void)hipStreamEndCapture(s, &g);
           if (g) (void)hipGraphDestroy(g);
           throw;

@@ -201,8 +201,12 @@ int cqx_grad_two_body_contract(cqx_ctx *ctx, int ngrad, int nmat, const cqx_cont
  * (at most 8 contractions per molecule) and one list batch only, otherwise
  * every build runs eagerly), "graph_margin" (capacity of each class pair's
  * candidates and lists in a recorded graph, as a multiple of the recording
- * build's candidates; default 1.25).  Setting any option drops a recorded
- * graph. */
+ * build's candidates; default 1.25), "streams" (1-4: a build, eager or
+ * recorded, deals its ERI launches round-robin over this many streams,
+ * forked from and joined back into the context's stream, so that launches
+ * of different class pairs overlap; default 4; results differ from one
+ * stream only in the order of the FP64 atomic additions).  Setting any
+ * option drops a recorded graph. */
 int cqx_set_option(cqx_ctx *ctx, const char *name, double value);
 
 /* ---------------------------------------------------------------------

@@ -183,3 +183,33 @@ def test_replays_of_nonhermitian_and_unscreened_builds(case):
         say(f"  build {it}: graph mode {mode} max|d| {err:.2e}")
         assert mode == (1 if it == 0 else 2)
         assert err < TOL
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_concurrent_streams_equal_one_stream(cplx):
+    """Option streams (ERI launches dealt over 1-4 streams, also inside a
+    recorded build): every stream count, eager and replayed, equals the
+    one-stream build up to the order of the atomic additions (1e-11)."""
+    from chronusq_public_amd import cqx
+    from chronusq_public_amd.chem import water_cluster
+    say("case: streams 1-4, eager and recorded, cplx", cplx)
+    b = cqx.BasisSet("cc-pVDZ", water_cluster(4, 8.0, seed=0))
+    rng = np.random.Generator(np.random.PCG64(71))
+    D = _density(rng, b.nBasis, 20, cplx)
+    Xs, types, hers = [D, D, D], [COULOMB, EXCHANGE, EXCHANGE], [True, True, False]
+    tpi = cqx.DirectTPI(b, 1e-12)
+    gto = cqx.GTODirectTPIContraction(tpi)
+    tpi.set_option("streams", 1)
+    ref = _run(gto, Xs, types, hers)
+    for n in (2, 3, 4):
+        tpi.set_option("streams", n)
+        assert _maxdiff(_run(gto, Xs, types, hers), ref) <= TOL
+    tpi.set_option("graph", 1)
+    modes = []
+    for _ in range(3):
+        out = _run(gto, Xs, types, hers)
+        modes.append(tpi.last_stats()["graph"])
+        assert _maxdiff(out, ref) <= TOL
+    assert modes == [1, 2, 2]
+    with pytest.raises(cqx.CErr):
+        tpi.set_option("streams", 5)
