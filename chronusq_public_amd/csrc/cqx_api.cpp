@@ -1481,8 +1481,8 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
             apply_tuned(E);
             launch_eri(E, stream_of(nlaunched++, nst));
           }
-          forked = false;
           join_side(nst);
+          forked = false;
           back();
           HIPCHK(cqx::launch_gather(c->d_cand.p, c->d_gidx.p, (int)ncp, c->d_gchk.p, s));
         } catch (...) {
