@@ -173,6 +173,7 @@ struct cqx_ctx {
   std::vector<GraphSlot> g_slots;      // per ERI slot (class pair, precision); la < 0: not launched
   DevBuf<cqx::ListLaunch> d_glists;
   DevBuf<long long> d_glstart, d_gidx, d_gchk;
+  DevBuf<long long> d_tidx, d_tot;  // candidate totals of an eager build, gathered for one copy
 
   cqx_stats stats{};
 };
@@ -1100,10 +1101,17 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         }
       };
       front();
+      // every class pair's candidate total (the last entry of its offsets),
+      // gathered on the device and read back in one copy
       std::vector<long long> totals(cps.size());
-      for (size_t k = 0; k < cps.size(); ++k)
-        HIPCHK(hipMemcpyAsync(&totals[k], c->d_cand.p + cps[k].coff + cps[k].nbra, sizeof(long long),
-                              hipMemcpyDeviceToHost, s));
+      if (!cps.empty()) {
+        std::vector<long long> tidx(cps.size());
+        for (size_t k = 0; k < cps.size(); ++k) tidx[k] = (long long)(cps[k].coff + cps[k].nbra);
+        c->d_tidx.upload(tidx, s);
+        c->d_tot.ensure(cps.size());
+        HIPCHK(cqx::launch_gather(c->d_cand.p, c->d_tidx.p, (int)cps.size(), c->d_tot.p, s));
+        HIPCHK(hipMemcpyAsync(totals.data(), c->d_tot.p, sizeof(long long) * cps.size(), hipMemcpyDeviceToHost, s));
+      }
       HIPCHK(hipStreamSynchronize(s));
       long long total = 0;
       for (size_t k = 0; k < cps.size(); ++k) {
