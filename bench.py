@@ -40,6 +40,7 @@ METRIC = "Fock builds/sec, mixed vs FP64 at |ΔE|≤1µEh; SCF p50/p99 latency"
 COULOMB_, EXCHANGE_ = 0, 1
 FP64_VECTOR_PEAK = 78.6  # TFLOP/s, MI355X
 FP32_VECTOR_PEAK = 157.3  # TFLOP/s, MI355X
+LAUNCH_STREAMS = 4  # streams the ERI launches of a build are dealt over (--streams)
 
 
 def pauli_density(nb, nocc, seed):
@@ -110,6 +111,7 @@ class Build:
                      for x, a, t in zip(self.dX, self.dA, self.types)]
         # the first (warm-up) build times the lane counts of every class
         self.tpi.set_option("autotune", 1)
+        self.tpi.set_option("streams", LAUNCH_STREAMS)
 
     def step(self):
         for a in self.dA:
@@ -385,6 +387,9 @@ def main():
                     help="timed builds per precision of the mixed-precision comparison (0 skips it)")
     ap.add_argument("--opt", action="append", default=[], metavar="NAME=VALUE",
                     help="engine option of the main workload's context (cqx_set_option), repeatable")
+    ap.add_argument("--streams", type=int, default=4,
+                    help="streams the ERI launches of a build are dealt over, 1-4 (Fock-build "
+                         "contexts of the main and mixed-precision legs)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM "
                          "counter calibration for rocprofv3 --pmc runs)")
@@ -394,6 +399,8 @@ def main():
                          "on one MI355X (multi-GPU sharding is not part of this build)")
 
     from chronusq_public_amd import cqx
+    global LAUNCH_STREAMS
+    LAUNCH_STREAMS = args.streams
     bench = Build(args.workload)
     for o in args.opt:
         name, value = o.split("=", 1)
