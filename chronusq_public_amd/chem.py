@@ -142,3 +142,45 @@ def alkane_chain(ncarbon, seed, jitter=0.05, cc=1.53, ch=1.09):
     mol = Molecule.from_angstrom(atoms)
     mol.xyz = mol.xyz + rng.uniform(-jitter, jitter, size=mol.xyz.shape) / ANG_PER_BOHR
     return mol
+
+
+def cluster_box(n):
+    """Cube side (Angstrom) for (H2O)_n at the density of the 16-water,
+    12 A cluster: 12 (n / 16)^(1/3)."""
+    return 12.0 * (n / 16.0) ** (1.0 / 3.0)
+
+
+def batched_clusters(nmol, seed=0, nmin=2, nmax=24):
+    """The batched-build workload: nmol water clusters (H2O)_n, n ~
+    U{nmin..nmax} from seed (numpy PCG64), cluster i placed from seed
+    2000 + i in a cube of cluster_box(n); the densities D = C C^T,
+    symmetrised, with C of NB x 5n entries N(0, 1/sqrt(NB)) drawn after the
+    sizes from the same generator (NB = 24 n for cc-pVDZ).  Returns
+    (sizes, molecules, a function nb -> density drawing the next one)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = [int(n) for n in rng.integers(nmin, nmax + 1, size=nmol)]
+    mols = [water_cluster(n, cluster_box(n), seed=2000 + i) for i, n in enumerate(sizes)]
+
+    def density(nb, n):
+        C = rng.normal(0, 1 / np.sqrt(nb), size=(nb, 5 * n))
+        D = C @ C.T
+        return 0.5 * (D + D.T)
+
+    return sizes, mols, density
+
+
+def serving_jobs(njobs, seed=0, nmin=4, nmax=48):
+    """The serving workload: njobs water clusters (H2O)_n, n ~ U{nmin..nmax}
+    from seed (numpy PCG64), job i placed from seed 1000 + i in a cube of
+    cluster_box(n).  Returns (sizes, molecules, rng) -- the generator then
+    draws the Poisson arrival gaps."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = [int(n) for n in rng.integers(nmin, nmax + 1, size=njobs)]
+    mols = [water_cluster(n, cluster_box(n), seed=1000 + i) for i, n in enumerate(sizes)]
+    return sizes, mols, rng
+
+
+def poisson_arrivals(rng, njobs, rate):
+    """Arrival times of njobs jobs: the first at 0, then exponential gaps
+    of mean 1 / rate."""
+    return np.concatenate([[0.0], np.cumsum(rng.exponential(1.0 / rate, size=njobs - 1))])

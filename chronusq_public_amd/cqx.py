@@ -43,7 +43,8 @@ class Stats(C.Structure):
                 ("n_prim_quartets_max", C.c_longlong), ("flops_model", C.c_double),
                 ("ms_total", C.c_double), ("ms_eri", C.c_double),
                 ("n_launch_eri", C.c_int), ("n_quartets_f32", C.c_longlong),
-                ("flops_model_f32", C.c_double), ("graph", C.c_int)]
+                ("flops_model_f32", C.c_double), ("graph", C.c_int),
+                ("workspace_bytes", C.c_longlong)]
 
 
 SYMBOLS = ["cqx_basis_create", "cqx_basis_from_g94", "cqx_basis_destroy",
@@ -54,7 +55,9 @@ SYMBOLS = ["cqx_basis_create", "cqx_basis_from_g94", "cqx_basis_destroy",
            "cqx_memcpy", "cqx_device_zero", "cqx_one_body", "cqx_basis_batch", "cqx_scf_default_opts",
            "cqx_scf_create", "cqx_scf_iterate", "cqx_scf_run", "cqx_scf_result_get",
            "cqx_scf_matrix", "cqx_scf_orbital_energies", "cqx_scf_destroy",
-           "cqx_scf_last_error", "cqx_serve_default_opts", "cqx_serve", "cqx_grad_two_body_contract"]
+           "cqx_scf_last_error", "cqx_serve_default_opts", "cqx_serve", "cqx_grad_two_body_contract",
+           "cqx_scratch_size", "cqx_spin_scatter", "cqx_spin_gather", "cqx_compute_energy",
+           "cqx_count_nonfinite"]
 
 OVERLAP, KINETIC, POTENTIAL = 0, 1, 2
 
@@ -90,6 +93,7 @@ def lib():
         L.cqx_one_body.argtypes = [P, I, I, P, P, P]
         L.cqx_basis_batch.argtypes = [I, P, C.POINTER(P)]
         L.cqx_grad_two_body_contract.argtypes = [P, I, I, P, I]
+        L.cqx_scratch_size.argtypes = [P, I, P, I, C.POINTER(C.c_size_t)]
         _lib = L
     return _lib
 
@@ -218,6 +222,20 @@ class DirectTPI:
     def set_option(self, name, value):
         _check(lib().cqx_set_option(self._h, name.encode(), float(value)), self._h)
 
+    def scratch_size(self, types, hers=None, is_complex=False):
+        """GTODirectTPIContraction::directScaffoldNewSCRSize (direct.hpp:2125-2180):
+        device bytes one contraction call with these types (and HER flags)
+        allocates at most (cqx_scratch_size)."""
+        types = list(types)
+        hers = [True] * len(types) if hers is None else list(hers)
+        arr = (_Contraction * len(types))()
+        for i, (t, h) in enumerate(zip(types, hers)):
+            arr[i].X, arr[i].AX, arr[i].her, arr[i].type = None, None, int(h), int(t)
+        b = C.c_size_t()
+        _check(lib().cqx_scratch_size(self._h, len(types), C.cast(arr, C.c_void_p), int(is_complex),
+                                      C.byref(b)), self._h)
+        return b.value
+
     def last_stats(self):
         st = Stats()
         _check(lib().cqx_last_stats(self._h, C.byref(st)), self._h)
@@ -271,8 +289,23 @@ class GTODirectTPIContraction:
             raise CErr("GTODirectTPIContraction expects a DirectTPI")
         self.tpi = tpi
 
-    def twoBodyContract(self, contractions, screen=True):
-        """AX += J[X] / K[X] for every contraction (callers zero AX)."""
+    def twoBodyContract(self, *args, screen=True, pert=None):
+        """AX += J[X] / K[X] for every contraction (callers zero AX).
+
+        The reference's forms (twopints.hpp:219-244) -- (comm, screen,
+        contList, pert), (comm, screen, contList), (comm, contList, pert),
+        (comm, contList) -- and the short (contList[, screen]).  The
+        perturbation is not read, as GTODirectTPIContraction ignores it
+        (gtodirecttpi.hpp:199-206); comm is the single rank of this process."""
+        pos = [i for i, a in enumerate(args) if isinstance(a, (list, tuple))]
+        if not pos:
+            raise CErr("twoBodyContract: no contraction list")
+        i = pos[0]
+        contractions = args[i]
+        if i == 0 and len(args) > 1:
+            screen = bool(args[1])
+        elif i == 2:
+            screen = bool(args[1])
         if not contractions:
             raise CErr("Nothing specified to compute")
         dev = [hasattr(c.X, "data_ptr") for c in contractions]
@@ -359,41 +392,298 @@ def getGDGrad(onePDM, tpi, natom, xHFX=1.0):
     return grad, J, K
 
 
-class SingleSlaterState:
-    """The pieces of SingleSlater that FockBuilder::formGD reads and writes:
-    onePDM as Pauli components [S] (1C) or [S, Z, Y, X] (2C), and the
-    outputs coulombMatrix, exchangeMatrix[c], twoeH[c]."""
+SPIN_ZERO_ABBA, SPIN_BB_EQ_AA, SPIN_ZERO_XY, SPIN_ZERO_Z = 1, 2, 4, 8
 
-    def __init__(self, onePDM, TPI):
+
+def _pauli_lib():
+    L = lib()
+    if not getattr(L, "_pauli_typed", False):
+        P, I = C.c_void_p, C.c_int
+        L.cqx_spin_scatter.argtypes = [P, I, P, I, P, P, P, P, I, I, I]
+        L.cqx_spin_gather.argtypes = [P, I, P, I, P, P, P, P, I, I, I]
+        L.cqx_compute_energy.argtypes = [P, I, I, P, P, P, I, I, C.c_double, P]
+        L.cqx_count_nonfinite.argtypes = [P, P, C.c_size_t, I, C.POINTER(C.c_size_t)]
+        L._pauli_typed = True
+    return L
+
+
+def _tpi_handle(tpi):
+    """The context handle behind a DirectTPI or a contraction object."""
+    t = tpi.tpi if isinstance(tpi, GTODirectTPIContraction) else tpi
+    return t._h
+
+
+def SpinScatter(tpi, A, zeroABBA=False, BBeqAA=False):
+    """SpinScatter (src/cqlinalg/blasutil.cxx:71-129) on the GPU of a
+    DirectTPI: the 2NB x 2NB spinor matrix A -> [S, Z, Y, X], NB x NB each,
+    S = AA + BB, Z = AA - BB, Y = f (AB - BA), X = AB + BA (f = i complex,
+    -1 real)."""
+    A = np.asfortranarray(A)
+    cplx = np.iscomplexobj(A)
+    dt = np.complex128 if cplx else np.float64
+    A = A.astype(dt, order="F", copy=False)
+    nb = A.shape[0] // 2
+    if A.shape != (2 * nb, 2 * nb):
+        raise CErr("SpinScatter takes a square matrix of even dimension")
+    out = [np.zeros((nb, nb), dtype=dt, order="F") for _ in range(4)]
+    flags = (SPIN_ZERO_ABBA if zeroABBA else 0) | (SPIN_BB_EQ_AA if BBeqAA else 0)
+    h = _tpi_handle(tpi)
+    _check(_pauli_lib().cqx_spin_scatter(h, nb, _ptr(A), 2 * nb, *[_ptr(o) for o in out], int(cplx), flags, HOST), h)
+    return out
+
+
+def SpinGather(tpi, S, Z=None, Y=None, X=None):
+    """SpinGather (blasutil.cxx:131-189): [S, Z, Y, X] -> the 2NB x 2NB
+    spinor matrix (Z None: zeroZ; Y and X None: zeroXY)."""
+    comps = [S, Z, Y, X]
+    cplx = any(c is not None and np.iscomplexobj(c) for c in comps)
+    dt = np.complex128 if cplx else np.float64
+    arrs = [None if c is None else np.asfortranarray(c, dtype=dt) for c in comps]
+    nb = arrs[0].shape[0]
+    zero_z, zero_xy = arrs[1] is None, arrs[2] is None or arrs[3] is None
+    flags = (SPIN_ZERO_Z if zero_z else 0) | (SPIN_ZERO_XY if zero_xy else 0)
+    A = np.zeros((2 * nb, 2 * nb), dtype=dt, order="F")
+    ptrs = [None if a is None else _ptr(a) for a in arrs]
+    h = _tpi_handle(tpi)
+    _check(_pauli_lib().cqx_spin_gather(h, nb, _ptr(A), 2 * nb, *ptrs, int(cplx), flags, HOST), h)
+    return A
+
+
+class PauliSpinorMatrices:
+    """cqmatrix::PauliSpinorMatrices (include/matrix/paulispinormatrices.hpp:30-392):
+    the S component and, where present, Z (hasZ) and Y, X (hasXY); each an
+    NB x NB column-major array."""
+
+    def __init__(self, nb, hasXY=False, hasZ=False, dtype=np.float64):
+        self.hasZ, self.hasXY = bool(hasZ or hasXY), bool(hasXY)
+        self.dtype = np.dtype(dtype)
+        self.comps = [np.zeros((nb, nb), dtype=dtype, order="F") for _ in range(1 + self.hasZ + 2 * self.hasXY)]
+
+    @classmethod
+    def from_components(cls, comps):
+        comps = [np.asfortranarray(c) for c in comps]
+        dt = np.result_type(*[c.dtype for c in comps])
+        self = cls(comps[0].shape[0], hasXY=len(comps) == 4, hasZ=len(comps) >= 2, dtype=dt)
+        for k, c in enumerate(comps):
+            self.comps[k][...] = c
+        return self
+
+    @classmethod
+    def spinScatter(cls, tpi, A, zeroABBA=False, BBeqAA=False):
+        return cls.from_components(SpinScatter(tpi, A, zeroABBA, BBeqAA))
+
+    def spinGather(self, tpi):
+        return SpinGather(tpi, *(self.comps + [None] * (4 - len(self.comps))))
+
+    def S(self):
+        return self.comps[0]
+
+    def Z(self):
+        if not self.hasZ:
+            raise CErr("PauliSpinorMatrices has no Z component")
+        return self.comps[1]
+
+    def Y(self):
+        if not self.hasXY:
+            raise CErr("PauliSpinorMatrices has no Y component")
+        return self.comps[2]
+
+    def X(self):
+        if not self.hasXY:
+            raise CErr("PauliSpinorMatrices has no X component")
+        return self.comps[3]
+
+    def clear(self):
+        for c in self.comps:
+            c[...] = 0
+
+    def hasNaN(self, tpi):
+        """PauliSpinorMatrices::hasNaN (paulispinormatrices.hpp:383-388), on the GPU."""
+        h = _tpi_handle(tpi)
+        n = C.c_size_t()
+        for c in self.comps:
+            flat = np.ascontiguousarray(c).view(np.float64)
+            _check(_pauli_lib().cqx_count_nonfinite(h, _ptr(flat), flat.size, HOST, C.byref(n)), h)
+            if n.value:
+                return True
+        return False
+
+
+def computeEnergy(tpi, onePDM, coreH, twoeH, nucRepEnergy=0.0):
+    """SingleSlater::computeEnergy (include/singleslater/quantum.hpp:147-212)
+    on the GPU: (OBEnergy, MBEnergy, totalEnergy) = (1/2 sum_c Re<D_c,H_c>,
+    1/4 sum_c Re<D_c,G_c>, their sum + V_NN) over the Pauli components."""
+    D = [np.asfortranarray(d) for d in onePDM]
+    cplx = any(np.iscomplexobj(x) for x in list(D) + list(coreH or []) + list(twoeH or []))
+    dt = np.complex128 if cplx else np.float64
+    D = [d.astype(dt, order="F", copy=False) for d in D]
+    nc, nb = len(D), D[0].shape[0]
+
+    def arr(mats):
+        if mats is None:
+            return None, []
+        keep = [None if m is None else np.asfortranarray(m, dtype=dt) for m in list(mats)[:nc]]
+        keep += [None] * (nc - len(keep))
+        return (C.c_void_p * nc)(*[None if k is None else k.ctypes.data for k in keep]), keep
+
+    pd, kd = arr(D)
+    ph, kh = arr(coreH)
+    pg, kg = arr(twoeH)
+    out = np.zeros(3)
+    h = _tpi_handle(tpi)
+    _check(_pauli_lib().cqx_compute_energy(h, nb, nc, C.cast(pd, C.c_void_p),
+                                          None if ph is None else C.cast(ph, C.c_void_p),
+                                          None if pg is None else C.cast(pg, C.c_void_p),
+                                          int(cplx), HOST, float(nucRepEnergy), _ptr(out)), h)
+    return float(out[0]), float(out[1]), float(out[2])
+
+
+class EMPerturbation:
+    """EMPerturbation (include/fields.hpp:124-195), dipole fields only:
+    fields is a list of (type, amplitude[3]) with type "Electric" or
+    "Magnetic"."""
+
+    def __init__(self, fields=None):
+        self.fields = list(fields or [])
+
+    def addField(self, typ, amp):
+        self.fields.append((typ, np.asarray(amp, dtype=float)))
+
+    def has(self, typ):
+        return any(t == typ for t, _ in self.fields)
+
+    def getDipoleAmp(self, typ):
+        amp = np.zeros(3)
+        for t, a in self.fields:
+            if t == typ:
+                amp += a
+        return amp
+
+
+class SingleSlaterState:
+    """The pieces of SingleSlater that FockBuilder reads and writes: onePDM
+    (and deltaOnePDM for incremental builds) as Pauli components [S] (1C) or
+    [S, Z, Y, X] (2C); the outputs coulombMatrix, exchangeMatrix[c],
+    twoeH[c], fockMatrix[c]; coreH[c] for formFock; nC; the contraction
+    engine TPI; GDDur (seconds of the last formGD inside formFock)."""
+
+    def __init__(self, onePDM, TPI, coreH=None, deltaOnePDM=None, nC=None, lenElectric=None):
         self.onePDM = [np.asfortranarray(d) for d in onePDM]
         self.TPI = TPI
         nb = self.onePDM[0].shape[0]
-        dt = np.result_type(*[d.dtype for d in self.onePDM])
+        mats = list(self.onePDM) + list(coreH or []) + list(deltaOnePDM or [])
+        dt = np.result_type(*[m.dtype for m in mats])
+        self.deltaOnePDM = None if deltaOnePDM is None else [np.asfortranarray(d, dtype=dt) for d in deltaOnePDM]
+        self.coreH = None if coreH is None else [np.asfortranarray(h, dtype=dt) for h in coreH]
+        self.nC = nC if nC is not None else (2 if len(self.onePDM) > 1 else 1)
+        self.lenElectric = lenElectric
         self.coulombMatrix = np.zeros((nb, nb), dtype=dt, order="F")
         self.exchangeMatrix = [np.zeros((nb, nb), dtype=dt, order="F") for _ in self.onePDM]
         self.twoeH = [np.zeros((nb, nb), dtype=dt, order="F") for _ in self.onePDM]
+        self.fockMatrix = [np.zeros((nb, nb), dtype=dt, order="F") for _ in self.onePDM]
+        self.GDDur = 0.0
 
 
 class FockBuilder:
-    """FockBuilder<MatsT,IntsT>::formGD (include/fockbuilder/impl.hpp:75-244)
-    for electronic, non-incremental builds: J on S, K on S (+ Z, Y, X for
-    two-component densities), G = 2J - xHFX K componentwise."""
+    """FockBuilder<MatsT,IntsT> (include/fockbuilder/impl.hpp:75-319) for
+    electronic densities: formGD, formRawGDInBatches, formRawGDSCRSizePerBatch
+    and formFock, with the reference's list building (J on S; K on S, Z and,
+    for two-component densities, Y and X), zeroing unless incremental, and
+    G = 2J - xHFX K componentwise."""
 
     @staticmethod
-    def formGD(ss, xHFX=1.0, HerDen=True):
+    def formGD(ss, pert=None, increment=False, xHFX=1.0, HerDen=True):
+        """impl.hpp:75-102: G[D] (or, with increment, G[D] += G[deltaOnePDM],
+        K += K[deltaOnePDM] and coulombMatrix = J[deltaOnePDM])."""
+        src = ss.deltaOnePDM if increment else ss.onePDM
+        if src is None:
+            raise CErr("formGD: increment needs deltaOnePDM")
         dt = ss.coulombMatrix.dtype
-        D = [np.asfortranarray(d, dtype=dt) for d in ss.onePDM]
-        J = np.zeros_like(ss.coulombMatrix, order="F")
-        K = [np.zeros_like(ss.coulombMatrix, order="F") for _ in D]
-        cont = [TwoBodyContraction(D[0], J, HerDen, COULOMB)]
-        doK = abs(xHFX) > 1e-12
-        if doK:
-            cont += [TwoBodyContraction(D[c], K[c], HerDen, EXCHANGE) for c in range(len(D))]
+        J = [np.zeros_like(ss.coulombMatrix, order="F")]
+        FockBuilder.formRawGDInBatches(ss, pert, increment, xHFX, HerDen, [src], J,
+                                       [ss.exchangeMatrix], [ss.twoeH])
+        ss.coulombMatrix[...] = J[0].astype(dt, copy=False)
+        return ss
+
+    @staticmethod
+    def formRawGDInBatches(ss, pert, increment, xHFX, HerDen, onePDMs, coulombMatrices,
+                           exchangeMatrices, twoeHs):
+        """impl.hpp:109-244: every batch b takes onePDMs[b] (Pauli
+        components) and writes coulombMatrices[b] (the S component, one
+        NB x NB array), exchangeMatrices[b][c] and twoeHs[b][c]; all batches
+        go to one twoBodyContract call."""
+        nBatch = len(onePDMs)
+        computeCoulomb = coulombMatrices is not None and len(coulombMatrices) > 0
+        computeExchange = abs(xHFX) > 1e-12 and exchangeMatrices is not None and len(exchangeMatrices) > 0
+        computeTwoeHs = twoeHs is not None and len(twoeHs) > 0
+        if not (computeCoulomb or computeExchange or computeTwoeHs):
+            raise CErr("Nothing specified to compute in FockBuilder::formRawGDInBatches")
+        if getattr(ss, "nC", 1) == 4:
+            raise CErr("4C formGD is implemented in class FourCompFock.")
+        if not increment:
+            for i in range(nBatch):
+                if computeCoulomb:
+                    coulombMatrices[i][...] = 0
+                if computeExchange:
+                    for m in exchangeMatrices[i]:
+                        m[...] = 0
+                if computeTwoeHs:
+                    for m in twoeHs[i]:
+                        m[...] = 0
+        dt = np.result_type(*[np.asarray(d).dtype for b in onePDMs for d in b])
+        if computeCoulomb:
+            dt = np.result_type(dt, coulombMatrices[0].dtype)
+        D = [[np.asfortranarray(d, dtype=dt) for d in b] for b in onePDMs]
+        cont = []
+        if computeCoulomb or computeTwoeHs:
+            cont_j = coulombMatrices if computeCoulomb else [t[0] for t in twoeHs]
+            for i in range(nBatch):
+                cont.append(TwoBodyContraction(D[i][0], cont_j[i], HerDen, COULOMB))
+        if computeExchange:
+            for i in range(nBatch):
+                for c in range(len(exchangeMatrices[i])):
+                    cont.append(TwoBodyContraction(D[i][c], exchangeMatrices[i][c], HerDen, EXCHANGE))
         ss.TPI.twoBodyContract(cont)
-        ss.coulombMatrix[...] = J
-        for c in range(len(D)):
-            ss.exchangeMatrix[c][...] = K[c]
-            ss.twoeH[c][...] = (2.0 * J if c == 0 else 0.0) - (xHFX * K[c] if doK else 0.0)
+        if computeTwoeHs:
+            for i in range(nBatch):
+                if computeCoulomb:
+                    twoeHs[i][0] += 2.0 * coulombMatrices[i]
+                else:
+                    twoeHs[i][0] *= 2.0
+                if computeExchange:
+                    for c in range(len(twoeHs[i])):
+                        twoeHs[i][c] -= xHFX * exchangeMatrices[i][c]
+        return ss
+
+    @staticmethod
+    def formRawGDSCRSizePerBatch(ss, computeExchange=True, HerDen=True):
+        """impl.hpp:252-269: device workspace (in MatsT elements) of one
+        batch's contractions (J, plus K on every Pauli component)."""
+        cplx = np.iscomplexobj(ss.coulombMatrix)
+        nK = len(ss.onePDM) if computeExchange else 0
+        types = [COULOMB] + [EXCHANGE] * nK
+        b = ss.TPI.tpi.scratch_size(types, [HerDen] * len(types), cplx)
+        return -(-b // (16 if cplx else 8))
+
+    @staticmethod
+    def formFock(ss, pert=None, increment=False, xHFX=1.0):
+        """impl.hpp:280-319: G[D] by formGD (timed into GDDur), then
+        F = H + G componentwise, minus 2 E.mu_i for an electric dipole field
+        (ss.lenElectric: the three length-gauge dipole matrices)."""
+        import time
+        if ss.coreH is None:
+            raise CErr("formFock needs the core Hamiltonian")
+        t0 = time.perf_counter()
+        FockBuilder.formGD(ss, pert, increment, xHFX)
+        ss.GDDur = time.perf_counter() - t0
+        for c in range(len(ss.fockMatrix)):
+            ss.fockMatrix[c][...] = ss.coreH[c] + ss.twoeH[c]
+        if pert is not None and pert.has("Electric"):
+            if ss.lenElectric is None:
+                raise CErr("formFock: an electric field needs the length-gauge dipole integrals")
+            amp = pert.getDipoleAmp("Electric")
+            for i in range(3):
+                ss.fockMatrix[0][...] -= 2.0 * amp[i] * ss.lenElectric[i]
         return ss
 
 
@@ -502,7 +792,8 @@ class RHF:
 
 class _Job(C.Structure):
     _fields_ = [("g94", C.c_char_p), ("natom", C.c_int), ("Z", C.c_void_p), ("xyz", C.c_void_p),
-                ("nocc", C.c_int), ("arrival_s", C.c_double)]
+                ("nocc", C.c_int), ("arrival_s", C.c_double), ("density_out", C.c_void_p),
+                ("fock_out", C.c_void_p)]
 
 
 class ServeOpts(C.Structure):
@@ -518,10 +809,11 @@ class JobResult(C.Structure):
 
 
 def serve(jobs, max_inflight=8, hbm_budget_bytes=200e9, fp32_cut=0.0, guess="SAD",
-          list_cap=1 << 25, device=0, rmsd_tol=1e-8, verbose=0, graph=0):
+          list_cap=1 << 25, device=0, rmsd_tol=1e-8, verbose=0, graph=0, matrices=False):
     """Run SCF jobs [(basis name or G94 text, Molecule, arrival seconds)]
     through the serving loop; returns one dict per job (latency =
-    t_done - t_arrival)."""
+    t_done - t_arrival), with its final density "D" and Fock matrix "F"
+    (Pauli-scalar convention, NB x NB) when matrices is set."""
     L = _scf_lib()
     if not getattr(L, "_serve_typed", False):
         L.cqx_serve_default_opts.argtypes = [C.POINTER(ServeOpts)]
@@ -545,9 +837,18 @@ def serve(jobs, max_inflight=8, hbm_budget_bytes=200e9, fp32_cut=0.0, guess="SAD
             raise CErr("RHF serving jobs need an even electron count")
         arr[i].g94, arr[i].natom, arr[i].Z, arr[i].xyz = text, len(Z), Z.ctypes.data, xyz.ctypes.data
         arr[i].nocc, arr[i].arrival_s = ne // 2, float(t)
+    mats = []
+    if matrices:
+        for i, (bas, mol, t) in enumerate(jobs):
+            nb = BasisSet(bas, mol).nBasis
+            D, F = np.zeros((nb, nb), order="F"), np.zeros((nb, nb), order="F")
+            mats.append((D, F))
+            arr[i].density_out, arr[i].fock_out = D.ctypes.data, F.ctypes.data
     res = (JobResult * len(jobs))()
     rc = L.cqx_serve(C.cast(arr, C.c_void_p), len(jobs), C.byref(o), C.cast(res, C.c_void_p))
     out = [{k: getattr(r, k) for k, _ in JobResult._fields_} for r in res]
+    for r, m in zip(out, mats):
+        r["D"], r["F"] = m
     if rc:
         bad = [i for i, r in enumerate(out) if r["status"]]
         raise CErr(f"FATAL (cqx status {rc}): serving jobs {bad} failed")

@@ -21,6 +21,7 @@
 #include "kernels.hpp"
 #include "onebody.hpp"
 #include "ctx_access.hpp"
+#include "limits.hpp"
 
 namespace {
 
@@ -53,6 +54,13 @@ inline void class_ls(int c, int &la, int &lb) {
 struct HipError : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
+
+// Sizes the device kernels cannot take (index packing, launch limits, a
+// scratch region too small): reported as CQX_ERR_UNSUPPORTED.
+struct Unsupported : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
 
 #define HIPCHK(x)                                                                   \
   do {                                                                              \
@@ -143,6 +151,23 @@ struct cqx_ctx {
   DevBuf<double> d_psum;
   DevBuf<double> d_store;
   DevBuf<long long> d_store_off;
+  DevBuf<long long> d_tidx, d_tot;  // candidate totals of an eager build, gathered for one copy
+  // The workspace of the contraction group in flight: views of the buffers
+  // above (allocated on demand) or of a fixed scratch region
+  struct Ws {
+    double *X, *AX, *Bm, *Bg, *molmax, *planes, *acc;
+    double *pn[kNumClass];
+    int *her, *ket_lo;
+    long long *cand, *tidx, *tot, *lstart;
+    void *scan;
+    size_t scan_bytes;
+    cqx::MolPlanes *molplanes;
+    Quartet *quart;
+    unsigned long long *count;
+    double *psum;
+    cqx::ListLaunch *lists;
+  } w{};
+  size_t ws_scratch_peak = 0;  // bytes of the scratch region the last call used
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
   // Concurrent class-pair launches (option "streams"): a build, eager or
   // recorded, deals its ERI launches round-robin over the context stream and
@@ -173,7 +198,6 @@ struct cqx_ctx {
   std::vector<GraphSlot> g_slots;      // per ERI slot (class pair, precision); la < 0: not launched
   DevBuf<cqx::ListLaunch> d_glists;
   DevBuf<long long> d_glstart, d_gidx, d_gchk;
-  DevBuf<long long> d_tidx, d_tot;  // candidate totals of an eager build, gathered for one copy
 
   cqx_stats stats{};
 };
@@ -421,6 +445,11 @@ void build_pairs(cqx_ctx *c) {
       }
     c->cls[class_of(B.shells[a].l, B.shells[b].l)].push_back(make_pair_rec(B, a, b, s1, s2, off, n));
   }
+  // a quartet carries its ket's index in 24 bits (Quartet::ket_mask)
+  for (int k = 0; k < kNumClass; ++k)
+    if (!cqx::ket_index_fits(c->cls[k].size()))
+      throw Unsupported("a shell-pair class holds " + std::to_string(c->cls[k].size()) +
+                        " pairs, beyond the 2^24 a quartet can index");
   // Schwarz: (ab|ab) for every significant pair, with all primitives
   // (the reference evaluates these at precision 0, aointegrals_utilities.cxx:70)
   if (c->nmol == 1) c->schwarz.assign((size_t)c->ns * c->ns, 0.0);
@@ -705,6 +734,8 @@ int cqx_ctx_create(const cqx_basis *b, double thresh, int device, cqx_ctx **out)
     HIPCHK(hipStreamSynchronize(c->stream));
     setup_device_maps(c.get());
     build_pairs(c.get());
+  } catch (const Unsupported &e) {
+    return fail(nullptr, CQX_ERR_UNSUPPORTED, e.what());
   } catch (const std::exception &e) {
     return fail(nullptr, CQX_ERR_HIP, e.what());
   }
@@ -854,8 +885,155 @@ int cqx_eri_quartets(cqx_ctx *c, int nq, const int *s1, const int *s2, const int
   return CQX_OK;
 }
 
-int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int is_complex,
-                          int mem) {
+}  // extern "C"
+
+namespace {
+
+// A bump allocator over a fixed device region, 256-byte aligned pieces.
+struct Arena {
+  char *base = nullptr;
+  size_t cap = 0, used = 0;
+  template <class T>
+  T *take(size_t n, const char *what) {
+    const size_t off = (used + 255) & ~(size_t)255;
+    const size_t b = sizeof(T) * std::max<size_t>(n, 1);
+    if (off > cap || b > cap - off)
+      throw Unsupported(std::string("cqx_two_body_contract_noalloc: the scratch region (") + std::to_string(cap) +
+                        " bytes) cannot hold " + what);
+    used = off + b;
+    return reinterpret_cast<T *>(base + off);
+  }
+  size_t left() const {
+    const size_t off = (used + 255) & ~(size_t)255;
+    return off < cap ? cap - off : 0;
+  }
+};
+
+// Contractions processed together (sharing one ERI evaluation): at most
+// kMaxMat per molecule and kMaxPlanes real planes.
+int group_size(const cqx_contraction *L, int left, int is_complex, int *nplane_out) {
+  int nmat = 0, nplane = 0;
+  while (nmat < left && nmat < cqx::kMaxMat) {
+    const auto &cc = L[nmat];
+    const int np = (!is_complex || (cc.her && cc.type == CQX_COULOMB)) ? 1 : 2;
+    if (nplane + np > cqx::kMaxPlanes) break;
+    nplane += np;
+    ++nmat;
+  }
+  if (nplane_out) *nplane_out = nplane;
+  return nmat;
+}
+
+// The density-independent workspace of one contraction group (elements of
+// each buffer), computed the same way for a call and for the scratch query.
+struct GroupPlan {
+  int nmat = 0;
+  size_t x = 0;       // doubles of the staged densities (and of the staged outputs)
+  size_t bm = 0, bg = 0;
+  size_t pn[kNumClass] = {};
+  size_t planes = 0;  // doubles of the Cartesian planes (and of the accumulators)
+  size_t cand = 0;    // candidate offsets (long long) and first kets (int)
+  size_t ncp = 0;     // class pairs with pairs on both sides
+  size_t scan_bytes = 0;
+  long long max_ket = 0;   // largest ket class of a class pair
+  long long cand_all = 0;  // candidates of every class pair without the Schwarz pre-cut
+  int kcnt[4] = {0, 0, 0, 0};
+  int kps[4] = {1, 1, 1, 1};
+};
+
+GroupPlan plan_group(const cqx_ctx *c, const cqx_contraction *L, int nmat, int is_complex, hipStream_t s) {
+  GroupPlan g;
+  g.nmat = nmat;
+  const cqx::Basis &BS = c->basis;
+  const size_t ew = is_complex ? 2 : 1;
+  size_t nb2 = 0, nbc2 = 0;
+  for (int m = 0; m < c->nmol; ++m) {
+    const size_t nb = BS.mol_bf0[m + 1] - BS.mol_bf0[m], nbc = BS.mol_bfc0[m + 1] - BS.mol_bfc0[m];
+    nb2 += nb * nb;
+    nbc2 += nbc * nbc;
+  }
+  g.x = (size_t)nmat * nb2 * ew;
+  g.bm = (size_t)nmat * c->mol_boff[c->nmol];
+  g.bg = c->mol_boff[c->nmol];
+  for (int k = 0; k < kNumClass; ++k) g.pn[k] = c->cls[k].size() * (size_t)(nmat + 1);
+  for (int j = 0; j < nmat; ++j) {
+    const bool two = is_complex && !(L[j].her && L[j].type == CQX_COULOMB);
+    const int kind = L[j].type == CQX_COULOMB ? (L[j].her ? cqx::kJHer : cqx::kJNonHer)
+                                              : (L[j].her ? cqx::kKHer : cqx::kKNonHer);
+    g.kcnt[kind] += two ? 2 : 1;
+  }
+  int kpsum = 0;
+  for (int k = 0; k < 4; ++k) {
+    while (g.kps[k] < g.kcnt[k]) g.kps[k] *= 2;
+    if (g.kcnt[k]) kpsum += g.kps[k];
+  }
+  g.planes = (size_t)kpsum * nbc2;
+  for (int cb = 0; cb < kNumClass; ++cb)
+    for (int ck = 0; ck <= cb; ++ck) {
+      if (c->cls[cb].empty() || c->cls[ck].empty()) continue;
+      const long long nbra = (long long)c->cls[cb].size(), nket = (long long)c->cls[ck].size();
+      g.cand += (size_t)nbra + 1;
+      ++g.ncp;
+      g.max_ket = std::max(g.max_ket, nket);
+      g.cand_all += nbra * nket;
+      cqx::CandLaunch CL{};
+      CL.nbra = (int)nbra;
+      size_t b = 0;
+      HIPCHK(cqx::launch_candidates(CL, nullptr, b, s));  // size query only
+      g.scan_bytes = std::max(g.scan_bytes, b);
+    }
+  return g;
+}
+
+// Quartet-list entries of one batch: list_cap, or more when one bra pair
+// alone has more candidates (a piece holds at least one bra pair), and never
+// more than every candidate.
+long long list_entries_bound(const GroupPlan &g, long long list_cap) {
+  return std::max<long long>(1, std::min(std::max(list_cap, g.max_ket), g.cand_all));
+}
+
+// Device bytes of the per-call workspace a context holds (every buffer a
+// contraction may grow).
+size_t workspace_bytes(const cqx_ctx *c) {
+  size_t b = 0;
+  auto add = [&](const auto &buf) { b += buf.n * sizeof(*buf.p); };
+  add(c->d_X); add(c->d_AX); add(c->d_Bm); add(c->d_Bg); add(c->d_planes); add(c->d_acc);
+  add(c->d_her); add(c->d_cand); add(c->d_ket_lo); add(c->d_scan_tmp); add(c->d_lists);
+  add(c->d_lstart); add(c->d_molmax); add(c->d_molplanes); add(c->d_quart); add(c->d_count);
+  add(c->d_psum); add(c->d_tidx); add(c->d_tot); add(c->d_acc_tune); add(c->d_molplanes_tune);
+  add(c->d_glists); add(c->d_glstart); add(c->d_gidx); add(c->d_gchk);
+  for (int k = 0; k < kNumClass; ++k) add(c->d_pn[k]);
+  return b;
+}
+
+// Upload a host array into a workspace view (the context's buffer, grown
+// on demand, or a piece of the scratch region).
+template <class T>
+T *ws_upload(DevBuf<T> &buf, Arena *scr, const std::vector<T> &v, hipStream_t s, const char *what) {
+  T *p;
+  if (scr) {
+    p = scr->take<T>(v.size(), what);
+  } else {
+    buf.ensure(std::max<size_t>(v.size(), 1));
+    p = buf.p;
+  }
+  if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s));
+  return p;
+}
+template <class T>
+T *ws_take(DevBuf<T> &buf, Arena *scr, size_t n, const char *what) {
+  if (scr) return scr->take<T>(n, what);
+  buf.ensure(std::max<size_t>(n, 1));
+  return buf.p;
+}
+
+// The direct J/K build of a contraction list (directScaffoldNew,
+// direct.hpp:1412-2123).  scr == nullptr: the workspace lives in the
+// context's own buffers, grown on demand (and a build may be recorded into
+// a graph and replayed); otherwise every per-call buffer is carved from
+// the scratch region, X / AX are device pointers, AX is accumulated in
+// place, and nothing is allocated, recorded or created.
+int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int is_complex, int mem, Arena *scr) {
   if (!c || ncon < 0 || (ncon && !list)) return fail(c, CQX_ERR_ARG, "cqx_two_body_contract: bad arguments");
   if (ncon == 0) return fail(c, CQX_ERR_ARG, "Nothing specified to compute");
   for (int i = 0; i < ncon; ++i)
@@ -869,21 +1047,23 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
     for (int i = 0; i < nper; ++i)
       if (list[m * nper + i].type != list[i].type || (list[m * nper + i].her != 0) != (list[i].her != 0))
         return fail(c, CQX_ERR_ARG, "batched molecules must share the contraction types and HER flags");
+  if (scr && mem != CQX_DEVICE) return fail(c, CQX_ERR_ARG, "the allocation-free contraction takes device pointers");
   const auto t_start = std::chrono::steady_clock::now();
   c->stats = cqx_stats{};
   const std::string key = contraction_key(screen, is_complex, ncon, list);
   try {
     HIPCHK(hipSetDevice(c->device));
-    if (c->use_graph && c->graph_exec && c->graph_key == key) {
+    if (!scr && c->use_graph && c->graph_exec && c->graph_key == key) {
       if (replay_graph(c, ncon, list, is_complex, mem)) {
         c->stats.ms_total =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+        c->stats.workspace_bytes = (long long)workspace_bytes(c);
         return CQX_OK;
       }
       c->stats = cqx_stats{};  // capacities exceeded: build eagerly and record again
     }
     // an eager build may reallocate the workspace a recorded graph refers to
-    drop_graph(c);
+    if (!scr) drop_graph(c);
     hipStream_t s = c->stream;
     const cqx::Basis &BS = c->basis;
     const size_t ew = is_complex ? 2 : 1;  // doubles per element
@@ -901,42 +1081,41 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
     bool pure_d = false;
     for (auto &sh : BS.shells)
       if (sh.l > 1 && sh.pure) pure_d = true;
+    cqx_ctx::Ws &w = c->w;
+    size_t scratch_peak = 0;
     // groups of at most kMaxMat contractions / kMaxPlanes planes (per molecule)
     int start = 0;
     while (start < nper) {
-      int nmat = 0, nplane = 0;
-      while (start + nmat < nper && nmat < cqx::kMaxMat) {
-        const auto &cc = list[start + nmat];
-        int np = (!is_complex || (cc.her && cc.type == CQX_COULOMB)) ? 1 : 2;
-        if (nplane + np > cqx::kMaxPlanes) break;
-        nplane += np;
-        ++nmat;
-      }
+      int nplane = 0;
+      const int nmat = group_size(list + start, nper - start, is_complex, &nplane);
       const cqx_contraction *L = list + start;  // molecule 0's contractions of the group
-      // densities and outputs on the device: molecule m's nmat matrices at xoff[m]
+      const GroupPlan gp = plan_group(c, L, nmat, is_complex, s);
+      if (scr) scr->used = 0;  // each group reuses the whole region
+      // densities (and, when allocating, the outputs) on the device:
+      // molecule m's nmat matrices at xoff[m]
       std::vector<size_t> xoff(nmol + 1, 0);
       for (int m = 0; m < nmol; ++m) xoff[m + 1] = xoff[m] + (size_t)nmat * mnb2[m] * ew;
-      c->d_X.ensure(std::max<size_t>(xoff[nmol], 1));
-      c->d_AX.ensure(std::max<size_t>(xoff[nmol], 1));
+      w.X = ws_take(c->d_X, scr, gp.x, "the staged densities");
+      w.AX = scr ? nullptr : ws_take(c->d_AX, scr, gp.x, "the staged outputs");
       const hipMemcpyKind hk = mem == CQX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
       for (int m = 0; m < nmol; ++m)
         for (int j = 0; j < nmat; ++j) {
           const cqx_contraction &cc = list[m * nper + start + j];
           const size_t o = xoff[m] + (size_t)j * mnb2[m] * ew, n = sizeof(double) * mnb2[m] * ew;
-          HIPCHK(hipMemcpyAsync(c->d_X.p + o, cc.X, n, hk, s));
-          HIPCHK(hipMemcpyAsync(c->d_AX.p + o, cc.AX, n, hk, s));
+          HIPCHK(hipMemcpyAsync(w.X + o, cc.X, n, hk, s));
+          if (!scr) HIPCHK(hipMemcpyAsync(w.AX + o, cc.AX, n, hk, s));
         }
       // --- plan: every upload and allocation of the build comes first, so
       // that the launches below can also be recorded into a graph ---
       std::vector<int> her(nmat);
       for (int j = 0; j < nmat; ++j) her[j] = L[j].her ? 1 : 0;
-      c->d_her.upload(her, s);
+      w.her = ws_upload(c->d_her, scr, her, s, "the HER flags");
       const long long nbl = c->mol_boff[nmol];
-      c->d_Bm.ensure((size_t)nmat * nbl);
-      c->d_Bg.ensure((size_t)nbl);
-      c->d_molmax.ensure(nmol);
+      w.Bm = ws_take(c->d_Bm, scr, gp.bm, "the shell-block norms");
+      w.Bg = ws_take(c->d_Bg, scr, gp.bg, "the shell-block norms");
+      w.molmax = ws_take(c->d_molmax, scr, (size_t)nmol, "the molecule norms");
       for (int k = 0; k < kNumClass; ++k)
-        if (!c->cls[k].empty()) c->d_pn[k].ensure(c->cls[k].size() * (size_t)(nmat + 1));
+        w.pn[k] = c->cls[k].empty() ? nullptr : ws_take(c->d_pn[k], scr, gp.pn[k], "the pair norms");
       cqx::PlaneSet ps{};
       // planes (one per real part), ordered by kind; slot = position, so the
       // planes that one digestion term visits sit in consecutive slots
@@ -952,8 +1131,8 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
       // the kind with the most planes first, so that its slots start on an
       // interleave-group boundary and the lanes digesting one element
       // touch as few 64-byte segments as possible
-      int kcnt[4] = {0, 0, 0, 0}, krank[4];
-      for (const auto &d : pd) ++kcnt[d.kind];
+      int krank[4];
+      const int *kcnt = gp.kcnt;
       for (int k = 0; k < 4; ++k) {
         krank[k] = 0;
         for (int j = 0; j < 4; ++j)
@@ -966,26 +1145,25 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
       // rounded up to a power of two)
       int kpsum = 0;
       for (int k = 0; k < 4; ++k) {
-        ps.kps[k] = 1;
-        while (ps.kps[k] < kcnt[k]) ps.kps[k] *= 2;
+        ps.kps[k] = gp.kps[k];
         if (kcnt[k]) kpsum += ps.kps[k];
       }
       std::vector<size_t> poff(nmol + 1, 0);
       for (int m = 0; m < nmol; ++m) poff[m + 1] = poff[m] + (size_t)kpsum * mnbc2[m];
-      c->d_planes.ensure(std::max<size_t>(poff[nmol], 1));
-      c->d_acc.ensure(std::max<size_t>(poff[nmol], 1));
+      w.planes = ws_take(c->d_planes, scr, gp.planes, "the Cartesian density planes");
+      w.acc = ws_take(c->d_acc, scr, gp.planes, "the Cartesian accumulators");
       std::vector<cqx::MolPlanes> mp(nmol);
       for (int m = 0; m < nmol; ++m) {
         size_t o = poff[m];
         for (int k = 0; k < 4; ++k) {
-          mp[m].kx[k] = c->d_planes.p + o;
-          mp[m].ka[k] = c->d_acc.p + o;
+          mp[m].kx[k] = w.planes + o;
+          mp[m].ka[k] = w.acc + o;
           if (kcnt[k]) o += (size_t)ps.kps[k] * mnbc2[m];
         }
         mp[m].nbc = mnbc[m];
       }
-      c->d_molplanes.upload(mp, s);
-      ps.mols = c->d_molplanes.p;
+      w.molplanes = ws_upload(c->d_molplanes, scr, mp, s, "the molecule plane table");
+      ps.mols = w.molplanes;
       ps.m0 = mp[0];
       ps.nmol = nmol;
       std::vector<int> slot_re(nmat, -1), slot_im(nmat, -1), pidx(nplane);
@@ -1033,10 +1211,11 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           cand_total += (size_t)cp.nbra + 1;
           cps.push_back(std::move(cp));
         }
-      c->d_cand.ensure(std::max<size_t>(cand_total, 1));
-      c->d_ket_lo.ensure(std::max<size_t>(cand_total, 1));
+      w.cand = ws_take(c->d_cand, scr, cand_total, "the candidate offsets");
+      w.ket_lo = ws_take(c->d_ket_lo, scr, cand_total, "the candidate offsets");
+      w.scan = ws_take(c->d_scan_tmp, scr, gp.scan_bytes, "the scan workspace");
+      w.scan_bytes = std::max<size_t>(gp.scan_bytes, 1);
       std::vector<cqx::CandLaunch> cls_launch(cps.size());
-      size_t scan_bytes = 0;
       for (size_t k = 0; k < cps.size(); ++k) {
         const auto &cp = cps[k];
         cqx::CandLaunch &CL = cls_launch[k];
@@ -1046,44 +1225,52 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         CL.nbra = cp.nbra;
         CL.bra_base = 0;
         CL.seg = c->d_seg.p + (size_t)cp.ck * (nmol + 1);
-        CL.molmax = c->d_molmax.p;
+        CL.molmax = w.molmax;
         CL.thresh = c->thresh;
         CL.screen = screen;
         CL.same_class = cp.cb == cp.ck;
-        CL.off = c->d_cand.p + cp.coff;
-        CL.klo = c->d_ket_lo.p + cp.coff;
-        size_t b = 0;
-        HIPCHK(cqx::launch_candidates(CL, nullptr, b, s));  // size query only
-        scan_bytes = std::max(scan_bytes, b);
+        CL.off = w.cand + cp.coff;
+        CL.klo = w.ket_lo + cp.coff;
       }
-      c->d_scan_tmp.ensure(std::max<size_t>(scan_bytes, 1));
+      std::vector<long long> tidx(cps.size());
+      for (size_t k = 0; k < cps.size(); ++k) tidx[k] = (long long)(cps[k].coff + cps[k].nbra);
+      if (!cps.empty()) {
+        w.tidx = ws_upload(c->d_tidx, scr, tidx, s, "the candidate totals");
+        w.tot = ws_take(c->d_tot, scr, cps.size(), "the candidate totals");
+      }
+      // AX of the outputs: the staged copies, or the caller's own device
+      // matrices for the allocation-free entry point
+      auto ax_of = [&](int m, int j) -> double * {
+        if (scr) return static_cast<double *>(list[m * nper + start + j].AX);
+        return w.AX + xoff[m] + (size_t)j * mnb2[m] * ew;
+      };
       // --- launches up to the candidate counts: shell-block norms, per
       // molecule, and each molecule's largest one; the pairs' own norms;
       // cleared accumulators; Cartesian density planes; candidate offsets ---
       auto front = [&]() {
         for (int m = 0; m < nmol; ++m)
-          HIPCHK(cqx::launch_block_norms(c->d_X.p + xoff[m], nmat, is_complex, c->d_her.p, mnb[m], mns[m],
+          HIPCHK(cqx::launch_block_norms(w.X + xoff[m], nmat, is_complex, w.her, mnb[m], mns[m],
                                          c->d_sh_bf.p + BS.mol_sh0[m], BS.mol_bf0[m], c->d_sh_nf.p + BS.mol_sh0[m],
-                                         c->d_Bm.p + (size_t)c->mol_boff[m] * nmat, c->d_Bg.p + c->mol_boff[m], s));
-        HIPCHK(cqx::launch_mol_max(c->d_Bg.p, c->d_mol_boff.p, c->d_mol_ns.p, nmol, c->d_molmax.p, s));
+                                         w.Bm + (size_t)c->mol_boff[m] * nmat, w.Bg + c->mol_boff[m], s));
+        HIPCHK(cqx::launch_mol_max(w.Bg, c->d_mol_boff.p, c->d_mol_ns.p, nmol, w.molmax, s));
         for (int k = 0; k < kNumClass; ++k)
           if (!c->cls[k].empty())
-            HIPCHK(cqx::launch_pair_norms(c->d_scr[k].p, (int)c->cls[k].size(), c->d_Bg.p, c->d_Bm.p,
-                                          c->d_mol_boff.p, c->d_mol_ns.p, nmat, c->d_pn[k].p, s));
-        HIPCHK(cqx::launch_zero(c->d_acc.p, poff[nmol], s));
+            HIPCHK(cqx::launch_pair_norms(c->d_scr[k].p, (int)c->cls[k].size(), w.Bg, w.Bm, c->d_mol_boff.p,
+                                          c->d_mol_ns.p, nmat, w.pn[k], s));
+        HIPCHK(cqx::launch_zero(w.acc, poff[nmol], s));
         for (int m = 0; m < nmol; ++m) {
           const cqx::CartMap M = mol_map(m);
           for (int j = 0; j < nmat; ++j) {
             const int pr = slot_re[j], pi = slot_im[j];
             double *xre = const_cast<double *>(mp[m].kx[pd[pr].kind]) + pidx[pr];
             double *xim = pi >= 0 ? const_cast<double *>(mp[m].kx[pd[pi].kind]) + pidx[pi] : nullptr;
-            HIPCHK(cqx::launch_to_cart(M, c->d_X.p + xoff[m] + (size_t)j * mnb2[m] * ew, is_complex, xre, xim,
+            HIPCHK(cqx::launch_to_cart(M, w.X + xoff[m] + (size_t)j * mnb2[m] * ew, is_complex, xre, xim,
                                        ps.kps[pd[pr].kind], s));
           }
         }
         for (auto &CL : cls_launch) {
-          size_t b = c->d_scan_tmp.n;
-          HIPCHK(cqx::launch_candidates(CL, c->d_scan_tmp.p, b, s));
+          size_t b = w.scan_bytes;
+          HIPCHK(cqx::launch_candidates(CL, w.scan, b, s));
         }
       };
       // --- back to pure functions, symmetrise, accumulate into AX ---
@@ -1095,8 +1282,7 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
             const int pr = slot_re[j], pi = slot_im[j];
             const double *are = mp[m].ka[pd[pr].kind] + pidx[pr];
             const double *aim = two ? mp[m].ka[pd[pi].kind] + pidx[pi] : nullptr;
-            HIPCHK(cqx::launch_from_cart(M, are, aim, ps.kps[pd[pr].kind], L[j].her, is_complex,
-                                         c->d_AX.p + xoff[m] + (size_t)j * mnb2[m] * ew, s));
+            HIPCHK(cqx::launch_from_cart(M, are, aim, ps.kps[pd[pr].kind], L[j].her, is_complex, ax_of(m, j), s));
           }
         }
       };
@@ -1105,22 +1291,31 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
       // gathered on the device and read back in one copy
       std::vector<long long> totals(cps.size());
       if (!cps.empty()) {
-        std::vector<long long> tidx(cps.size());
-        for (size_t k = 0; k < cps.size(); ++k) tidx[k] = (long long)(cps[k].coff + cps[k].nbra);
-        c->d_tidx.upload(tidx, s);
-        c->d_tot.ensure(cps.size());
-        HIPCHK(cqx::launch_gather(c->d_cand.p, c->d_tidx.p, (int)cps.size(), c->d_tot.p, s));
-        HIPCHK(hipMemcpyAsync(totals.data(), c->d_tot.p, sizeof(long long) * cps.size(), hipMemcpyDeviceToHost, s));
+        HIPCHK(cqx::launch_gather(w.cand, w.tidx, (int)cps.size(), w.tot, s));
+        HIPCHK(hipMemcpyAsync(totals.data(), w.tot, sizeof(long long) * cps.size(), hipMemcpyDeviceToHost, s));
       }
       HIPCHK(hipStreamSynchronize(s));
+      const bool mixed = c->fp32_cut > 0.0;
+      const int qmul = mixed ? 2 : 1;  // quartet lists per class pair
+      // entries of one list batch: the option, or what the scratch region
+      // leaves after this group's fixed workspace (less a reserve for the
+      // per-batch tables)
+      long long list_cap = c->list_cap;
+      if (scr) {
+        const size_t reserve = 4096 + (cps.size() + 64) * (2 * sizeof(cqx::ListLaunch) + 64);
+        const size_t left = scr->left() > reserve ? scr->left() - reserve : 0;
+        list_cap = std::min<long long>(list_cap, (long long)(left / (sizeof(Quartet) * qmul)));
+        if (list_cap < 1) throw Unsupported("cqx_two_body_contract_noalloc: the scratch region leaves no room for quartet lists");
+      }
       long long total = 0;
       for (size_t k = 0; k < cps.size(); ++k) {
         cps[k].ncand = totals[k];
         total += totals[k];
-        if (totals[k] > c->list_cap) {  // split into bra ranges on the host
+        if (totals[k] > list_cap) {  // split into bra ranges on the host
           cps[k].off.resize((size_t)cps[k].nbra + 1);
-          HIPCHK(hipMemcpy(cps[k].off.data(), c->d_cand.p + cps[k].coff, sizeof(long long) * cps[k].off.size(),
-                           hipMemcpyDeviceToHost));
+          HIPCHK(hipMemcpyAsync(cps[k].off.data(), w.cand + cps[k].coff, sizeof(long long) * cps[k].off.size(),
+                                hipMemcpyDeviceToHost, s));
+          HIPCHK(hipStreamSynchronize(s));
         }
       }
       c->stats.n_candidates += total;
@@ -1131,7 +1326,6 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         if (c->use_wave && cqx::eri_wave_available(la, lb, lc, ld)) return 1;
         return 2;
       };
-      const bool mixed = c->fp32_cut > 0.0;
       // the ERI launch of class pair k's bras from i0 on, in one precision
       // (the count, list and lane choice are set by the caller)
       auto eri_base = [&](const CP &cp, int i0, bool f32) {
@@ -1163,23 +1357,26 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         if (r == 0 && !c->fast_lanes) E.force_g = it->second;
         if (r == 1) E.force_qt = it->second;
       };
-      // one list in launches of at most 2^25 quartets, so that no grid
-      // exceeds 2^31 work-items (the wave kernels take 4 quartets per block)
+      // one list in launches of at most kLaunchPiece quartets, so that no
+      // grid reaches 2^32 work-items (limits.hpp)
       auto launch_eri = [&](const cqx::EriLaunch &E, hipStream_t st) {
-        constexpr long long kPiece = 1ll << 25;
         const int r = route(E.la, E.lb, E.lc, E.ld);
         if (r == 2 && E.fp32) throw std::runtime_error("FP32 quartet list for a class without an FP32 kernel");
-        for (long long b = 0; b < E.nq; b += kPiece) {
+        // the shape a launch picks for itself is at worst the smallest tile
+        const int shape = r == 0 ? (E.force_g > 0 ? E.force_g : 16) : r == 1 ? (E.force_qt > 0 ? E.force_qt : 1) : 1;
+        if (!cqx::launch_fits(E.nq, r, shape)) throw Unsupported("an ERI launch beyond 2^32 work-items");
+        for (long long b = 0; b < E.nq; b += cqx::kLaunchPiece) {
           cqx::EriLaunch P = E;
           P.quartets = E.quartets + b;
-          P.nq = std::min(kPiece, E.nq - b);
+          P.nq = std::min(cqx::kLaunchPiece, E.nq - b);
           P.nq_base = b;
           HIPCHK(r == 0 ? cqx::launch_eri_fast(P, st) : r == 1 ? cqx::launch_eri_wave(P, st) : cqx::launch_eri_team(P, st));
         }
       };
       // side streams for concurrent class-pair launches: created outside any
       // capture, forked from the context stream before the launches and
-      // joined back into it after them (also inside a recorded build)
+      // joined back into it after them (also inside a recorded build); the
+      // allocation-free path launches on the context stream alone
       auto ensure_side = [&](int nst) {
         for (int i = 0; i < nst - 1; ++i)
           if (!c->side[i]) {
@@ -1209,7 +1406,7 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
       long long in_batch = 0, cap_need = 1;
       auto add_piece = [&](int k, int i0, int i1, long long base, long long n) {
         if (n <= 0) return;
-        if (in_batch > 0 && in_batch + n > c->list_cap) {
+        if (in_batch > 0 && in_batch + n > list_cap) {
           batches.emplace_back();
           in_batch = 0;
         }
@@ -1226,26 +1423,32 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         int i0 = 0;
         while (i0 < cp.nbra) {
           int i1 = i0 + 1;  // at least one bra pair per piece
-          while (i1 < cp.nbra && cp.off[i1 + 1] - cp.off[i0] <= c->list_cap) ++i1;
+          while (i1 < cp.nbra && cp.off[i1 + 1] - cp.off[i0] <= list_cap) ++i1;
           add_piece(k, i0, i1, cp.off[i0], cp.off[i1] - cp.off[i0]);
           i0 = i1;
         }
       }
-      c->d_quart.ensure((size_t)cap_need * (mixed ? 2 : 1));
+      size_t max_np = 1;
+      for (const auto &b : batches) max_np = std::max(max_np, b.size());
+      // list workspace: the per-batch tables first, then the lists
+      w.count = ws_take(c->d_count, scr, 2 * max_np, "the list counters");
+      w.psum = ws_take(c->d_psum, scr, 2 * max_np, "the list counters");
+      w.lists = ws_take(c->d_lists, scr, max_np, "the list launch table");
+      w.lstart = ws_take(c->d_lstart, scr, max_np + 1, "the list launch table");
+      w.quart = ws_take(c->d_quart, scr, (size_t)cap_need * qmul, "the quartet lists");
       // per class pair and precision: quartets and primitive quartets of this
       // build (they size the launches of a recorded graph)
       std::vector<double> cp_count(2 * cps.size(), 0.0), cp_psum(2 * cps.size(), 0.0);
       bool tune_ready = false;
+      const bool tuning = c->autotune_pending && !scr;  // the allocation-free path never times
       cqx::PlaneSet ps_tune = ps;
       for (const auto &batch : batches) {
         const size_t np = batch.size();
         if (!np) continue;
         const size_t nslot = 2 * np;
-        c->d_count.ensure(nslot);
-        c->d_psum.ensure(nslot);
         // counters cleared by a fill kernel (no memset nodes if the caller captures a graph)
-        HIPCHK(cqx::launch_zero(reinterpret_cast<double *>(c->d_count.p), nslot, s));
-        HIPCHK(cqx::launch_zero(c->d_psum.p, nslot, s));
+        HIPCHK(cqx::launch_zero(reinterpret_cast<double *>(w.count), nslot, s));
+        HIPCHK(cqx::launch_zero(w.psum, nslot, s));
         std::vector<cqx::ListLaunch> lls(np);
         std::vector<long long> lstart(np + 1, 0);
         for (size_t k = 0; k < np; ++k) {
@@ -1255,24 +1458,24 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           LL = cqx::ListLaunch{};
           LL.bra = c->d_scr[cp.cb].p + pc.i0;
           LL.ket = c->d_scr[cp.ck].p;
-          LL.bra_pn = c->d_pn[cp.cb].p + (size_t)pc.i0 * (nmat + 1);
-          LL.ket_pn = c->d_pn[cp.ck].p;
+          LL.bra_pn = w.pn[cp.cb] + (size_t)pc.i0 * (nmat + 1);
+          LL.ket_pn = w.pn[cp.ck];
           LL.same_class = cp.cb == cp.ck;
-          LL.cand_off = c->d_cand.p + cp.coff + pc.i0;
+          LL.cand_off = w.cand + cp.coff + pc.i0;
           LL.cand_base = pc.base;
-          LL.ket_lo = c->d_ket_lo.p + cp.coff + pc.i0;
+          LL.ket_lo = w.ket_lo + cp.coff + pc.i0;
           LL.nbra = pc.i1 - pc.i0;
           LL.ncand = pc.ncand;
-          LL.Bg = c->d_Bg.p;
-          LL.Bm = c->d_Bm.p;
+          LL.Bg = w.Bg;
+          LL.Bm = w.Bm;
           LL.mol_boff = c->d_mol_boff.p;
           LL.mol_ns = c->d_mol_ns.p;
           LL.nmat = nmat;
           LL.thresh = c->thresh;
           LL.screen = screen;
-          LL.out = c->d_quart.p + pc.seg;
-          LL.count = c->d_count.p + k;
-          LL.prim_sum = c->d_psum.p + k;
+          LL.out = w.quart + pc.seg;
+          LL.count = w.count + k;
+          LL.prim_sum = w.psum + k;
           LL.cap = pc.ncand;
           LL.cap32 = pc.ncand;
           int la_, lb_, lc_, ld_;
@@ -1280,26 +1483,25 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           class_ls(cp.ck, lc_, ld_);
           if (mixed && route(la_, lb_, lc_, ld_) < 2) {
             LL.cut32 = c->fp32_cut;
-            LL.out32 = c->d_quart.p + cap_need + pc.seg;
-            LL.count32 = c->d_count.p + np + k;
-            LL.prim_sum32 = c->d_psum.p + np + k;
+            LL.out32 = w.quart + cap_need + pc.seg;
+            LL.count32 = w.count + np + k;
+            LL.prim_sum32 = w.psum + np + k;
           }
           lstart[k + 1] = lstart[k] + (pc.ncand + cqx::kListChunk - 1) / cqx::kListChunk;
         }
-        c->d_lists.upload(lls, s);
-        c->d_lstart.upload(lstart, s);
-        HIPCHK(cqx::launch_quartet_lists(c->d_lists.p, c->d_lstart.p, (int)np, lstart[np], s));
+        HIPCHK(hipMemcpyAsync(w.lists, lls.data(), sizeof(cqx::ListLaunch) * np, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(w.lstart, lstart.data(), sizeof(long long) * (np + 1), hipMemcpyHostToDevice, s));
+        HIPCHK(cqx::launch_quartet_lists(w.lists, w.lstart, (int)np, lstart[np], s));
         std::vector<unsigned long long> counts(nslot);
         std::vector<double> psum(nslot);
-        HIPCHK(hipMemcpyAsync(counts.data(), c->d_count.p, sizeof(unsigned long long) * nslot,
-                              hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(psum.data(), c->d_psum.p, sizeof(double) * nslot, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(counts.data(), w.count, sizeof(unsigned long long) * nslot, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(psum.data(), w.psum, sizeof(double) * nslot, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         // ERI + digestion: one launch per class pair and precision for every
         // molecule of the batch
         HIPCHK(hipEventRecord(c->ev0, s));
         // autotuning times launches one by one on the context stream
-        const int nst = c->autotune_pending ? 1 : c->nstreams;
+        const int nst = (tuning || scr) ? 1 : c->nstreams;
         ensure_side(nst);
         fork_side(nst);
         int nlaunched = 0;
@@ -1312,19 +1514,19 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           cp_count[pc.cp + (f32 ? cps.size() : 0)] += (double)counts[kk];
           cp_psum[pc.cp + (f32 ? cps.size() : 0)] += psum[kk];
           cqx::EriLaunch E = eri_base(cp, pc.i0, f32);
-          E.quartets = c->d_quart.p + (f32 ? cap_need : 0) + pc.seg;
+          E.quartets = w.quart + (f32 ? cap_need : 0) + pc.seg;
           E.nq = (long long)counts[kk];
           E.mean_prims = psum[kk] / (double)counts[kk];
           const int rt = route(E.la, E.lb, E.lc, E.ld);
           if (rt <= 1) {
             const int key = tune_key(E);
-            if (c->autotune_pending && !(rt == 0 && c->fast_lanes) && !c->tuned_lanes.count(key)) {
+            if (tuning && !(rt == 0 && c->fast_lanes) && !c->tuned_lanes.count(key)) {
               // time each shape on scratch accumulators (results discarded)
               if (!tune_ready) {
                 c->d_acc_tune.ensure(std::max<size_t>(poff[nmol], 1));
                 std::vector<cqx::MolPlanes> mt = mp;
                 for (int m = 0; m < nmol; ++m)
-                  for (int k = 0; k < 4; ++k) mt[m].ka[k] = c->d_acc_tune.p + (mp[m].ka[k] - c->d_acc.p);
+                  for (int k = 0; k < 4; ++k) mt[m].ka[k] = c->d_acc_tune.p + (mp[m].ka[k] - w.acc);
                 c->d_molplanes_tune.upload(mt, s);
                 ps_tune = ps;
                 ps_tune.mols = c->d_molplanes_tune.p;
@@ -1383,12 +1585,15 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         eri_ms_total += ms;
       }
       back();
-      const hipMemcpyKind dk = mem == CQX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-      for (int m = 0; m < nmol; ++m)
-        for (int j = 0; j < nmat; ++j)
-          HIPCHK(hipMemcpyAsync(list[m * nper + start + j].AX, c->d_AX.p + xoff[m] + (size_t)j * mnb2[m] * ew,
-                                sizeof(double) * mnb2[m] * ew, dk, s));
+      if (!scr) {
+        const hipMemcpyKind dk = mem == CQX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+        for (int m = 0; m < nmol; ++m)
+          for (int j = 0; j < nmat; ++j)
+            HIPCHK(hipMemcpyAsync(list[m * nper + start + j].AX, w.AX + xoff[m] + (size_t)j * mnb2[m] * ew,
+                                  sizeof(double) * mnb2[m] * ew, dk, s));
+      }
       HIPCHK(hipStreamSynchronize(s));
+      if (scr) scratch_peak = std::max(scratch_peak, scr->used);
       // Record the same build into a graph for the next calls (option
       // graph; one contraction group and one list batch).  Capacities are
       // graph_margin times this build's own sizes: each class pair's
@@ -1410,11 +1615,14 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           lcap[ncp + k] = (long long)std::ceil(cp_count[ncp + k] * c->graph_margin) + 1024;
       }
       for (size_t kk = 0; kk < 2 * ncp; ++kk) lseg[kk + 1] = lseg[kk] + lcap[kk];
-      if (c->use_graph && start == 0 && nmat == nper && batches.size() == 1 && ncp > 0 &&
-          lseg[2 * ncp] <= c->list_cap * (mixed ? 2 : 1)) {
+      if (!scr && c->use_graph && start == 0 && nmat == nper && batches.size() == 1 && ncp > 0 &&
+          lseg[2 * ncp] <= c->list_cap * qmul) {
         c->d_quart.ensure((size_t)lseg[2 * ncp]);
         c->d_count.ensure(2 * ncp);
         c->d_psum.ensure(2 * ncp);
+        w.quart = c->d_quart.p;
+        w.count = c->d_count.p;
+        w.psum = c->d_psum.p;
         std::vector<cqx::ListLaunch> gl(ncp);
         std::vector<long long> gidx(ncp);
         std::vector<cqx_ctx::GraphSlot> slots(2 * ncp, cqx_ctx::GraphSlot{-1, 0, 0, 0, 0});
@@ -1424,25 +1632,25 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           LL = cqx::ListLaunch{};
           LL.bra = c->d_scr[cp.cb].p;
           LL.ket = c->d_scr[cp.ck].p;
-          LL.bra_pn = c->d_pn[cp.cb].p;
-          LL.ket_pn = c->d_pn[cp.ck].p;
+          LL.bra_pn = w.pn[cp.cb];
+          LL.ket_pn = w.pn[cp.ck];
           LL.same_class = cp.cb == cp.ck;
-          LL.cand_off = c->d_cand.p + cp.coff;
+          LL.cand_off = w.cand + cp.coff;
           LL.cand_base = 0;
-          LL.ket_lo = c->d_ket_lo.p + cp.coff;
+          LL.ket_lo = w.ket_lo + cp.coff;
           LL.nbra = cp.nbra;
           LL.ncand = gcap[k];
-          LL.ncand_dev = c->d_cand.p + cp.coff + cp.nbra;
-          LL.Bg = c->d_Bg.p;
-          LL.Bm = c->d_Bm.p;
+          LL.ncand_dev = w.cand + cp.coff + cp.nbra;
+          LL.Bg = w.Bg;
+          LL.Bm = w.Bm;
           LL.mol_boff = c->d_mol_boff.p;
           LL.mol_ns = c->d_mol_ns.p;
           LL.nmat = nmat;
           LL.thresh = c->thresh;
           LL.screen = screen;
-          LL.out = c->d_quart.p + lseg[k];
-          LL.count = c->d_count.p + k;
-          LL.prim_sum = c->d_psum.p + k;
+          LL.out = w.quart + lseg[k];
+          LL.count = w.count + k;
+          LL.prim_sum = w.psum + k;
           LL.cap = lcap[k];
           int la_, lb_, lc_, ld_;
           class_ls(cp.cb, la_, lb_);
@@ -1450,10 +1658,10 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           slots[k] = {la_, lb_, lc_, ld_, 0};
           if (mixed && route(la_, lb_, lc_, ld_) < 2) {
             LL.cut32 = c->fp32_cut;
-            LL.out32 = c->d_quart.p + lseg[ncp + k];
+            LL.out32 = w.quart + lseg[ncp + k];
             LL.cap32 = lcap[ncp + k];
-            LL.count32 = c->d_count.p + ncp + k;
-            LL.prim_sum32 = c->d_psum.p + ncp + k;
+            LL.count32 = w.count + ncp + k;
+            LL.prim_sum32 = w.psum + ncp + k;
             slots[ncp + k] = {la_, lb_, lc_, ld_, 1};
           }
           gidx[k] = (long long)(cp.coff + cp.nbra);
@@ -1470,8 +1678,8 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
         bool forked = false;
         try {
           front();
-          HIPCHK(cqx::launch_zero(reinterpret_cast<double *>(c->d_count.p), 2 * ncp, s));
-          HIPCHK(cqx::launch_zero(c->d_psum.p, 2 * ncp, s));
+          HIPCHK(cqx::launch_zero(reinterpret_cast<double *>(w.count), 2 * ncp, s));
+          HIPCHK(cqx::launch_zero(w.psum, 2 * ncp, s));
           HIPCHK(cqx::launch_quartet_lists(c->d_glists.p, c->d_glstart.p, (int)ncp, glst[ncp], s));
           fork_side(nst);
           forked = true;
@@ -1481,9 +1689,9 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
             const size_t k = kk % ncp;
             const bool f32 = kk >= ncp;
             cqx::EriLaunch E = eri_base(cps[k], 0, f32);
-            E.quartets = c->d_quart.p + lseg[kk];
+            E.quartets = w.quart + lseg[kk];
             E.nq = lcap[kk];
-            E.nq_dev = c->d_count.p + kk;
+            E.nq_dev = w.count + kk;
             const double n0 = cp_count[kk];
             E.mean_prims = n0 > 0.0 ? cp_psum[kk] / n0 : 1.0;
             apply_tuned(E);
@@ -1492,7 +1700,7 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
           join_side(nst);
           forked = false;
           back();
-          HIPCHK(cqx::launch_gather(c->d_cand.p, c->d_gidx.p, (int)ncp, c->d_gchk.p, s));
+          HIPCHK(cqx::launch_gather(w.cand, c->d_gidx.p, (int)ncp, c->d_gchk.p, s));
         } catch (...) {
           if (forked)  // an unjoined side stream would leave the capture open
             for (int i = 0; i < nst - 1; ++i) {
@@ -1527,12 +1735,81 @@ int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contractio
       start += nmat;
     }
     c->stats.ms_eri = eri_ms_total;
-    c->autotune_pending = false;
+    if (!scr) c->autotune_pending = false;
+    c->ws_scratch_peak = scratch_peak;
+    c->stats.workspace_bytes = scr ? (long long)scratch_peak : (long long)workspace_bytes(c);
+  } catch (const Unsupported &e) {
+    return fail(c, CQX_ERR_UNSUPPORTED, e.what());
   } catch (const std::exception &e) {
     return fail(c, CQX_ERR_HIP, e.what());
   }
   c->stats.ms_total =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  return CQX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int is_complex,
+                          int mem) {
+  return contract(c, screen, ncon, list, is_complex, mem, nullptr);
+}
+
+int cqx_scratch_size(cqx_ctx *c, int ncon, const cqx_contraction *list, int is_complex, size_t *bytes) {
+  if (!c || ncon <= 0 || !list || !bytes) return fail(c, CQX_ERR_ARG, "cqx_scratch_size: bad arguments");
+  const int nmol = c->nmol;
+  if (ncon % nmol)
+    return fail(c, CQX_ERR_ARG, "a batched context takes the same number of contractions for every molecule");
+  const int nper = ncon / nmol;
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    // the buffers grow to the largest group's need; every group is sized alike
+    size_t x = 0, bm = 0, bg = 0, planes = 0, cand = 0, scan = 0, ncp = 0, nmatm = 0, pn[kNumClass] = {};
+    long long quart = 1;
+    const bool mixed = c->fp32_cut > 0.0;
+    const int qmul = mixed ? 2 : 1;
+    int start = 0;
+    while (start < nper) {
+      const int nmat = group_size(list + start, nper - start, is_complex, nullptr);
+      const GroupPlan g = plan_group(c, list + start, nmat, is_complex, c->stream);
+      x = std::max(x, g.x);
+      bm = std::max(bm, g.bm);
+      bg = std::max(bg, g.bg);
+      planes = std::max(planes, g.planes);
+      cand = std::max(cand, g.cand);
+      scan = std::max(scan, g.scan_bytes);
+      ncp = std::max(ncp, g.ncp);
+      nmatm = std::max(nmatm, (size_t)nmat);
+      for (int k = 0; k < kNumClass; ++k) pn[k] = std::max(pn[k], g.pn[k]);
+      long long q = list_entries_bound(g, c->list_cap) * qmul;
+      if (c->use_graph) q = std::max(q, c->list_cap * qmul);  // a recorded build's lists
+      quart = std::max(quart, q);
+      start += nmat;
+    }
+    // pieces of a batch: each class pair whole, or bra ranges of one
+    size_t nbra_all = 0;
+    for (int cb = 0; cb < kNumClass; ++cb) nbra_all += c->cls[cb].size() * (size_t)(cb + 1);
+    const size_t npiece = ncp + nbra_all;
+    size_t b = 0;
+    b += 2 * x * sizeof(double);                     // staged densities and outputs
+    b += nmatm * sizeof(int);                        // HER flags
+    b += (bm + bg + nmol) * sizeof(double);          // block norms, molecule maxima
+    for (int k = 0; k < kNumClass; ++k) b += pn[k] * sizeof(double);
+    b += 2 * planes * sizeof(double);                // Cartesian planes and accumulators
+    if (c->autotune_pending) b += planes * sizeof(double) + nmol * sizeof(cqx::MolPlanes);
+    b += 2 * nmol * sizeof(cqx::MolPlanes);
+    b += cand * (sizeof(long long) + sizeof(int)) + scan;
+    b += 2 * ncp * sizeof(long long);                // gathered candidate totals
+    b += (size_t)quart * sizeof(Quartet);            // quartet lists
+    b += npiece * (2 * sizeof(unsigned long long) + 2 * sizeof(double) + sizeof(cqx::ListLaunch) +
+                   sizeof(long long)) + sizeof(long long);
+    if (c->use_graph) b += ncp * (sizeof(cqx::ListLaunch) + 4 * sizeof(long long)) + sizeof(long long);
+    *bytes = b;
+  } catch (const std::exception &e) {
+    return fail(c, CQX_ERR_HIP, e.what());
+  }
   return CQX_OK;
 }
 

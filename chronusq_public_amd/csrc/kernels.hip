@@ -11,6 +11,8 @@
 // integral batch to pure functions.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "digest.hpp"
 #include "kernels.hpp"
 #include "rys.hpp"
@@ -544,13 +546,14 @@ hipError_t launch_eri_team(const EriLaunch &a, hipStream_t s) {
   const int ncq = nc(a.la) * nc(a.lb) * nc(a.lc) * nc(a.ld);
   const size_t lds = sizeof(double) * (16 + (size_t)nr * 3 * nh + ncq);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
-  static bool attr_set = false;
-  if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute((const void *)eri_team_kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    if (e != hipSuccess) return e;
-    attr_set = true;
-  }
+  // once per process (contexts on several host threads launch concurrently)
+  static std::once_flag attr_once;
+  static hipError_t attr_err = hipSuccess;
+  std::call_once(attr_once, [] {
+    attr_err = hipFuncSetAttribute((const void *)eri_team_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   160 * 1024);
+  });
+  if (attr_err != hipSuccess) return attr_err;
   const long long grid = a.nq < 65536 ? a.nq : 65536;
   hipLaunchKernelGGL(eri_team_kernel, dim3((unsigned)grid), dim3(64), lds, s, a);
   return hipGetLastError();

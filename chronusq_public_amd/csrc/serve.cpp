@@ -95,6 +95,8 @@ int cqx_serve(const cqx_job *jobs, int njobs, const cqx_serve_opts *opts, cqx_jo
     if (rc == CQX_OK) {
       cqx_scf_result r{};
       rc = cqx_scf_run(scf, &r);
+      if (rc == CQX_OK && J.density_out) rc = cqx_scf_matrix(scf, CQX_SCF_DENSITY, J.density_out);
+      if (rc == CQX_OK && J.fock_out) rc = cqx_scf_matrix(scf, CQX_SCF_FOCK, J.fock_out);
       R.e_total = r.e_total;
       R.iterations = r.iterations;
       R.converged = r.converged;

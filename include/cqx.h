@@ -139,6 +139,19 @@ int cqx_two_body_contract(cqx_ctx *ctx, int screen, int ncontract,
                           const cqx_contraction *list, int is_complex,
                           int mem);
 
+/* GTODirectTPIContraction::directScaffoldNewSCRSize (direct.hpp:2125-2180),
+ * which FockBuilder::formRawGDSCRSizePerBatch (fockbuilder/impl.hpp:252-269)
+ * multiplies by the contractions of a batch: *bytes receives an upper bound
+ * of the device memory one cqx_two_body_contract call with this list
+ * allocates on the context (densities and outputs staged on the device,
+ * shell-block norms, Cartesian planes and accumulators, candidate offsets,
+ * quartet lists and their tables), under the context's current options
+ * ("list_cap", "fp32_cut", "graph", "autotune").  Only the types and HER
+ * flags of the list are read; X / AX may be NULL.  The workspace stays
+ * with the context for later calls (cqx_stats.workspace_bytes). */
+int cqx_scratch_size(cqx_ctx *ctx, int ncontract, const cqx_contraction *list,
+                     int is_complex, size_t *bytes);
+
 /* Contracted shell-quartet ERIs for nq quartets (s1,s2,s3,s4 arrays), in
  * the libint2 layout the reference digests (row-major ijkl, pure
  * functions where the shell is pure).  out receives the quartets back to
@@ -161,6 +174,8 @@ typedef struct {
   int graph;                /* 0 launched eagerly, 1 eagerly and recorded into a graph,
                                2 a replay of the recorded graph (ms_eri then covers
                                the whole recorded build) */
+  long long workspace_bytes; /* device workspace the context holds after the call
+                                (cqx_two_body_contract_noalloc: scratch bytes it used) */
 } cqx_stats;
 int cqx_last_stats(const cqx_ctx *ctx, cqx_stats *st);
 
@@ -208,6 +223,45 @@ int cqx_grad_two_body_contract(cqx_ctx *ctx, int ngrad, int nmat, const cqx_cont
  * stream only in the order of the FP64 atomic additions).  Setting any
  * option drops a recorded graph. */
 int cqx_set_option(cqx_ctx *ctx, const char *name, double value);
+
+/* ---------------------------------------------------------------------
+ * Pauli-spinor components.  A two-component matrix A (2NB x 2NB, spin
+ * blocks AA, AB / BA, BB) and its components S = AA + BB, Z = AA - BB,
+ * Y = f (AB - BA), X = AB + BA, with f = i for complex matrices and -1 for
+ * real ones (ComplexScale, src/cqlinalg/blasutil.cxx:67-69).  Each call runs
+ * on the context's GPU; mem says whether every matrix argument is a host or
+ * a device pointer.  Column-major throughout; lda >= 2 NB.
+ * ------------------------------------------------------------------- */
+enum {
+  CQX_SPIN_ZERO_ABBA = 1, /* scatter: Y = X = 0 (zeroABBA) */
+  CQX_SPIN_BB_EQ_AA = 2,  /* scatter: S = 2 AA, Z = 0 (BBeqAA) */
+  CQX_SPIN_ZERO_XY = 4,   /* gather: AB = BA = 0 (zeroXY) */
+  CQX_SPIN_ZERO_Z = 8     /* gather: AA = BB = S / 2 (zeroZ) */
+};
+/* SpinScatter(N, A, LDA, AS, AZ, AY, AX, zeroABBA, BBeqAA)
+ * (blasutil.cxx:71-129): NB x NB components out; any of S, Z, Y, X may be
+ * NULL (not written). */
+int cqx_spin_scatter(cqx_ctx *ctx, int nb, const void *A, int lda, void *S, void *Z,
+                     void *Y, void *X, int is_complex, int flags, int mem);
+/* SpinGather(N, A, LDA, AS, AZ, AY, AX, zeroXY, zeroZ) (blasutil.cxx:131-189):
+ * AA = (S + Z)/2, BB = (S - Z)/2, BA = (X + f Y)/2, AB = (X - f Y)/2.  Z may be
+ * NULL with CQX_SPIN_ZERO_Z, Y and X with CQX_SPIN_ZERO_XY. */
+int cqx_spin_gather(cqx_ctx *ctx, int nb, void *A, int lda, const void *S, const void *Z,
+                    const void *Y, const void *X, int is_complex, int flags, int mem);
+/* SingleSlater::computeEnergy (include/singleslater/quantum.hpp:147-212):
+ * out[0] = E1 = 1/2 sum_c Re<D_c, H_c>, out[1] = E2 = 1/4 sum_c Re<D_c, G_c>,
+ * out[2] = E1 + E2 + e_nuc, over ncomp (1: S; 2: S, Z; 4: S, Z, Y, X)
+ * NB x NB components, Re<A, B> = Re sum conj(A) B (OperatorTrace,
+ * quantum/base.hpp:46-51).  H or G (or single components of them) may be
+ * NULL: that term is then zero.  A non-finite energy is CQX_ERR_ARG, as the
+ * reference asserts. */
+int cqx_compute_energy(cqx_ctx *ctx, int nb, int ncomp, const void *const *D,
+                       const void *const *H, const void *const *G, int is_complex,
+                       int mem, double e_nuc, double *out);
+/* PauliSpinorMatrices::hasNaN (include/matrix/paulispinormatrices.hpp:383-388),
+ * extended to infinities: *count receives how many of the n doubles at M
+ * are not finite (a complex matrix is 2 NB^2 doubles). */
+int cqx_count_nonfinite(cqx_ctx *ctx, const double *M, size_t n, int mem, size_t *count);
 
 /* ---------------------------------------------------------------------
  * Device memory for callers that keep densities and results resident in
@@ -296,6 +350,8 @@ typedef struct {
   const double *xyz;   /* 3 natom, bohr */
   int nocc;            /* doubly occupied orbitals */
   double arrival_s;    /* arrival time, seconds after the call starts */
+  double *density_out; /* NULL, or NB x NB host buffer: the final density D_S */
+  double *fock_out;    /* NULL, or NB x NB host buffer: the Fock matrix F_S of D_S */
 } cqx_job;
 
 typedef struct {

@@ -20,8 +20,12 @@ def test_adapter_header_compiles():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["water_631gd_rhf", "kcakr_sto3g_ghf"])
+@pytest.mark.parametrize("name", ["water_631gd_rhf", "kcakr_sto3g_ghf", "water_ccpvdz_unc_x2chf_readden"])
 def test_cpp_formgd_against_fixture(name, tmp_path):
+    """formGD / formFock / computeEnergy through the C++ adapter's reference
+    signatures: G = F - H, F and the one- and many-body energies of the
+    ChronusQ file; the driver itself checks graph replay, the incremental
+    build and the SpinGather / SpinScatter round trip."""
     assert os.path.exists(DRIVER), "build tests/cpp/formgd first"
     meta, d = load_fixture("scf_" + name)
     mol, gbs = fixture_molecule(meta)
@@ -32,17 +36,22 @@ def test_cpp_formgd_against_fixture(name, tmp_path):
     with open(tmp_path / "g.txt", "w") as f:
         for z, r in zip(mol.Z, mol.xyz):
             f.write("%d %.17g %.17g %.17g\n" % (int(z), r[0], r[1], r[2]))
-    with open(tmp_path / "d.bin", "wb") as f:
-        for c in comps:
-            f.write(np.asfortranarray(d["PDM_" + c].astype(dt)).tobytes(order="F"))
-    print("case: cpp formGD", name, flush=True, file=sys.stderr)
+    for fn, key in (("d.bin", "PDM_"), ("h.bin", "HCORE_")):
+        with open(tmp_path / fn, "wb") as f:
+            for c in comps:
+                f.write(np.asfortranarray(d[key + c].astype(dt)).tobytes(order="F"))
+    print("case: cpp formGD/formFock", name, flush=True, file=sys.stderr)
     r = subprocess.run([DRIVER, str(tmp_path / "b.g94"), str(tmp_path / "g.txt"), str(len(comps)),
-                        str(tmp_path / "d.bin"), str(tmp_path / "o.bin")],
+                        str(tmp_path / "d.bin"), str(tmp_path / "h.bin"), str(tmp_path / "o.bin")],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     nb = d["OVERLAP"].shape[0]
-    G = np.fromfile(tmp_path / "o.bin", dtype=dt).reshape(len(comps), nb, nb)
+    out = np.fromfile(tmp_path / "o.bin", dtype=dt).reshape(2 * len(comps), nb, nb)
     for i, c in enumerate(comps):
-        Gi = G[i].T  # column-major buffer
-        ref = d["FOCK_" + c] - d["HCORE_" + c]
-        assert np.abs(Gi - ref).max() < 1e-9, c
+        G, F = out[i].T, out[len(comps) + i].T  # column-major buffers
+        assert np.abs(G - (d["FOCK_" + c] - d["HCORE_" + c])).max() < 1e-9, c
+        assert np.abs(F - d["FOCK_" + c]).max() < 1e-9, c
+    e1, e2 = (float(x) for x in r.stdout.split("energies")[1].split()[:2])
+    ref1, ref2 = float(d["ONE_BODY_ENERGY"].ravel()[0]), float(d["MANY_BODY_ENERGY"].ravel()[0])
+    assert abs(e1 - ref1) < 1e-9 * max(1.0, abs(ref1)), (e1, ref1)
+    assert abs(e2 - ref2) < 1e-9 * max(1.0, abs(ref2)), (e2, ref2)

@@ -1,0 +1,211 @@
+"""The drop-in boundary beyond a single contraction call, on the GPU:
+FockBuilder's incremental build, several batches in one call, formFock,
+the reference's twoBodyContract forms, the Pauli-spinor operations
+(SpinScatter / SpinGather, computeEnergy, hasNaN) and the scratch-size
+query.  Every result is held to ChronusQ's own files or to the oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import fixture_molecule, load_fixture
+from oracle.oracle import OracleBasis
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True, file=sys.stderr)
+
+
+def setup(name):
+    from chronusq_public_amd import cqx
+    meta, d = load_fixture("scf_" + name)
+    mol, gbs = fixture_molecule(meta)
+    b = cqx.BasisSet(gbs, mol)
+    tpi = cqx.DirectTPI(b, 1e-12)
+    gto = cqx.GTODirectTPIContraction(tpi)
+    comps = [c for c in ("SCALAR", "MZ", "MY", "MX") if "PDM_" + c in d]
+    cplx = len(comps) > 1 or any(np.iscomplexobj(d["PDM_" + c]) for c in comps)
+    dt = np.complex128 if cplx else np.float64
+    D = [d["PDM_" + c].astype(dt) for c in comps]
+    H = [d["HCORE_" + c].astype(dt) for c in comps]
+    F = [d["FOCK_" + c].astype(dt) for c in comps]
+    return d, mol, gbs, tpi, gto, D, H, F
+
+
+@pytest.mark.parametrize("name", ["water_631gd_rhf", "b_sto3g_ghf"])
+def test_increment_equals_sum_of_builds(name):
+    """formGD(increment=True) on deltaOnePDM accumulates G and K without
+    zeroing (impl.hpp:75-137): G[D/3] followed by the increment D - D/3
+    equals G[D], which equals ChronusQ's F - H; coulombMatrix holds
+    J[deltaOnePDM] alone."""
+    from chronusq_public_amd import cqx
+    d, mol, gbs, tpi, gto, D, H, F = setup(name)
+    say("case: increment", name)
+    part = [x / 3.0 for x in D]
+    delta = [x - p for x, p in zip(D, part)]
+    ss = cqx.SingleSlaterState(part, gto, deltaOnePDM=delta)
+    cqx.FockBuilder.formGD(ss)
+    Jpart = ss.coulombMatrix.copy()
+    cqx.FockBuilder.formGD(ss, increment=True)
+    full = cqx.SingleSlaterState(D, gto)
+    cqx.FockBuilder.formGD(full)
+    for c in range(len(D)):
+        assert np.abs(ss.twoeH[c] - full.twoeH[c]).max() < 1e-10
+        assert np.abs(ss.exchangeMatrix[c] - full.exchangeMatrix[c]).max() < 1e-10
+        assert np.abs(ss.twoeH[c] - (F[c] - H[c])).max() < 1e-9
+    # J is linear: J[delta] = 2 J[D/3] here, and J[D] = J[D/3] + J[delta]
+    assert np.abs(ss.coulombMatrix - 2.0 * Jpart).max() < 1e-10
+    assert np.abs(full.coulombMatrix - (Jpart + ss.coulombMatrix)).max() < 1e-10
+
+
+def test_several_batches_in_one_call():
+    """formRawGDInBatches with three densities (impl.hpp:109-244): one
+    twoBodyContract call builds every batch; each equals its own formGD,
+    and the batch of the file's density equals ChronusQ's F - H."""
+    from chronusq_public_amd import cqx
+    d, mol, gbs, tpi, gto, D, H, F = setup("water_631gd_rhf")
+    say("case: formRawGDInBatches, 3 batches")
+    rng = np.random.Generator(np.random.PCG64(5))
+    nb = D[0].shape[0]
+    C = rng.normal(0, 1 / np.sqrt(nb), size=(nb, 5))
+    dens = [D, [C @ C.T], [0.5 * (D[0] + C @ C.T)]]
+    ss = cqx.SingleSlaterState(D, gto)
+    J = [np.zeros((nb, nb), order="F") for _ in dens]
+    K = [[np.zeros((nb, nb), order="F")] for _ in dens]
+    G = [[np.zeros((nb, nb), order="F")] for _ in dens]
+    cqx.FockBuilder.formRawGDInBatches(ss, None, False, 1.0, True, dens, J, K, G)
+    assert tpi.last_stats()["n_quartets"] > 0
+    for i, Di in enumerate(dens):
+        one = cqx.SingleSlaterState(Di, gto)
+        cqx.FockBuilder.formGD(one)
+        assert np.abs(G[i][0] - one.twoeH[0]).max() < 1e-11
+        assert np.abs(J[i] - one.coulombMatrix).max() < 1e-11
+    assert np.abs(G[0][0] - (F[0] - H[0])).max() < 1e-9
+
+
+@pytest.mark.parametrize("name", ["water_631gd_rhf", "kcakr_sto3g_ghf", "water_ccpvdz_unc_x2chf_readden"])
+def test_formfock_and_energy_against_file(name):
+    """formFock: F = H + G[D] against ChronusQ's FOCK; computeEnergy on the
+    GPU: the file's ONE_BODY_ENERGY and MANY_BODY_ENERGY (E2 = 1/4 sum_c
+    Re<D_c, G_c>, quantum.hpp:147-212); fockMatrix has no NaN."""
+    from chronusq_public_amd import cqx
+    d, mol, gbs, tpi, gto, D, H, F = setup(name)
+    say("case: formFock + computeEnergy", name)
+    ss = cqx.SingleSlaterState(D, gto, coreH=H)
+    cqx.FockBuilder.formFock(ss, cqx.EMPerturbation())
+    assert ss.GDDur > 0.0
+    for c in range(len(D)):
+        assert np.abs(ss.fockMatrix[c] - F[c]).max() < 1e-9
+    e1, e2, _ = cqx.computeEnergy(tpi, ss.onePDM, ss.coreH, ss.twoeH)
+    r1, r2 = float(d["ONE_BODY_ENERGY"].ravel()[0]), float(d["MANY_BODY_ENERGY"].ravel()[0])
+    say(f"  E1 {e1:.12f} (file {r1:.12f}), E2 {e2:.12f} (file {r2:.12f})")
+    assert abs(e1 - r1) < 1e-9 * max(1.0, abs(r1))
+    assert abs(e2 - r2) < 1e-9 * max(1.0, abs(r2))
+    assert not cqx.PauliSpinorMatrices.from_components(ss.fockMatrix).hasNaN(tpi)
+
+
+def test_formfock_electric_field_needs_dipoles():
+    from chronusq_public_amd import cqx
+    d, mol, gbs, tpi, gto, D, H, F = setup("water_631gd_rhf")
+    ss = cqx.SingleSlaterState(D, gto, coreH=H)
+    pert = cqx.EMPerturbation()
+    pert.addField("Electric", [0.0, 0.0, 0.01])
+    with pytest.raises(cqx.CErr):
+        cqx.FockBuilder.formFock(ss, pert)
+    # with dipole matrices the field enters as -2 E.mu on the scalar part
+    mu = [np.eye(D[0].shape[0]) * (k + 1) for k in range(3)]
+    ss = cqx.SingleSlaterState(D, gto, coreH=H, lenElectric=mu)
+    cqx.FockBuilder.formFock(ss, pert)
+    assert np.abs(ss.fockMatrix[0] - (F[0] - 2 * 0.01 * mu[2])).max() < 1e-9
+
+
+def test_spin_scatter_gather():
+    """SpinScatter / SpinGather (blasutil.cxx:71-189) on the GPU: the
+    components of a spinor matrix built from known blocks, the BBeqAA and
+    zeroABBA forms, the real-matrix Y factor (-1), and gather(scatter(A)) = A."""
+    from chronusq_public_amd import cqx
+    d, mol, gbs, tpi, gto, D, H, F = setup("b_sto3g_ghf")
+    say("case: SpinScatter / SpinGather")
+    nb = D[0].shape[0]
+    rng = np.random.Generator(np.random.PCG64(2))
+    blk = [rng.normal(size=(nb, nb)) + 1j * rng.normal(size=(nb, nb)) for _ in range(4)]
+    aa, ab, ba, bb = blk
+    A = np.block([[aa, ab], [ba, bb]])
+    S, Z, Y, X = cqx.SpinScatter(tpi, A)
+    # the Pauli decomposition: A = 1/2 (S I + Z sz + Y sy + X sx) blockwise
+    assert np.abs(S - (aa + bb)).max() == 0.0 and np.abs(Z - (aa - bb)).max() == 0.0
+    assert np.abs(X - (ab + ba)).max() == 0.0
+    assert np.abs(Y - 1j * (ab - ba)).max() < 1e-15
+    assert np.abs(cqx.SpinGather(tpi, S, Z, Y, X) - A).max() < 1e-15
+    S2, Z2, Y2, X2 = cqx.SpinScatter(tpi, A, zeroABBA=True, BBeqAA=True)
+    assert np.abs(S2 - 2 * aa).max() == 0.0 and not Z2.any() and not Y2.any() and not X2.any()
+    Ar = np.block([[aa.real, ab.real], [ba.real, bb.real]])
+    _, _, Yr, _ = cqx.SpinScatter(tpi, Ar)
+    assert np.abs(Yr - (ba.real - ab.real)).max() == 0.0
+    # the file's GHF density: gather then scatter returns its components
+    back = cqx.SpinScatter(tpi, cqx.SpinGather(tpi, *D))
+    for x, y in zip(back, D):
+        assert np.abs(x - y).max() < 1e-15
+
+
+def test_has_nan():
+    from chronusq_public_amd import cqx
+    d, mol, gbs, tpi, gto, D, H, F = setup("water_631gd_rhf")
+    P = cqx.PauliSpinorMatrices.from_components(D)
+    assert not P.hasNaN(tpi)
+    P.comps[0][3, 2] = np.nan
+    assert P.hasNaN(tpi)
+
+
+@pytest.mark.parametrize("form", ["short", "comm_screen_list_pert", "comm_list_pert", "comm_list"])
+def test_twobodycontract_reference_forms(form):
+    """The reference's twoBodyContract overloads (twopints.hpp:219-244) give
+    the same J and K as the oracle, and the perturbation is ignored."""
+    from chronusq_public_amd import cqx
+    d, mol, gbs, tpi, gto, D, H, F = setup("water_631gd_rhf")
+    nb = D[0].shape[0]
+    J, K = np.zeros((nb, nb), order="F"), np.zeros((nb, nb), order="F")
+    lst = [cqx.TwoBodyContraction(D[0], J, True, cqx.COULOMB), cqx.TwoBodyContraction(D[0], K, True, cqx.EXCHANGE)]
+    pert = cqx.EMPerturbation()
+    pert.addField("Electric", [0.1, 0.0, 0.0])
+    {"short": lambda: gto.twoBodyContract(lst),
+     "comm_screen_list_pert": lambda: gto.twoBodyContract(0, True, lst, pert),
+     "comm_list_pert": lambda: gto.twoBodyContract(0, lst, pert),
+     "comm_list": lambda: gto.twoBodyContract(0, lst)}[form]()
+    Jr, Kr = OracleBasis(gbs, mol.Z, mol.xyz).contract([D[0], D[0]], [0, 1], [True, True])
+    assert np.abs(J - Jr).max() < 1e-10 and np.abs(K - Kr).max() < 1e-10
+
+
+@pytest.mark.parametrize("opts", [{}, {"fp32_cut": 1e-4}, {"list_cap": 4096}, {"graph": 1},
+                                  {"autotune": 1}])
+def test_scratch_size_bounds_the_call(opts):
+    """cqx_scratch_size (directScaffoldNewSCRSize) bounds the workspace one
+    call allocates on a fresh context, for RHF and for 2C lists, under the
+    options that change the workspace."""
+    from chronusq_public_amd import cqx
+    from chronusq_public_amd.chem import basis_text, water_cluster
+    mol = water_cluster(4, 8.0, seed=0)
+    b = cqx.BasisSet(basis_text("cc-pVDZ"), mol)
+    nb = b.nBasis
+    rng = np.random.Generator(np.random.PCG64(3))
+    C = rng.normal(0, 1 / np.sqrt(nb), size=(nb, 20))
+    Dr = C @ C.T
+    Cc = C + 1j * rng.normal(0, 1 / np.sqrt(nb), size=(nb, 20))
+    Dc = Cc @ Cc.conj().T
+    for cplx, Ds, types in ((False, [Dr, Dr], [0, 1]), (True, [Dc] * 5, [0, 1, 1, 1, 1])):
+        tpi = cqx.DirectTPI(b, 1e-12)
+        for k, v in opts.items():
+            tpi.set_option(k, v)
+        gto = cqx.GTODirectTPIContraction(tpi)
+        bound = tpi.scratch_size(types, [True] * len(types), cplx)
+        dt = np.complex128 if cplx else np.float64
+        out = [np.zeros((nb, nb), dtype=dt, order="F") for _ in Ds]
+        gto.twoBodyContract([cqx.TwoBodyContraction(x.astype(dt), a, True, t) for x, a, t in zip(Ds, out, types)])
+        used = tpi.last_stats()["workspace_bytes"]
+        say(f"case: scratch {opts} cplx {cplx}: bound {bound} used {used}")
+        assert 0 < used <= bound
+        # and the formRawGDSCRSizePerBatch wrapper counts the same list in MatsT
+        ss = cqx.SingleSlaterState([Ds[0]] if not cplx else [Dc, Dc, Dc, Dc], gto)
+        assert cqx.FockBuilder.formRawGDSCRSizePerBatch(ss) * (16 if cplx else 8) >= bound
