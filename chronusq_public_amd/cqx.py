@@ -593,8 +593,10 @@ class FockBuilder:
 
     @staticmethod
     def formGD(ss, pert=None, increment=False, xHFX=1.0, HerDen=True):
-        """impl.hpp:75-102: G[D] (or, with increment, G[D] += G[deltaOnePDM],
-        K += K[deltaOnePDM] and coulombMatrix = J[deltaOnePDM])."""
+        """impl.hpp:75-102: G[D]; with increment, the reference's
+        accumulation on deltaOnePDM: coulombMatrix = J[delta],
+        K += K[delta] and G += 2 J[delta] - xHFX K, with K the accumulated
+        exchange matrix (impl.hpp:223-237)."""
         src = ss.deltaOnePDM if increment else ss.onePDM
         if src is None:
             raise CErr("formGD: increment needs deltaOnePDM")
