@@ -25,6 +25,9 @@ LIBPATH = os.path.join(HERE, "libcqx.so")
 
 COULOMB, EXCHANGE = 0, 1
 HOST, DEVICE = 0, 1
+# CQX_NOALLOC_SCRATCH_BYTES: the fixed per-GPU workspace region of the
+# allocation-free contraction
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
 class CErr(RuntimeError):
@@ -57,7 +60,7 @@ SYMBOLS = ["cqx_basis_create", "cqx_basis_from_g94", "cqx_basis_destroy",
            "cqx_scf_matrix", "cqx_scf_orbital_energies", "cqx_scf_destroy",
            "cqx_scf_last_error", "cqx_serve_default_opts", "cqx_serve", "cqx_grad_two_body_contract",
            "cqx_scratch_size", "cqx_spin_scatter", "cqx_spin_gather", "cqx_compute_energy",
-           "cqx_count_nonfinite"]
+           "cqx_count_nonfinite", "cqx_two_body_contract_noalloc"]
 
 OVERLAP, KINETIC, POTENTIAL = 0, 1, 2
 
@@ -94,6 +97,7 @@ def lib():
         L.cqx_basis_batch.argtypes = [I, P, C.POINTER(P)]
         L.cqx_grad_two_body_contract.argtypes = [P, I, I, P, I]
         L.cqx_scratch_size.argtypes = [P, I, P, I, C.POINTER(C.c_size_t)]
+        L.cqx_two_body_contract_noalloc.argtypes = [P, I, I, P, I]
         _lib = L
     return _lib
 
@@ -333,6 +337,24 @@ class GTODirectTPIContraction:
             arr[i].her, arr[i].type = int(c.HER), c.contType
         _check(lib().cqx_two_body_contract(self.tpi._h, int(screen), len(contractions),
                                            C.cast(arr, C.c_void_p), int(cplx), HOST),
+               self.tpi._h)
+
+    def twoBodyContractNoAlloc(self, contractions, screen=True):
+        """The same contraction through cqx_two_body_contract_noalloc: X and
+        AX are DeviceMatrix objects, AX is accumulated in place, and every
+        per-call buffer comes from the library's fixed scratch region
+        (no device allocation during the call)."""
+        if not contractions:
+            raise CErr("Nothing specified to compute")
+        if not all(hasattr(c.X, "data_ptr") and hasattr(c.AX, "data_ptr") for c in contractions):
+            raise CErr("the allocation-free contraction takes device matrices")
+        arr = (_Contraction * len(contractions))()
+        for i, c in enumerate(contractions):
+            arr[i].X, arr[i].AX = c.X.data_ptr(), c.AX.data_ptr()
+            arr[i].her, arr[i].type = int(c.HER), c.contType
+        _check(lib().cqx_two_body_contract_noalloc(self.tpi._h, int(screen), len(contractions),
+                                                   C.cast(arr, C.c_void_p),
+                                                   int(contractions[0].X.is_complex())),
                self.tpi._h)
 
 

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -30,6 +31,36 @@ const double kRysTable[] = {
 };
 
 thread_local std::string g_error;  // error of the last failing call of this thread
+
+}  // namespace
+
+// The workspace of cqx_two_body_contract_noalloc: one fixed region per GPU,
+// loaded with the code object and held until the process ends.  In host
+// code `scratch` names a host-side stand-in, never device memory: the
+// region's device address is looked up once per device (scratch_region)
+// and only that address is handed to kernels and copies.
+static __device__ __align__(256) char scratch[CQX_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, which must be the current
+// device; resolved on the first call per device (from cqx_ctx_create).
+void *scratch_region(int device) {
+  if (device < 0 || device >= kMaxDevices) throw std::runtime_error("device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+  }
+  return g_scratch_ptr[device];
+}
 
 }  // namespace
 
@@ -109,6 +140,7 @@ struct cqx_ctx {
   long long list_cap = 1ll << 28;  // quartet-list entries per batch (8 B each, x2 when mixed)
   int device = 0;
   hipStream_t stream = nullptr;
+  void *scratch = nullptr;  // this device's allocation-free workspace region
   std::string err;
 
   int ns = 0, nb = 0, nbc = 0;
@@ -717,6 +749,7 @@ int cqx_ctx_create(const cqx_basis *b, double thresh, int device, cqx_ctx **out)
     c->thresh = thresh;
     if (device >= 0) HIPCHK(hipSetDevice(device));
     HIPCHK(hipGetDevice(&c->device));
+    c->scratch = scratch_region(c->device);
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));
@@ -1755,6 +1788,57 @@ extern "C" {
 int cqx_two_body_contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int is_complex,
                           int mem) {
   return contract(c, screen, ncon, list, is_complex, mem, nullptr);
+}
+
+int cqx_two_body_contract_noalloc(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int is_complex) {
+  if (!c || ncon <= 0 || !list) return contract(c, screen, ncon, list, is_complex, CQX_DEVICE, nullptr);
+  if (!c->scratch) return fail(c, CQX_ERR_HIP, "cqx_two_body_contract_noalloc: no scratch region on this device");
+  if (ncon % c->nmol)
+    return fail(c, CQX_ERR_ARG, "a batched context takes the same number of contractions for every molecule");
+  const int nper = ncon / c->nmol;
+  // every group's fixed workspace (the pieces contract() carves before its
+  // quartet lists, 256-byte aligned) plus a reserve and one list entry
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    const int qmul = c->fp32_cut > 0.0 ? 2 : 1;
+    int start = 0;
+    while (start < nper) {
+      const int nmat = group_size(list + start, nper - start, is_complex, nullptr);
+      const GroupPlan g = plan_group(c, list + start, nmat, is_complex, c->stream);
+      size_t need = 0;
+      auto take = [&](size_t bytes) { need = ((need + 255) & ~(size_t)255) + std::max<size_t>(bytes, 1); };
+      take(g.x * sizeof(double));
+      take(nmat * sizeof(int));
+      take(g.bm * sizeof(double));
+      take(g.bg * sizeof(double));
+      take(c->nmol * sizeof(double));
+      for (int k = 0; k < kNumClass; ++k)
+        if (!c->cls[k].empty()) take(g.pn[k] * sizeof(double));
+      take(g.planes * sizeof(double));
+      take(g.planes * sizeof(double));
+      take(c->nmol * sizeof(cqx::MolPlanes));
+      take(g.cand * sizeof(long long));
+      take(g.cand * sizeof(int));
+      take(g.scan_bytes);
+      if (g.ncp) {
+        take(g.ncp * sizeof(long long));
+        take(g.ncp * sizeof(long long));
+      }
+      need += 4096 + (g.ncp + 64) * (2 * sizeof(cqx::ListLaunch) + 64) + 256 + qmul * sizeof(Quartet);
+      if (need > CQX_NOALLOC_SCRATCH_BYTES)
+        return fail(c, CQX_ERR_UNSUPPORTED,
+                    "cqx_two_body_contract_noalloc: the contractions need " + std::to_string(need) +
+                        " bytes of fixed workspace; the scratch region holds " +
+                        std::to_string((size_t)CQX_NOALLOC_SCRATCH_BYTES));
+      start += nmat;
+    }
+  } catch (const std::exception &e) {
+    return fail(c, CQX_ERR_HIP, e.what());
+  }
+  Arena a;
+  a.base = static_cast<char *>(c->scratch);
+  a.cap = CQX_NOALLOC_SCRATCH_BYTES;
+  return contract(c, screen, ncon, list, is_complex, CQX_DEVICE, &a);
 }
 
 int cqx_scratch_size(cqx_ctx *c, int ncon, const cqx_contraction *list, int is_complex, size_t *bytes) {

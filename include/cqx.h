@@ -152,6 +152,27 @@ int cqx_two_body_contract(cqx_ctx *ctx, int screen, int ncontract,
 int cqx_scratch_size(cqx_ctx *ctx, int ncontract, const cqx_contraction *list,
                      int is_complex, size_t *bytes);
 
+/* Allocation-free cqx_two_body_contract.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory during a call (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same contraction, screening and results as
+ * cqx_two_body_contract with mem = CQX_DEVICE: X and AX are device pointers
+ * and AX is accumulated in place (callers zero it).  Every per-call buffer
+ * comes from one fixed device region of CQX_NOALLOC_SCRATCH_BYTES per GPU,
+ * held by the library from the first context created on that GPU until the
+ * process ends; nothing is allocated, recorded or tuned.  A list whose
+ * fixed workspace (staged densities, shell-block norms, Cartesian planes
+ * and accumulators, candidate offsets) does not fit the region is rejected
+ * with CQX_ERR_UNSUPPORTED before any launch; the screened quartet lists
+ * are built in as many batches as the rest of the region requires.  All
+ * calls on one GPU share the region: calls that overlap in time on the same
+ * GPU (from any context or thread) race and must be serialised by the
+ * caller.  The call returns after its work on the context's stream is
+ * complete. */
+#define CQX_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int cqx_two_body_contract_noalloc(cqx_ctx *ctx, int screen, int ncontract,
+                                  const cqx_contraction *list, int is_complex);
+
 /* Contracted shell-quartet ERIs for nq quartets (s1,s2,s3,s4 arrays), in
  * the libint2 layout the reference digests (row-major ijkl, pure
  * functions where the shell is pure).  out receives the quartets back to
