@@ -823,7 +823,7 @@ class _Job(C.Structure):
 class ServeOpts(C.Structure):
     _fields_ = [("max_inflight", C.c_int), ("hbm_budget_bytes", C.c_double), ("device", C.c_int),
                 ("thresh", C.c_double), ("list_cap", C.c_longlong), ("scf", ScfOpts),
-                ("verbose", C.c_int), ("graph", C.c_int)]
+                ("verbose", C.c_int), ("graph", C.c_int), ("batch_fock", C.c_int)]
 
 
 class JobResult(C.Structure):
@@ -833,11 +833,15 @@ class JobResult(C.Structure):
 
 
 def serve(jobs, max_inflight=8, hbm_budget_bytes=200e9, fp32_cut=0.0, guess="SAD",
-          list_cap=1 << 25, device=0, rmsd_tol=1e-8, verbose=0, graph=0, matrices=False):
+          list_cap=1 << 25, device=0, rmsd_tol=1e-8, verbose=0, graph=0, matrices=False,
+          batch_fock=0, allow_refused=False):
     """Run SCF jobs [(basis name or G94 text, Molecule, arrival seconds)]
     through the serving loop; returns one dict per job (latency =
     t_done - t_arrival), with its final density "D" and Fock matrix "F"
-    (Pauli-scalar convention, NB x NB) when matrices is set."""
+    (Pauli-scalar convention, NB x NB) when matrices is set.  batch_fock:
+    the Fock builds of every job in flight go into one batched contraction
+    per round (cross-request batching).  allow_refused: jobs refused for
+    their footprint (status 3) are reported instead of raised."""
     L = _scf_lib()
     if not getattr(L, "_serve_typed", False):
         L.cqx_serve_default_opts.argtypes = [C.POINTER(ServeOpts)]
@@ -847,7 +851,7 @@ def serve(jobs, max_inflight=8, hbm_budget_bytes=200e9, fp32_cut=0.0, guess="SAD
     o = ServeOpts()
     L.cqx_serve_default_opts(C.byref(o))
     o.max_inflight, o.hbm_budget_bytes, o.device, o.list_cap = int(max_inflight), float(hbm_budget_bytes), int(device), int(list_cap)
-    o.verbose, o.graph = int(verbose), int(graph)
+    o.verbose, o.graph, o.batch_fock = int(verbose), int(graph), int(batch_fock)
     o.scf.fp32_cut, o.scf.guess, o.scf.rmsd_tol = fp32_cut, GUESS[guess.upper()], rmsd_tol
     arr = (_Job * len(jobs))()
     keep = []
@@ -873,8 +877,8 @@ def serve(jobs, max_inflight=8, hbm_budget_bytes=200e9, fp32_cut=0.0, guess="SAD
     out = [{k: getattr(r, k) for k, _ in JobResult._fields_} for r in res]
     for r, m in zip(out, mats):
         r["D"], r["F"] = m
-    if rc:
-        bad = [i for i, r in enumerate(out) if r["status"]]
+    bad = [i for i, r in enumerate(out) if r["status"] and not (allow_refused and r["status"] == 3)]
+    if bad:
         raise CErr(f"FATAL (cqx status {rc}): serving jobs {bad} failed")
     return out
 

@@ -443,6 +443,8 @@ void setup_device_maps(cqx_ctx *c) {
   c->d_mol_boff.upload(c->mol_boff, c->stream);
 }
 
+void upload_pairs(cqx_ctx *c);
+
 // Significant pairs, primitive pairs and GPU Schwarz bounds.
 void build_pairs(cqx_ctx *c) {
   const cqx::Basis &B = c->basis;
@@ -527,6 +529,13 @@ void build_pairs(cqx_ctx *c) {
       return x.mol != y.mol ? x.mol < y.mol : x.Q > y.Q;
     });
   }
+  upload_pairs(c);
+}
+
+// Device copies of the pair data (class arrays, screening records,
+// primitive pairs) and the molecule segments of every class; the class
+// arrays must be sorted by (molecule, Q descending).
+void upload_pairs(cqx_ctx *c) {
   // molecule segments of every class
   c->seg.assign(kNumClass, std::vector<int>(c->nmol + 1, 0));
   for (int k = 0; k < kNumClass; ++k) {
@@ -623,7 +632,86 @@ bool replay_graph(cqx_ctx *c, int ncon, const cqx_contraction *list, int is_comp
   return true;
 }
 
+// Everything of a new context but its pair data: basis, device, streams,
+// events, scratch region, Rys tables and the Cartesian maps.
+void init_ctx(cqx_ctx *c, const cqx::Basis &basis, double thresh, int device) {
+  c->basis = basis;
+  c->thresh = thresh;
+  if (device >= 0) HIPCHK(hipSetDevice(device));
+  HIPCHK(hipGetDevice(&c->device));
+  c->scratch = scratch_region(c->device);
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&c->ev0));
+  HIPCHK(hipEventCreate(&c->ev1));
+  HIPCHK(hipEventCreate(&c->ev_t0));
+  HIPCHK(hipEventCreate(&c->ev_t1));
+  HIPCHK(hipDeviceGetAttribute(&c->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+  c->ns = (int)c->basis.shells.size();
+  c->nb = c->basis.nbasis;
+  c->nbc = c->basis.ncart_total;
+  c->nmol = c->basis.nmol;
+  std::vector<double> rys(kRysTable, kRysTable + sizeof(kRysTable) / sizeof(double));
+  c->d_rys.upload(rys, c->stream);
+  std::vector<float> rys32(rys.begin(), rys.end());
+  c->d_rys32.upload(rys32, c->stream);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  setup_device_maps(c);
+}
+
 }  // namespace
+
+namespace cqx {
+
+// A batched context over the molecules of one-molecule contexts, built from
+// their shell pairs, primitive pairs and Schwarz bounds (no ERI is
+// evaluated): molecule m is parts[m].  The parts keep their own data.
+int ctx_merge(cqx_ctx *const *parts, int n, cqx_ctx **out) {
+  if (!parts || n <= 0 || !out) return fail(nullptr, CQX_ERR_ARG, "ctx_merge: bad arguments");
+  std::vector<const Basis *> bases;
+  for (int m = 0; m < n; ++m) {
+    if (!parts[m] || parts[m]->nmol != 1) return fail(nullptr, CQX_ERR_ARG, "ctx_merge: parts must be one-molecule contexts");
+    if (parts[m]->thresh != parts[0]->thresh || parts[m]->device != parts[0]->device ||
+        parts[m]->prim_thresh != parts[0]->prim_thresh)
+      return fail(nullptr, CQX_ERR_ARG, "ctx_merge: parts differ in device or thresholds");
+    bases.push_back(&parts[m]->basis);
+  }
+  auto c = std::make_unique<cqx_ctx>();
+  try {
+    init_ctx(c.get(), Basis::batch(bases), parts[0]->thresh, parts[0]->device);
+    c->prim_thresh = parts[0]->prim_thresh;
+    c->cls.assign(kNumClass, {});
+    int prim0 = 0;
+    for (int m = 0; m < n; ++m) {
+      const cqx_ctx *p = parts[m];
+      const int sh0 = c->basis.mol_sh0[m];
+      for (const auto &pr : p->ref_pairs) c->ref_pairs.emplace_back(pr.first + sh0, pr.second + sh0);
+      for (int k = 0; k < kNumClass; ++k)
+        for (PairRec r : p->cls[k]) {
+          r.a += sh0;
+          r.b += sh0;
+          r.pp_off += prim0;
+          r.mol = m;
+          c->cls[k].push_back(r);
+        }
+      c->prims.insert(c->prims.end(), p->prims.begin(), p->prims.end());
+      c->prim_alpha.insert(c->prim_alpha.end(), p->prim_alpha.begin(), p->prim_alpha.end());
+      prim0 = (int)c->prims.size();
+    }
+    for (int k = 0; k < kNumClass; ++k)
+      if (!ket_index_fits(c->cls[k].size()))
+        throw Unsupported("a shell-pair class holds " + std::to_string(c->cls[k].size()) +
+                          " pairs, beyond the 2^24 a quartet can index");
+    upload_pairs(c.get());
+  } catch (const Unsupported &e) {
+    return fail(nullptr, CQX_ERR_UNSUPPORTED, e.what());
+  } catch (const std::exception &e) {
+    return fail(nullptr, CQX_ERR_HIP, e.what());
+  }
+  *out = c.release();
+  return CQX_OK;
+}
+
+}  // namespace cqx
 
 extern "C" {
 
@@ -745,27 +833,7 @@ int cqx_ctx_create(const cqx_basis *b, double thresh, int device, cqx_ctx **out)
     return fail(nullptr, CQX_ERR_UNSUPPORTED, "device kernels cover shells up to l = 3");
   auto c = std::make_unique<cqx_ctx>();
   try {
-    c->basis = b->b;
-    c->thresh = thresh;
-    if (device >= 0) HIPCHK(hipSetDevice(device));
-    HIPCHK(hipGetDevice(&c->device));
-    c->scratch = scratch_region(c->device);
-    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    HIPCHK(hipEventCreate(&c->ev0));
-    HIPCHK(hipEventCreate(&c->ev1));
-    HIPCHK(hipEventCreate(&c->ev_t0));
-    HIPCHK(hipEventCreate(&c->ev_t1));
-    HIPCHK(hipDeviceGetAttribute(&c->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
-    c->ns = (int)c->basis.shells.size();
-    c->nb = c->basis.nbasis;
-    c->nbc = c->basis.ncart_total;
-    c->nmol = c->basis.nmol;
-    std::vector<double> rys(kRysTable, kRysTable + sizeof(kRysTable) / sizeof(double));
-    c->d_rys.upload(rys, c->stream);
-    std::vector<float> rys32(rys.begin(), rys.end());
-    c->d_rys32.upload(rys32, c->stream);
-    HIPCHK(hipStreamSynchronize(c->stream));
-    setup_device_maps(c.get());
+    init_ctx(c.get(), b->b, thresh, device);
     build_pairs(c.get());
   } catch (const Unsupported &e) {
     return fail(nullptr, CQX_ERR_UNSUPPORTED, e.what());

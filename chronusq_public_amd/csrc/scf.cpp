@@ -31,6 +31,7 @@
 #include "ctx_access.hpp"
 #include "kernels.hpp"
 #include "onebody.hpp"
+#include "scf_access.hpp"
 #include "scf_kernels.hpp"
 
 namespace {
@@ -127,6 +128,7 @@ struct cqx_scf {
   double nelec = 0.0;     // electrons, fractional mode
   Dev<double> S, H, X, D, Dp, J, K, G, F, Fx, W1, W2, W3, eps, ework, fscale, Fh, Eh;
   Dev<rocblas_int> info;
+  Dev<unsigned> nonfinite;  // partial counts of the Fock matrix's check
   std::vector<double> B;  // CDIIS error overlaps, keep x keep
   std::vector<double> eps_host;
   int iter = 0, nstored = 0;
@@ -152,15 +154,25 @@ struct cqx_scf {
     return r;
   }
 
-  // F_S = H_S + 2 J[D] - K[D] through the direct J/K engine
-  void fock() {
-    const double t0 = now_ms();
+  // J = J[D], K = K[D] through the direct J/K engine on this SCF's context
+  void fock_jk() {
     HIPOK(cqx::launch_zero(J.p, n2(), stream));
     HIPOK(cqx::launch_zero(K.p, n2(), stream));
     cqx_contraction c[2] = {{D.p, J.p, 1, CQX_COULOMB}, {D.p, K.p, 1, CQX_EXCHANGE}};
     CQXOK(ctx, cqx_two_body_contract(ctx, 1, 2, c, 0, CQX_DEVICE));
+  }
+
+  // G = 2 J - K and F_S = H_S + G; a Fock matrix with a non-finite entry
+  // stops the SCF (PauliSpinorMatrices::hasNaN, paulispinormatrices.hpp:383-388)
+  void fock_from_jk() {
     HIPOK(cqx::launch_scf_fock(H.p, J.p, K.p, G.p, F.p, n2(), stream));
-    ms_fock += now_ms() - t0;
+    HIPOK(cqx::launch_count_nonfinite(F.p, n2(), nonfinite.p, stream));
+    std::vector<unsigned> part(cqx::kNonfiniteBlocks);
+    HIPOK(hipMemcpyAsync(part.data(), nonfinite.p, sizeof(unsigned) * part.size(), hipMemcpyDeviceToHost, stream));
+    HIPOK(hipStreamSynchronize(stream));
+    size_t bad = 0;
+    for (unsigned v : part) bad += v;
+    if (bad) throw ScfError(CQX_ERR_ARG, "the Fock matrix has " + std::to_string(bad) + " non-finite entries");
   }
 
   // eigenvectors of Fm in the Lowdin basis -> C = X V in W3, eigenvalues
@@ -249,9 +261,19 @@ struct cqx_scf {
 
   // one iteration; returns true once converged
   bool iterate() {
-    const double t0 = now_ms();
     if (converged) return true;
-    fock();
+    const double t0 = now_ms();
+    fock_jk();
+    ms_fock += now_ms() - t0;
+    return after_jk(t0);
+  }
+
+  // the rest of an iteration once J and K hold J[D] and K[D] (t0: when the
+  // iteration started); returns true once converged
+  bool after_jk(double t0) {
+    const double tf = now_ms();
+    fock_from_jk();
+    ms_fock += now_ms() - tf;
     e1 = 0.5 * dot(D.p, H.p);
     e2 = 0.25 * dot(D.p, G.p);
     E = e1 + e2 + e_nuc;
@@ -288,6 +310,7 @@ struct cqx_scf {
     const size_t m = n2();
     for (auto *b : {&S, &H, &X, &D, &Dp, &J, &K, &G, &F, &Fx, &W1, &W2, &W3}) b->alloc(m);
     eps.alloc(nb);
+    nonfinite.alloc(cqx::kNonfiniteBlocks);
     ework.alloc(nb);
     fscale.alloc(nb);
     info.alloc(1);
@@ -544,3 +567,37 @@ int cqx_scf_destroy(cqx_scf *s) {
 const char *cqx_scf_last_error(const cqx_scf *s) { return s ? s->err.c_str() : g_scf_error.c_str(); }
 
 }  // extern "C"
+
+namespace cqx {
+
+int scf_fock_buffers(cqx_scf *s, const double **D, double **J, double **K, int *nb) {
+  if (!s || !D || !J || !K || !nb) return scf_fail(s, CQX_ERR_ARG, "scf_fock_buffers: bad arguments");
+  *D = s->D.p;
+  *J = s->J.p;
+  *K = s->K.p;
+  *nb = s->nb;
+  return CQX_OK;
+}
+
+int scf_after_jk(cqx_scf *s, double ms_jk, int *done) {
+  if (!s) return scf_fail(nullptr, CQX_ERR_ARG, "scf_after_jk: null handle");
+  try {
+    s->ms_fock += ms_jk;
+    const bool d = s->converged ? true : s->after_jk(now_ms() - ms_jk);
+    if (done) *done = d ? 1 : 0;
+    return CQX_OK;
+  } catch (const ScfError &e) {
+    return scf_fail(s, e.code, e.what());
+  } catch (const std::exception &e) {
+    return scf_fail(s, CQX_ERR_HIP, e.what());
+  }
+}
+
+int scf_iterations(const cqx_scf *s, int *iter, int *max_iter) {
+  if (!s) return scf_fail(nullptr, CQX_ERR_ARG, "scf_iterations: null handle");
+  if (iter) *iter = s->iter;
+  if (max_iter) *max_iter = s->o.max_iter;
+  return CQX_OK;
+}
+
+}  // namespace cqx

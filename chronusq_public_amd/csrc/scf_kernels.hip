@@ -35,7 +35,27 @@ __global__ void sub_kernel(const double *A, const double *B, double *C, size_t n
   if (i < n) C[i] = A[i] - B[i];
 }
 
+// per block: how many of the n doubles are not finite
+__global__ __launch_bounds__(256) void count_nonfinite_kernel(const double *a, size_t n, unsigned *part) {
+  unsigned c = 0;
+  for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256)
+    c += isfinite(a[k]) ? 0u : 1u;
+  __shared__ unsigned red[256];
+  red[threadIdx.x] = c;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
 }  // namespace
+
+hipError_t launch_count_nonfinite(const double *a, size_t n, unsigned *part, hipStream_t s) {
+  hipLaunchKernelGGL(count_nonfinite_kernel, dim3(kNonfiniteBlocks), dim3(256), 0, s, a, n, part);
+  return hipGetLastError();
+}
 
 hipError_t launch_scf_fock(const double *H, const double *J, const double *K, double *G, double *F,
                            size_t n, hipStream_t s) {

@@ -360,9 +360,17 @@ const char *cqx_scf_last_error(const cqx_scf *s);
  * process): SCF jobs arrive over time and run concurrently on one GPU.
  * The oldest waiting job is admitted as soon as fewer than max_inflight
  * jobs run and its footprint estimate fits hbm_budget_bytes beside the
- * running ones; each admitted job runs on its own context and stream and
- * keeps its state on the device until it converges.  Blocks until every
- * job has finished; returns the first non-zero job status, if any.
+ * running ones; a job whose footprint alone exceeds the budget is refused
+ * (status CQX_ERR_MEM).  Each admitted job sets up its own context and SCF
+ * and keeps its state on the device until it converges.  Its iterations
+ * run either on its own stream (batch_fock = 0: the jobs' Fock builds
+ * overlap through their streams) or in rounds (batch_fock = 1): every
+ * round builds the J and K of all jobs in flight in one batched
+ * contraction over a context merged from the jobs' pair data, then each
+ * job finishes its iteration on its own stream; jobs join at the next
+ * round after set-up and leave once converged.  A Fock matrix with a
+ * non-finite entry stops its job (CQX_ERR_ARG).  Blocks until every job
+ * has finished; returns the first non-zero job status, if any.
  * ------------------------------------------------------------------- */
 typedef struct {
   const char *g94;     /* basis set (G94 text) */
@@ -383,9 +391,12 @@ typedef struct {
   long long list_cap;      /* per-job quartet-list batch (see "list_cap"), default 2^25 */
   cqx_scf_opts scf;        /* per-job SCF options (nocc taken from the job) */
   int verbose;             /* 1: one line per finished job on stderr */
-  int graph;               /* 1: each job's context records its Fock build into a
-                              HIP graph (option "graph") and its later SCF
-                              iterations replay it; default 0 */
+  int graph;               /* 1: each job's context (batch_fock: each merged
+                              context) records its Fock build into a HIP graph
+                              (option "graph") and later iterations replay it;
+                              default 0 */
+  int batch_fock;          /* 1: cross-request batching of the Fock builds in
+                              rounds (see above); default 0 */
 } cqx_serve_opts;
 void cqx_serve_default_opts(cqx_serve_opts *o);
 

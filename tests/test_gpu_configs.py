@@ -106,7 +106,8 @@ def test_batched_full_size_mixed_equals_fp64_singles():
 
 @pytest.mark.timeout(900)
 @pytest.mark.parametrize("cut", [0.0, 1e-4])
-def test_served_jobs_against_oracle(cut):
+@pytest.mark.parametrize("batch_fock", [0, 1])
+def test_served_jobs_against_oracle(cut, batch_fock):
     """A Poisson mix of the serving workload's generator (cc-pVDZ water
     clusters, job i from seed 1000 + i; sizes U{4..14} here so that the
     oracle checks every job within the test's time) served with up to 4 jobs
@@ -118,9 +119,9 @@ def test_served_jobs_against_oracle(cut):
     sizes, mols, rng = serving_jobs(6, seed=0, nmin=4, nmax=14)
     arrivals = poisson_arrivals(rng, len(mols), 4.0)
     gbs = basis_text("cc-pVDZ")
-    say(f"case: served mix sizes {sizes}, fp32_cut {cut}")
+    say(f"case: served mix sizes {sizes}, fp32_cut {cut}, batch_fock {batch_fock}")
     res = cqx.serve([("cc-pVDZ", m, float(a)) for m, a in zip(mols, arrivals)], max_inflight=4,
-                    fp32_cut=cut, matrices=True)
+                    fp32_cut=cut, matrices=True, batch_fock=batch_fock)
     for i, (mol, r) in enumerate(zip(mols, res)):
         assert r["status"] == 0 and r["converged"], i
         b = cqx.BasisSet(gbs, mol)
