@@ -12,9 +12,14 @@ densities and outputs resident in HBM.
 
 Prints one JSON line.  Roofline: the ERI/digestion kernels' algorithmic
 FP64 flops (the Rys flop model over the quartets evaluated) divided by
-their device time from HIP events on the launch stream.  CPU baseline: the
-FP64 oracle (a C/OpenMP restatement of the reference scaffold) timed on one
-full build of the same workload on the host cores.
+their device time from HIP events on the launch stream; HBM traffic,
+executed FP64 FLOPs and memory-side atomics come from the committed
+counter profile of the same sources, when there is one.  CPU baseline:
+the FP64 oracle (a C/OpenMP restatement of the reference scaffold) on the
+host's physical cores (OMP_PROC_BIND=close, OMP_PLACES=cores), one warm-up
+and the median of five full builds of the same workload.  host_pointer:
+the same builds through host matrices (the ChronusQ adapter's call),
+PCIe copies included.
 
 The "serving" object reports SCF p50/p99 latency and jobs/s of the serving
 loop (see serving()).  The "mixed" object of the line is the
@@ -79,18 +84,65 @@ def densities(kind, nb, nocc=20):
     return [D, D], [0, 1], False
 
 
-def cpu_baseline(gbs, mol, mats, types, cplx, gpu_out):
-    from oracle.oracle import OracleBasis
-    threads = min(16, os.cpu_count() or 1)
-    ob = OracleBasis(gbs, mol.Z, mol.xyz)
-    ob.prepare()  # Schwarz, once per geometry (untimed, as on the GPU side)
-    t0 = time.perf_counter()
-    ref = ob.contract(mats, types, [True] * len(mats), nthreads=threads)
-    dt = time.perf_counter() - t0
-    err = max(float(np.abs(g - r).max()) for g, r in zip(gpu_out, ref))
-    return {"value": 1.0 / dt, "unit": "Fock builds/s", "cores": threads, "kind": "port",
-            "sample": "one full Fock build of the same workload (FP64 C/OpenMP oracle)",
-            "seconds": dt}, err
+def host_cpu():
+    """Logical CPUs this process may run on, the physical cores among them
+    (distinct package/core ids), and the CPU model name."""
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            with open(base + "physical_package_id") as f:
+                pkg = f.read().strip()
+            with open(base + "core_id") as f:
+                core = f.read().strip()
+            cores.add((pkg, core))
+        except OSError:
+            cores.add(("?", str(c)))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return len(cpus), len(cores), model
+
+
+def cpu_baseline(gbs, mol, mats, types, cplx, gpu_out, repeats=5):
+    """The FP64 oracle (a C/OpenMP restatement of the reference scaffold)
+    on the host's physical cores: a child process with OMP_PROC_BIND=close
+    and OMP_PLACES=cores, one warm-up build, then the median of `repeats`
+    builds of the same workload.  Threads: every physical core this process
+    may use, or OMP_NUM_THREADS where the environment sets it (the host's
+    share of CPUs on a shared machine)."""
+    import subprocess
+    import tempfile
+    nproc, phys, model = host_cpu()
+    env_threads = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(env_threads) if env_threads.isdigit() and int(env_threads) > 0 else phys
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    with tempfile.TemporaryDirectory() as tmp:
+        src, dst = os.path.join(tmp, "in.npz"), os.path.join(tmp, "out.npz")
+        np.savez(src, gbs=np.array(gbs), Z=np.asarray(mol.Z, dtype=np.float64),
+                 xyz=np.asarray(mol.xyz, dtype=np.float64), mats=np.stack(mats),
+                 types=np.array(types), hers=np.ones(len(mats), dtype=bool),
+                 threads=np.array(threads), repeats=np.array(repeats))
+        r = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", src, dst], cwd=ROOT, env=env,
+                           capture_output=True, text=True)
+        if r.returncode:
+            raise RuntimeError("cpu baseline failed:\n" + r.stderr[-2000:])
+        res = json.loads(r.stdout.strip().splitlines()[-1])
+        ref = list(np.load(dst)["out"])
+    err = max(float(np.abs(g - x).max()) for g, x in zip(gpu_out, ref))
+    return {"value": 1.0 / res["median_s"], "unit": "Fock builds/s", "cores": threads, "kind": "port",
+            "sample": ("full Fock builds of the same workload (FP64 C/OpenMP oracle): 1 warm-up, "
+                       "median of %d" % repeats),
+            "seconds_median": res["median_s"], "seconds_all": res["times_s"],
+            "nproc": nproc, "physical_cores": phys, "cpu_model": model,
+            "omp": {"OMP_NUM_THREADS": threads, "OMP_PROC_BIND": "close", "OMP_PLACES": "cores"}}, err
 
 
 class Build:
@@ -136,23 +188,21 @@ class Build:
         return [a.download() for a in self.dA]
 
 
-def traffic_per_build(workload):
-    """HBM bytes per build of the ERI launches measured by the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload (None when that
-    workload has no such measurement)."""
-    path = os.path.join(ROOT, "profiles", "eri_%s_traffic.json" % workload)
+def profile_counters(workload):
+    """Counter figures of this workload's ERI launches from the committed
+    rocprofv3 --pmc runs (profiles/eri_<workload>_counters.json): HBM
+    traffic per build from copy-calibrated FETCH_SIZE / WRITE_SIZE, the
+    executed FP64 FLOPs per build from SQ_INSTS_VALU_FLOPS_FP64 (calibrated
+    on a kernel of known FMA count) and the memory-side atomic requests.
+    Returned only when the file was measured on the sources being run (its
+    source digest equals this tree's), else None."""
+    from chronusq_public_amd.build import source_digest
+    path = os.path.join(ROOT, "profiles", "eri_%s_counters.json" % workload)
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f)["traffic_bytes_per_build"]
-
-
-def executed_flops_per_build(workload):
-    """Executed FP64 FLOPs per build of the ERI launches from the committed
-    rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 run of this workload, calibrated
-    against a kernel of known FMA count (profiles/eri_flops_counters.md);
-    None when that workload has no such measurement."""
-    return {"ghf16": 9.714e10}.get(workload)
+        d = json.load(f)
+    return d if d.get("source_digest") == source_digest() else None
 
 
 def progress(msg):
@@ -218,6 +268,11 @@ def serving(njobs, max_inflight=8, cut=1e-4, seed=0):
                     max_inflight=max_inflight, fp32_cut=cut, verbose=1)
     lat = np.array([r["t_done"] - r["t_arrival"] for r in res])
     dE = np.array([abs(r["e_total"] - q["e_total"]) for r, q in zip(res, ref)])
+    progress("serving: the same Poisson run with cross-request batching of the Fock builds")
+    resb = cqx.serve([("cc-pVDZ", m, float(a)) for m, a in zip(mols, arrivals)],
+                     max_inflight=max_inflight, fp32_cut=cut, verbose=1, batch_fock=1)
+    latb = np.array([r["t_done"] - r["t_arrival"] for r in resb])
+    dEb = np.array([abs(r["e_total"] - q["e_total"]) for r, q in zip(resb, ref)])
     return {
         "workload": ("RHF SCF jobs, (H2O)_n n ~ U{4..48} (seed %d), cc-pVDZ (NB 96-1152), SAD guess, "
                      "CDIIS(8), RMS dD < 1e-8; Poisson arrivals at 0.8 x closed-loop peak; "
@@ -232,6 +287,16 @@ def serving(njobs, max_inflight=8, cut=1e-4, seed=0):
         "max_abs_dE_vs_fp64": float(dE.max()),
         "iterations_mean": float(np.mean([r["iterations"] for r in res])),
         "all_converged": all(r["converged"] for r in res) and all(r["converged"] for r in ref),
+        "cross_request_batching": {
+            "note": ("the same arrivals and jobs; every round builds the Fock matrices of all jobs in "
+                     "flight in one batched contraction"),
+            "jobs_per_s": njobs / max(r["t_done"] for r in resb),
+            "p50_latency_s": float(np.percentile(latb, 50)),
+            "p99_latency_s": float(np.percentile(latb, 99)),
+            "max_abs_dE_vs_fp64": float(dEb.max()),
+            "iterations_mean": float(np.mean([r["iterations"] for r in resb])),
+            "all_converged": all(r["converged"] for r in resb),
+        },
     }
 
 
@@ -372,6 +437,33 @@ def batched_comparison(nmol, steps, cut=1e-4, seed=0):
     }
 
 
+def host_leg(bench, steps, out, line):
+    """The drop-in host-pointer path: X and AX in host memory (mem =
+    CQX_HOST), as the ChronusQ adapter calls the engine; PCIe copies
+    included, so never `value`."""
+    from chronusq_public_amd import cqx
+    progress(f"host-pointer builds (1 warmup + {steps} timed)")
+    mats, types, cplx = bench.mats, bench.types, bench.cplx
+    hmats = [np.asfortranarray(m, dtype=np.complex128 if cplx else np.float64) for m in mats]
+    hout = [np.zeros_like(m, order="F") for m in hmats]
+    hcont = [cqx.TwoBodyContraction(x, a, True, t) for x, a, t in zip(hmats, hout, types)]
+    for a in hout:
+        a[...] = 0
+    bench.gto.twoBodyContract(hcont)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for a in hout:
+            a[...] = 0
+        bench.gto.twoBodyContract(hcont)
+    hdt = time.perf_counter() - t0
+    line["host_pointer"] = {
+        "builds_per_s": steps / hdt,
+        "ms_per_step": 1e3 * hdt / steps,
+        "bytes_per_build_over_pcie": int(2 * sum(m.nbytes for m in hmats)),
+        "max_abs_d_vs_device": float(max(np.abs(a - b).max() for a, b in zip(hout, out))),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -390,6 +482,8 @@ def main():
     ap.add_argument("--streams", type=int, default=4,
                     help="streams the ERI launches of a build are dealt over, 1-4 (Fock-build "
                          "contexts of the main and mixed-precision legs)")
+    ap.add_argument("--no-graph", action="store_true", help="skip the recorded-build (HIP graph) leg")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-pointer leg")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM "
                          "counter calibration for rocprofv3 --pmc runs)")
@@ -449,26 +543,40 @@ def main():
             "candidates": st["n_candidates"],
         },
         "roofline": {
-            "bound": "mfma",
+            "bound": "fp64 valu",
             "compute": "fp64 vector ALU (MI355X fp64 vector and matrix peaks are both 78.6 TFLOP/s)",
             "kernel": "ERI + J/K digestion launches of one build (all angular-momentum classes)",
             "achieved": achieved,
             "peak": FP64_VECTOR_PEAK,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_VECTOR_PEAK,
-            "traffic": traffic_per_build(args.workload),
-            "traffic_note": ("HBM bytes per build of the ERI/digestion launches from rocprofv3 --pmc "
-                             "FETCH_SIZE and WRITE_SIZE runs of this workload, copy-calibrated "
-                             "(profiles/eri_ghf16_traffic.md); almost all of it is the memory-side "
-                             "FP64 atomic adds of the digestion"),
+            "traffic": None,
             "flops_per_build": flops_avg,
             "eri_ms_per_build": eri_avg,
         },
     }
+    pc = profile_counters(args.workload)
+    if pc:
+        r = line["roofline"]
+        r["traffic"] = pc["traffic_bytes_per_build"]
+        r["traffic_note"] = ("HBM bytes per build of the ERI/digestion launches, copy-calibrated "
+                             "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (%s)" % pc["summary"])
+        r["executed_flops_per_build"] = pc["executed_fp64_flops_per_build"]
+        r["executed_frac"] = pc["executed_fp64_flops_per_build"] / (eri_avg * 1e-3) / (FP64_VECTOR_PEAK * 1e12)
+        r["memory_side_atomics_per_build"] = pc["tcc_ea0_atomic_per_build"]
+        r["limiter"] = pc["limiter"]
+        r["profile_source_digest"] = pc["source_digest"]
+    else:
+        line["roofline"]["traffic_note"] = ("no rocprofv3 counter profile of these sources: "
+                                            "traffic and executed FLOPs omitted")
+    # the drop-in host-pointer path (X and AX in host memory, mem = CQX_HOST,
+    # as the ChronusQ adapter calls it): PCIe copies included, not `value`
+    if not args.no_host:
+        host_leg(bench, args.steps, out, line)
     # the same builds recorded once into a HIP graph and replayed (option
     # graph): the first build records, the timed ones replay; the outputs
     # are checked against the eager ones
-    if "graph" not in [o.split("=", 1)[0] for o in args.opt]:
+    if not args.no_graph and "graph" not in [o.split("=", 1)[0] for o in args.opt]:
         progress(f"{args.workload}: recorded build, 1 warmup + {args.steps} timed replays")
         tpi.set_option("graph", 1)
         gdt, gms, _, _ = bench.timed(args.steps, 1)
@@ -482,13 +590,6 @@ def main():
             "max_abs_d_vs_eager": float(max(np.abs(a - b).max() for a, b in zip(gout, out))),
         }
         tpi.set_option("graph", 0)
-    xf = executed_flops_per_build(args.workload)
-    if xf:
-        # the same time priced with the FP64 FLOPs the kernels execute
-        line["roofline"]["executed_flops_per_build"] = xf
-        line["roofline"]["executed_frac"] = xf / (eri_avg * 1e-3) / (FP64_VECTOR_PEAK * 1e12)
-        line["roofline"]["executed_note"] = ("rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes per build, "
-                                             "calibrated on a known-FMA kernel (profiles/eri_flops_counters.md)")
     if args.mixed_steps > 0:
         line["mixed"] = mixed_comparison(args.mixed_steps, 1)
     if args.batch_mols > 0:
@@ -496,7 +597,7 @@ def main():
     if args.serve_jobs > 0:
         line["serving"] = serving(args.serve_jobs)
     if not args.no_cpu:
-        progress("cpu baseline: one oracle build")
+        progress("cpu baseline: oracle builds on the host cores")
         cb, err = cpu_baseline(gbs, mol, mats, types, cplx, out)
         line["cpu_baseline"] = cb
         line["max_abs_dJK_vs_oracle"] = err

@@ -36,6 +36,28 @@ def write_rys_inc():
     return path
 
 
+# the sources of the ERI + digestion launches of a Fock build: the kernels,
+# their headers and the Rys table
+KERNEL_SOURCES = ("eri_", "digest", "kernels.", "device_types.hpp", "rys.hpp", "rys_table.bin", "cand.hip",
+                  "limits.hpp")
+
+
+def source_digest():
+    """First 16 hex digits of a SHA-256 over the sources of the ERI and
+    digestion kernels (file names included).  Kernel profiles record it,
+    so that a profiled figure is reported only beside the kernels it was
+    measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in sorted(os.listdir(CSRC)):
+        if not f.startswith(KERNEL_SOURCES) or f == "rys_table_data.inc":
+            continue
+        h.update(f.encode())
+        with open(os.path.join(CSRC, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def build(verbose=False, jobs=None):
     """Compile every source to an object in parallel, then link."""
     write_rys_inc()
