@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -231,6 +232,25 @@ struct cqx_ctx {
   DevBuf<cqx::ListLaunch> d_glists;
   DevBuf<long long> d_glstart, d_gidx, d_gchk;
 
+  // Tiles (option "tiles", default on): the lane-group class pairs of a
+  // build whose planes are all Hermitian go through tile lists instead of
+  // quartet lists.  Per class, its member array (pair indices grouped by
+  // molecule and first shell); per tile class pair, its sub-blocks, a range
+  // of d_sub.
+  int use_tiles = 0;
+  int tile_classes = 0xFF;  // lane-group classes (bit k: k-th of ss|ss, ps|ss, ps|ps, pp|ss, pp|ps, ds|ss, ds|ps, dp|ss)
+  int rys_lds = 1;  // lane-group kernels stage their Rys table section in LDS
+  DevBuf<int> d_members[kNumClass];
+  struct TileCP { int cb, ck; long long off, n; };
+  std::vector<TileCP> tile_cps;
+  DevBuf<cqx::SubBlock> d_sub;
+  long long nsub_total = 0;
+  DevBuf<cqx::Tile> d_tiles;                    // per build: the tile lists
+  DevBuf<unsigned long long> d_tcount;          // per tile class pair: tiles and quartets, FP64 / FP32
+  DevBuf<double> d_tpsum;                       // per tile class pair: sums of Ka Kb Kc Kd, FP64 / FP32
+  DevBuf<cqx::TileListLaunch> d_tll;
+  DevBuf<long long> d_tstart;
+
   cqx_stats stats{};
 };
 
@@ -444,6 +464,7 @@ void setup_device_maps(cqx_ctx *c) {
 }
 
 void upload_pairs(cqx_ctx *c);
+void build_tiles(cqx_ctx *c);
 
 // Significant pairs, primitive pairs and GPU Schwarz bounds.
 void build_pairs(cqx_ctx *c) {
@@ -567,7 +588,86 @@ void upload_pairs(cqx_ctx *c) {
     }
     c->d_scr[k].upload(scr, c->stream);
   }
+  build_tiles(c);
   HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+// Member arrays and sub-block tables of the tile path.  A class's pairs are
+// grouped by (molecule, first shell in kernel orientation), each group in
+// class order (Schwarz Q descending); a sub-block takes up to kTileDim
+// consecutive members of a bra group against up to kTileDim of a ket group of
+// the same molecule (bra class >= ket class; in one class, ket group <= bra
+// group, and in one group ket chunk <= bra chunk, the diagonal chunk keeping
+// i >= j).  Every pair of pairs of a tile class pair is thus in exactly one
+// sub-block, as in the quartet lists.
+void build_tiles(cqx_ctx *c) {
+  struct Group { int m0, n, mol; };
+  std::vector<std::vector<Group>> groups(kNumClass);
+  std::vector<std::vector<int>> mem_host(kNumClass);
+  for (int k = 0; k < kNumClass; ++k) {
+    const auto &v = c->cls[k];
+    auto &mem = mem_host[k];
+    mem.resize(v.size());
+    std::iota(mem.begin(), mem.end(), 0);
+    // by molecule, then first shell; class order (Q descending) within a
+    // group, so a chunk's first member has its largest Q
+    std::stable_sort(mem.begin(), mem.end(), [&](int x, int y) {
+      return v[x].mol != v[y].mol ? v[x].mol < v[y].mol : v[x].a < v[y].a;
+    });
+    for (size_t i = 0; i < mem.size();) {
+      size_t j = i + 1;
+      while (j < mem.size() && v[mem[j]].mol == v[mem[i]].mol && v[mem[j]].a == v[mem[i]].a) ++j;
+      groups[k].push_back({(int)i, (int)(j - i), v[mem[i]].mol});
+      i = j;
+    }
+    c->d_members[k].upload(mem, c->stream);
+  }
+  std::vector<cqx::SubBlock> sub;
+  c->tile_cps.clear();
+  for (int cb = 0; cb < kNumClass; ++cb)
+    for (int ck = 0; ck <= cb; ++ck) {
+      if (c->cls[cb].empty() || c->cls[ck].empty()) continue;
+      int la, lb, lc, ld;
+      class_ls(cb, la, lb);
+      class_ls(ck, lc, ld);
+      if (!cqx::eri_fast_available(la, lb, lc, ld)) continue;
+      cqx_ctx::TileCP tc{cb, ck, (long long)sub.size(), 0};
+      const auto &GB = groups[cb], &GK = groups[ck];
+      const auto &VB = c->cls[cb], &VK = c->cls[ck];
+      size_t y0 = 0;  // first ket group of the bra group's molecule
+      for (size_t x = 0; x < GB.size(); ++x) {
+        const int mol = GB[x].mol;
+        while (y0 < GK.size() && GK[y0].mol < mol) ++y0;
+        for (size_t y = y0; y < GK.size() && GK[y].mol == mol; ++y) {
+          if (cb == ck && y > x) break;
+          const Group &gx = GB[x], &gy = GK[y];
+          for (int u = 0; u * cqx::kTileDim < gx.n; ++u)
+            for (int w = 0; w * cqx::kTileDim < gy.n; ++w) {
+              const bool diag = cb == ck && x == y && u == w;
+              if (cb == ck && x == y && w > u) break;
+              cqx::SubBlock sb{};
+              sb.bra_m0 = gx.m0 + u * cqx::kTileDim;
+              sb.ket_m0 = gy.m0 + w * cqx::kTileDim;
+              sb.nb = (unsigned short)std::min(cqx::kTileDim, gx.n - u * cqx::kTileDim);
+              sb.nk = (unsigned short)std::min(cqx::kTileDim, gy.n - w * cqx::kTileDim);
+              sb.diag = diag ? 1 : 0;
+              sb.qq = VB[mem_host[cb][sb.bra_m0]].Q * VK[mem_host[ck][sb.ket_m0]].Q;
+              sub.push_back(sb);
+            }
+        }
+      }
+      tc.n = (long long)sub.size() - tc.off;
+      if (!tc.n) continue;
+      // padded to whole list-kernel chunks with empty sub-blocks
+      while (((long long)sub.size() - tc.off) % cqx::kTileChunk) {
+        cqx::SubBlock e{};
+        sub.push_back(e);
+      }
+      tc.n = (long long)sub.size() - tc.off;
+      c->tile_cps.push_back(tc);
+    }
+  c->nsub_total = (long long)sub.size();
+  c->d_sub.upload(sub, c->stream);
 }
 
 // Replay of the captured build for the densities of `list`: stage X and AX,
@@ -928,6 +1028,26 @@ int cqx_set_option(cqx_ctx *c, const char *name, double value) {
     if (value == 0.0) c->tuned_lanes.clear();
     return CQX_OK;
   }
+  if (!std::strcmp(name, "tiles")) {
+    // 1: lane-group class pairs of Hermitian builds through tile lists
+    // (shared J / K targets summed per tile); 0 (default): quartet lists
+    c->use_tiles = value != 0.0;
+    return CQX_OK;
+  }
+  if (!std::strcmp(name, "tile_classes")) {
+    // bit k set: the k-th lane-group class (ss|ss), (ps|ss), (ps|ps), (pp|ss),
+    // (pp|ps), (ds|ss), (ds|ps), (dp|ss) takes tiles when "tiles" is on
+    const int m = (int)value;
+    if (!(value == (double)m && m >= 0 && m <= 0xFF)) return fail(c, CQX_ERR_ARG, "tile_classes must be 0-255");
+    c->tile_classes = m;
+    return CQX_OK;
+  }
+  if (!std::strcmp(name, "rys_lds")) {
+    // 1 (default): the lane-group kernels read their Rys table section from
+    // an LDS copy per block; 0: from global memory
+    c->rys_lds = value != 0.0;
+    return CQX_OK;
+  }
   if (!std::strcmp(name, "streams")) {
     const int n = (int)value;
     if (!(value == (double)n && n >= 1 && n <= cqx_ctx::kMaxSide + 1))
@@ -1103,6 +1223,7 @@ size_t workspace_bytes(const cqx_ctx *c) {
   add(c->d_lstart); add(c->d_molmax); add(c->d_molplanes); add(c->d_quart); add(c->d_count);
   add(c->d_psum); add(c->d_tidx); add(c->d_tot); add(c->d_acc_tune); add(c->d_molplanes_tune);
   add(c->d_glists); add(c->d_glstart); add(c->d_gidx); add(c->d_gchk);
+  add(c->d_tiles); add(c->d_tcount); add(c->d_tpsum); add(c->d_tll); add(c->d_tstart);
   for (int k = 0; k < kNumClass; ++k) add(c->d_pn[k]);
   return b;
 }
@@ -1302,12 +1423,34 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
       // candidate lists per class pair (bra class >= ket class): bra i of
       // molecule m meets a prefix of molecule m's kets (Q descending); the
       // counts and offsets are computed on the device
+      // tile class pairs: the lane-group class pairs with a sub-block table,
+      // when every plane of the group is Hermitian and the build is neither
+      // recorded nor allocation-free; their quartets come from tile lists
+      const bool tiles_ok = c->use_tiles && !scr && !c->use_graph && !c->tile_cps.empty() &&
+                            gp.kcnt[cqx::kJNonHer] == 0 && gp.kcnt[cqx::kKNonHer] == 0;
+      std::vector<const cqx_ctx::TileCP *> tcps;
+      auto lane_group = [&](int cb, int ck) {
+        int la_, lb_, lc_, ld_;
+        class_ls(cb, la_, lb_);
+        class_ls(ck, lc_, ld_);
+        const int ncq = cqx::ncart(la_) * cqx::ncart(lb_) * cqx::ncart(lc_) * cqx::ncart(ld_);
+        const int idx = cqx::fast_class_index(la_, lb_, lc_, ld_);
+        return ncq <= c->fast_max_nc && idx >= 0 && ((c->tile_classes >> idx) & 1);
+      };
+      if (tiles_ok)
+        for (const auto &tc : c->tile_cps)
+          if (lane_group(tc.cb, tc.ck)) tcps.push_back(&tc);
+      auto is_tile_cp = [&](int cb, int ck) {
+        for (const auto *tc : tcps)
+          if (tc->cb == cb && tc->ck == ck) return true;
+        return false;
+      };
       struct CP { int cb, ck, nbra; long long ncand; size_t coff; std::vector<long long> off; };
       std::vector<CP> cps;
       size_t cand_total = 0;
       for (int cb = 0; cb < kNumClass; ++cb)
         for (int ck = 0; ck <= cb; ++ck) {
-          if (c->cls[cb].empty() || c->cls[ck].empty()) continue;
+          if (c->cls[cb].empty() || c->cls[ck].empty() || is_tile_cp(cb, ck)) continue;
           CP cp{cb, ck, (int)c->cls[cb].size(), 0, cand_total, {}};
           cand_total += (size_t)cp.nbra + 1;
           cps.push_back(std::move(cp));
@@ -1388,6 +1531,57 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
         }
       };
       front();
+      const bool mixed = c->fp32_cut > 0.0;
+      const int qmul = mixed ? 2 : 1;  // quartet lists per class pair
+      // tile lists of the tile class pairs: one launch over their sub-blocks;
+      // each sub-block gives at most one tile per precision
+      const int nt = (int)tcps.size();
+      std::vector<unsigned long long> tcount(3 * (size_t)nt);  // quartets FP64, FP32, candidates
+      std::vector<double> tpsum(2 * (size_t)nt);
+      std::vector<long long> tcap_off(nt + 1, 0);
+      std::vector<cqx::TileListLaunch> tll(nt);  // alive until the synchronisation below
+      cqx::Tile *tiles = nullptr;
+      if (nt) {
+        for (int k = 0; k < nt; ++k) tcap_off[k + 1] = tcap_off[k] + tcps[k]->n;
+        tiles = ws_take(c->d_tiles, nullptr, (size_t)tcap_off[nt] * qmul, "the tile lists");
+        unsigned long long *tcnt = ws_take(c->d_tcount, nullptr, 3 * (size_t)nt, "the tile counters");
+        double *tps = ws_take(c->d_tpsum, nullptr, 2 * (size_t)nt, "the tile counters");
+        for (int k = 0; k < nt; ++k) {
+          const auto &tc = *tcps[k];
+          cqx::TileListLaunch &T = tll[k];
+          T = cqx::TileListLaunch{};
+          T.sub = c->d_sub.p + tc.off;
+          T.bra = c->d_scr[tc.cb].p;
+          T.ket = c->d_scr[tc.ck].p;
+          T.bmem = c->d_members[tc.cb].p;
+          T.kmem = c->d_members[tc.ck].p;
+          T.bra_pn = w.pn[tc.cb];
+          T.ket_pn = w.pn[tc.ck];
+          T.Bg = w.Bg;
+          T.Bm = w.Bm;
+          T.mol_boff = c->d_mol_boff.p;
+          T.mol_ns = c->d_mol_ns.p;
+          T.molmax = w.molmax;
+          T.nmat = nmat;
+          T.thresh = c->thresh;
+          T.screen = screen;
+          T.out = tiles + tcap_off[k];
+          T.nquart = tcnt + 3 * k;
+          T.ncand = tcnt + 3 * k + 2;
+          T.prim_sum = tps + 2 * k;
+          T.cut32 = mixed ? c->fp32_cut : 0.0;
+          T.out32 = mixed ? tiles + tcap_off[nt] + tcap_off[k] : nullptr;
+          T.nquart32 = tcnt + 3 * k + 1;
+          T.prim_sum32 = tps + 2 * k + 1;
+        }
+        c->d_tll.upload(tll, s);
+        c->d_tstart.upload(tcap_off, s);
+        HIPCHK(cqx::launch_zero(reinterpret_cast<double *>(tcnt), 3 * (size_t)nt, s));
+        HIPCHK(cqx::launch_zero(tps, 2 * (size_t)nt, s));
+        HIPCHK(cqx::launch_tile_lists(c->d_tll.p, c->d_tstart.p, nt, tcap_off[nt], s));
+        HIPCHK(hipMemcpyAsync(tcount.data(), tcnt, sizeof(unsigned long long) * tcount.size(), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(tpsum.data(), tps, sizeof(double) * tpsum.size(), hipMemcpyDeviceToHost, s));
+      }
       // every class pair's candidate total (the last entry of its offsets),
       // gathered on the device and read back in one copy
       std::vector<long long> totals(cps.size());
@@ -1396,8 +1590,6 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
         HIPCHK(hipMemcpyAsync(totals.data(), w.tot, sizeof(long long) * cps.size(), hipMemcpyDeviceToHost, s));
       }
       HIPCHK(hipStreamSynchronize(s));
-      const bool mixed = c->fp32_cut > 0.0;
-      const int qmul = mixed ? 2 : 1;  // quartet lists per class pair
       // entries of one list batch: the option, or what the scratch region
       // leaves after this group's fixed workspace (less a reserve for the
       // per-batch tables)
@@ -1420,6 +1612,7 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
         }
       }
       c->stats.n_candidates += total;
+      for (int k = 0; k < nt; ++k) c->stats.n_candidates += (long long)tcount[3 * k + 2];
       // kernel family of a class: 0 lane-group, 1 wave-per-quartet, 2 generic
       auto route = [&](int la, int lb, int lc, int ld) {
         const int ncq = cqx::ncart(la) * cqx::ncart(lb) * cqx::ncart(lc) * cqx::ncart(ld);
@@ -1440,6 +1633,7 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
         E.rys32 = c->d_rys32.p;
         E.fp32 = f32 ? 1 : 0;
         E.force_g = c->fast_lanes;
+        E.rys_lds = c->rys_lds;
         E.planes = ps;
         E.store_mode = 0;
         return E;
@@ -1543,6 +1737,102 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
       bool tune_ready = false;
       const bool tuning = c->autotune_pending && !scr;  // the allocation-free path never times
       cqx::PlaneSet ps_tune = ps;
+      // scratch accumulators for the timing launches of the autotuner
+      auto prepare_tune = [&]() {
+        if (tune_ready) return;
+        c->d_acc_tune.ensure(std::max<size_t>(poff[nmol], 1));
+        std::vector<cqx::MolPlanes> mt = mp;
+        for (int m = 0; m < nmol; ++m)
+          for (int k = 0; k < 4; ++k) mt[m].ka[k] = c->d_acc_tune.p + (mp[m].ka[k] - w.acc);
+        c->d_molplanes_tune.upload(mt, s);
+        HIPCHK(hipStreamSynchronize(s));  // mt is uploaded from this frame
+        ps_tune = ps;
+        ps_tune.mols = c->d_molplanes_tune.p;
+        ps_tune.m0 = mt[0];
+        tune_ready = true;
+      };
+      // time a launch's shapes on scratch accumulators (results discarded)
+      // and return the fastest
+      auto time_shapes = [&](const std::vector<int> &shapes, const std::function<void(int)> &launch) {
+        float best = 1e30f;
+        int best_g = shapes.empty() ? 0 : shapes[0];
+        for (int g : shapes) {
+          float ms = 0.0f;
+          for (int rep = 0; rep < 2; ++rep) {  // the second run is timed
+            HIPCHK(hipEventRecord(c->ev_t0, s));
+            launch(g);
+            HIPCHK(hipEventRecord(c->ev_t1, s));
+            HIPCHK(hipEventSynchronize(c->ev_t1));
+            HIPCHK(hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1));
+          }
+          if (ms < best) {
+            best = ms;
+            best_g = g;
+          }
+        }
+        return best_g;
+      };
+      // ERI + digestion of the tile lists: launched with the first batch's
+      // ERI launches (or alone when no quartet list remains), each over its
+      // list's capacity (empty tiles exit at once)
+      const int nst_all = (tuning || scr) ? 1 : c->nstreams;
+      bool tiles_done = nt == 0;
+      auto launch_tiles = [&](int nst, int &nl) {
+        tiles_done = true;
+        for (int f = 0; f < qmul; ++f)
+          for (int k = 0; k < nt; ++k) {
+            const unsigned long long nq = tcount[3 * k + f];
+            if (!nq) continue;
+            const auto &tc = *tcps[k];
+            cqx::TileEriLaunch T{};
+            class_ls(tc.cb, T.la, T.lb);
+            class_ls(tc.ck, T.lc, T.ld);
+            T.tiles = tiles + (f ? tcap_off[nt] : 0) + tcap_off[k];
+            T.ntile = tc.n;
+            T.bmem = c->d_members[tc.cb].p;
+            T.kmem = c->d_members[tc.ck].p;
+            T.bra = c->d_pairs[tc.cb].p;
+            T.ket = c->d_pairs[tc.ck].p;
+            T.prims = c->d_prims.p;
+            T.rys = c->d_rys.p;
+            T.rys32 = c->d_rys32.p;
+            T.fp32 = f;
+            T.mean_prims = tpsum[2 * k + f] / (double)nq;
+            T.force_g = c->fast_lanes;
+            T.rys_lds = c->rys_lds;
+            T.planes = ps;
+            if ((T.ntile + 3) / 4 >= (1ll << 31)) throw Unsupported("a tile launch beyond 2^31 blocks");
+            const int key = (((((T.la * 4 + T.lb) * 4 + T.lc) * 4 + T.ld) << 1) | f) + 8192;
+            if (!c->fast_lanes) {
+              if (tuning && !c->tuned_lanes.count(key)) {
+                prepare_tune();
+                const int nc = cqx::ncart(T.la) * cqx::ncart(T.lb) * cqx::ncart(T.lc) * cqx::ncart(T.ld);
+                std::vector<int> shapes;
+                for (int g : {1, 2, 4, 8, 16})
+                  if (!(g == 1 && nc > 9)) shapes.push_back(g);
+                cqx::TileEriLaunch TT = T;
+                TT.planes = ps_tune;
+                c->tuned_lanes[key] = time_shapes(shapes, [&](int g) {
+                  TT.force_g = g;
+                  HIPCHK(cqx::launch_eri_tile(TT, s));
+                });
+              }
+              auto it = c->tuned_lanes.find(key);
+              if (it != c->tuned_lanes.end()) T.force_g = it->second;
+            }
+            HIPCHK(cqx::launch_eri_tile(T, stream_of(nl++, nst)));
+            const double fl = class_pair_flops(T.la, T.lb, T.lc, T.ld, pure_d, tpsum[2 * k + f], (double)nq,
+                                               is_complex ? 2 : 1, nJ, nK);
+            c->stats.n_quartets += (long long)nq;
+            c->stats.n_prim_quartets_max += (long long)tpsum[2 * k + f];
+            c->stats.flops_model += fl;
+            if (f) {
+              c->stats.n_quartets_f32 += (long long)nq;
+              c->stats.flops_model_f32 += fl;
+            }
+            c->stats.n_launch_eri++;
+          }
+      };
       for (const auto &batch : batches) {
         const size_t np = batch.size();
         if (!np) continue;
@@ -1599,13 +1889,14 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
         HIPCHK(hipMemcpyAsync(psum.data(), w.psum, sizeof(double) * nslot, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         // ERI + digestion: one launch per class pair and precision for every
-        // molecule of the batch
+        // molecule of the batch (and, with the first batch, the tile lists')
         HIPCHK(hipEventRecord(c->ev0, s));
         // autotuning times launches one by one on the context stream
-        const int nst = (tuning || scr) ? 1 : c->nstreams;
+        const int nst = nst_all;
         ensure_side(nst);
         fork_side(nst);
         int nlaunched = 0;
+        if (!tiles_done) launch_tiles(nst, nlaunched);
         for (size_t kk = 0; kk < nslot; ++kk) {
           if (!counts[kk]) continue;
           const size_t k = kk % np;
@@ -1623,17 +1914,7 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
             const int key = tune_key(E);
             if (tuning && !(rt == 0 && c->fast_lanes) && !c->tuned_lanes.count(key)) {
               // time each shape on scratch accumulators (results discarded)
-              if (!tune_ready) {
-                c->d_acc_tune.ensure(std::max<size_t>(poff[nmol], 1));
-                std::vector<cqx::MolPlanes> mt = mp;
-                for (int m = 0; m < nmol; ++m)
-                  for (int k = 0; k < 4; ++k) mt[m].ka[k] = c->d_acc_tune.p + (mp[m].ka[k] - w.acc);
-                c->d_molplanes_tune.upload(mt, s);
-                ps_tune = ps;
-                ps_tune.mols = c->d_molplanes_tune.p;
-                ps_tune.m0 = mt[0];
-                tune_ready = true;
-              }
+              prepare_tune();
               cqx::EriLaunch T = E;
               T.planes = ps_tune;
               const int nc = cqx::ncart(E.la) * cqx::ncart(E.lb) * cqx::ncart(E.lc) * cqx::ncart(E.ld);
@@ -1645,24 +1926,10 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
                 for (int qt : {1, 2, 4, 8})
                   if (cqx::eri_wave_qt_built(E.la, E.lb, E.lc, E.ld, E.fp32, qt)) shapes.push_back(qt);
               }
-              float best = 1e30f;
-              int best_g = 0;
-              for (int g : shapes) {
+              c->tuned_lanes[key] = time_shapes(shapes, [&](int g) {
                 (rt == 0 ? T.force_g : T.force_qt) = g;
-                float ms = 0.0f;
-                for (int rep = 0; rep < 2; ++rep) {  // the second run is timed
-                  HIPCHK(hipEventRecord(c->ev_t0, s));
-                  launch_eri(T, s);
-                  HIPCHK(hipEventRecord(c->ev_t1, s));
-                  HIPCHK(hipEventSynchronize(c->ev_t1));
-                  HIPCHK(hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1));
-                }
-                if (ms < best) {
-                  best = ms;
-                  best_g = g;
-                }
-              }
-              c->tuned_lanes[key] = best_g;
+                launch_eri(T, s);
+              });
             }
             apply_tuned(E);
           }
@@ -1679,6 +1946,19 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
           c->stats.n_launch_eri++;
         }
         join_side(nst);
+        HIPCHK(hipEventRecord(c->ev1, s));
+        HIPCHK(hipEventSynchronize(c->ev1));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        eri_ms_total += ms;
+      }
+      if (!tiles_done) {  // no quartet-list batch: the tile lists' launches alone
+        HIPCHK(hipEventRecord(c->ev0, s));
+        ensure_side(nst_all);
+        fork_side(nst_all);
+        int nl = 0;
+        launch_tiles(nst_all, nl);
+        join_side(nst_all);
         HIPCHK(hipEventRecord(c->ev1, s));
         HIPCHK(hipEventSynchronize(c->ev1));
         float ms = 0.0f;
@@ -1958,6 +2238,24 @@ int cqx_scratch_size(cqx_ctx *c, int ncon, const cqx_contraction *list, int is_c
     b += npiece * (2 * sizeof(unsigned long long) + 2 * sizeof(double) + sizeof(cqx::ListLaunch) +
                    sizeof(long long)) + sizeof(long long);
     if (c->use_graph) b += ncp * (sizeof(cqx::ListLaunch) + 4 * sizeof(long long)) + sizeof(long long);
+    // tile lists: at most one tile per sub-block and precision, when the
+    // build takes the tile path (Hermitian contractions, not recorded)
+    bool her_all = true;
+    for (int i = 0; i < ncon; ++i) her_all = her_all && list[i].her;
+    if (c->use_tiles && !c->use_graph && her_all) {
+      size_t nsub = 0, nt = 0;
+      for (const auto &tc : c->tile_cps) {
+        int la, lb, lc, ld;
+        class_ls(tc.cb, la, lb);
+        class_ls(tc.ck, lc, ld);
+        if (cqx::ncart(la) * cqx::ncart(lb) * cqx::ncart(lc) * cqx::ncart(ld) > c->fast_max_nc) continue;
+        nsub += (size_t)tc.n;
+        ++nt;
+      }
+      b += nsub * qmul * sizeof(cqx::Tile) + nt * (3 * sizeof(unsigned long long) + 2 * sizeof(double) +
+                                                   sizeof(cqx::TileListLaunch) + sizeof(long long)) +
+           sizeof(long long);
+    }
     *bytes = b;
   } catch (const std::exception &e) {
     return fail(c, CQX_ERR_HIP, e.what());

@@ -79,6 +79,35 @@ struct PlaneSet {
   unsigned negs;             // bit s: the slot's K coefficient is negated (conj of Im X)
 };
 
+// Tiles of quartets that share shells.  The pairs of a class are grouped by
+// (molecule, first shell in kernel orientation); a class's member array
+// lists its pair indices group by group.  A sub-block is up to kTileDim
+// consecutive members of one bra group against up to kTileDim consecutive
+// members of one ket group, so every quartet of it shares the bra's first
+// shell a and the ket's first shell c; its surviving candidates make a Tile.
+constexpr int kTileDim = 8;
+// a tile class pair's sub-blocks are padded to a multiple of this many (empty
+// sub-blocks with nb = nk = 0), so that one block of the list kernel takes a
+// chunk of one list
+constexpr int kTileChunk = 64;
+struct SubBlock {
+  int bra_m0, ket_m0;     // first bra / ket member of the chunk
+  unsigned short nb, nk;  // chunk sizes, <= kTileDim
+  int diag;               // same class, group and chunk on both sides: only i >= j
+  double qq;              // max Q of the bra chunk x max Q of the ket chunk
+};
+// Candidate (i, j) = bra member bra_m0 + i against ket member ket_m0 + j is
+// bit i kTileDim + j of bits; mask[i kTileDim + j] holds the contractions
+// that passed their separated screen.
+struct Tile {
+  int bra_m0, ket_m0;
+  unsigned short nb, nk;
+  unsigned pad;
+  unsigned long long bits;
+  unsigned char mask[kTileDim * kTileDim];
+  unsigned long long pad2;
+};
+
 // A quartet: bra / ket pair indices within their class arrays plus the
 // mask of contractions whose separated screen it passed.
 struct Quartet {

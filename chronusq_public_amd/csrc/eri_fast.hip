@@ -19,13 +19,15 @@ hipError_t launch_fast_r(const EriLaunch &a, int G, hipStream_t s) {
   }
 }
 
-bool eri_fast_available(int la, int lb, int lc, int ld) {
+int fast_class_index(int la, int lb, int lc, int ld) {
   const int q[8][4] = {{0, 0, 0, 0}, {1, 0, 0, 0}, {1, 0, 1, 0}, {1, 1, 0, 0},
                        {1, 1, 1, 0}, {2, 0, 0, 0}, {2, 0, 1, 0}, {2, 1, 0, 0}};
-  for (auto &c : q)
-    if (c[0] == la && c[1] == lb && c[2] == lc && c[3] == ld) return true;
-  return false;
+  for (int k = 0; k < 8; ++k)
+    if (q[k][0] == la && q[k][1] == lb && q[k][2] == lc && q[k][3] == ld) return k;
+  return -1;
 }
+
+bool eri_fast_available(int la, int lb, int lc, int ld) { return fast_class_index(la, lb, lc, ld) >= 0; }
 
 hipError_t launch_eri_fast(const EriLaunch &a, hipStream_t s) {
   if (a.nq <= 0) return hipSuccess;

@@ -150,43 +150,32 @@ __device__ __forceinline__ PrimRecip<R> prim_recip(R ap, R bq, double kk) {
 }
 
 // Rys table of the evaluation precision
-template <class R> __device__ __forceinline__ const R *rys_table(const EriLaunch &a);
-template <> __device__ __forceinline__ const double *rys_table<double>(const EriLaunch &a) { return a.rys; }
-template <> __device__ __forceinline__ const float *rys_table<float>(const EriLaunch &a) { return a.rys32; }
+template <class R> __device__ __forceinline__ const R *rys_table(const double *r64, const float *r32);
+template <> __device__ __forceinline__ const double *rys_table<double>(const double *r64, const float *) { return r64; }
+template <> __device__ __forceinline__ const float *rys_table<float>(const double *, const float *r32) { return r32; }
 
-// G lanes per quartet: the lanes split the primitive quartets of the shell
-// quartet and combine the contracted Cartesian integrals with a butterfly of
-// lane shuffles; the wave's 64 / G quartets then go to LDS and all 64 lanes
-// digest them (digest_tile.hpp), so the registers of the evaluation and of
-// the digestion are never live together.
-// R is the evaluation precision (Rys nodes, 2D integrals, the contracted
-// quartet); the digestion and the accumulators are FP64 either way.
+// The Rys table section of the class's root count: staged in LDS by the
+// whole block when the launch asks for it (the lane-group classes need at
+// most 2 roots: 9 KB / 20 KB of FP64 coefficients), else read from global
+// memory.  Every thread of the block must call this before any returns.
+template <int NR, class R>
+__device__ __forceinline__ const R *rys_section(const double *r64, const float *r32, int lds, R *s_tab) {
+  const R *g = rys_table<R>(r64, r32) + rys_tab_off(NR);
+  if (!lds) return g;
+  for (int e = threadIdx.x; e < rys_tab_len(NR); e += blockDim.x) s_tab[e] = g[e];
+  __syncthreads();
+  return s_tab;
+}
+
+// The contracted Cartesian quartet (bp | kp) by the G lanes of a group:
+// lane g takes primitive quartets g, g + G, ... and the group's sum is
+// combined by a butterfly of lane shuffles, so every lane of the group ends
+// with the whole quartet in I (unscaled).  R is the evaluation precision.
 template <int LA, int LB, int LC, int LD, int G, class R>
-__global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
+__device__ __forceinline__ void eval_quartet(const PrimPair *__restrict__ prims, const R *__restrict__ rysn,
+                                             const PairRec &bp, const PairRec &kp, int g,
+                                             R (&I)[Cls<LA, LB, LC, LD>::NC]) {
   using K = Cls<LA, LB, LC, LD>;
-  constexpr int QPW = 64 / G;  // quartets per wave
-  __shared__ double s_I[4][QPW * K::NC];
-  __shared__ int s_bf[4][QPW][4];
-  __shared__ unsigned s_mask[4][QPW];
-  __shared__ int s_mol[4][QPW];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int g = lane & (G - 1), qw = lane / G;
-  // quartet count: the list's device counter for a launch sized to the
-  // list's capacity before the list was built
-  const long long nq = a.nq_dev ? min((long long)*a.nq_dev - a.nq_base, a.nq) : a.nq;
-  const long long q0 = ((long long)blockIdx.x * 4 + wv) * QPW;
-  if (q0 >= nq) return;  // uniform across the wave; no block barriers below
-  const long long q = q0 + qw;
-  const bool valid = q < nq;
-  Quartet qt{0u, 0u};
-  PairRec bp{}, kp{};
-  if (valid) {
-    qt = a.quartets[q];
-    bp = a.bra[qt.bra];
-    kp = a.ket[qt.ket_mask & 0xFFFFFFu];
-  }
-  const R *__restrict__ rys = rys_table<R>(a);
-  R I[K::NC];
 #pragma unroll
   for (int o = 0; o < K::NC; ++o) I[o] = 0;
   const int nk = kp.npp, ntot = bp.npp * kp.npp;
@@ -196,8 +185,8 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
     ++i;
   }
   for (int k = g; k < ntot; k += G) {
-    const PrimPair A = a.prims[bp.pp_off + i];
-    const PrimPair B = a.prims[kp.pp_off + j];
+    const PrimPair A = prims[bp.pp_off + i];
+    const PrimPair B = prims[kp.pp_off + j];
     j += G;
     while (j >= nk) {
       j -= nk;
@@ -211,7 +200,7 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
     const R PQ[3] = {(R)(A.P[0] - B.P[0]), (R)(A.P[1] - B.P[1]), (R)(A.P[2] - B.P[2])};
     const R T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
     R x[K::NR], w[K::NR];
-    rys_roots<K::NR, R>(T, rys, x, w);
+    rys_roots_sec<K::NR, R>(T, rysn, x, w);
     if constexpr (K::NC == 1) {
       R s = 0;
 #pragma unroll
@@ -245,9 +234,51 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
 #pragma unroll
       for (int o = 0; o < K::NC; ++o) I[o] += __shfl_xor(I[o], off);
   }
-  // degeneracy (direct.hpp:1664-1667, 1793-1802) and the 1/2 of :1849-1850
+}
+
+// degeneracy of a quartet in the 8-fold loop (direct.hpp:1664-1667,
+// 1793-1802) and the 1/2 of :1849-1850
+__device__ __forceinline__ double quartet_scale(const PairRec &bp, const PairRec &kp) {
   const bool same_pair = (bp.a == kp.a && bp.b == kp.b);
-  const double scale = 0.5 * (bp.a == bp.b ? 1.0 : 2.0) * (kp.a == kp.b ? 1.0 : 2.0) * (same_pair ? 1.0 : 2.0);
+  return 0.5 * (bp.a == bp.b ? 1.0 : 2.0) * (kp.a == kp.b ? 1.0 : 2.0) * (same_pair ? 1.0 : 2.0);
+}
+
+// G lanes per quartet: the lanes split the primitive quartets of the shell
+// quartet and combine the contracted Cartesian integrals with a butterfly of
+// lane shuffles; the wave's 64 / G quartets then go to LDS and all 64 lanes
+// digest them (digest_tile.hpp), so the registers of the evaluation and of
+// the digestion are never live together.
+// R is the evaluation precision (Rys nodes, 2D integrals, the contracted
+// quartet); the digestion and the accumulators are FP64 either way.
+template <int LA, int LB, int LC, int LD, int G, class R>
+__global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
+  using K = Cls<LA, LB, LC, LD>;
+  constexpr int QPW = 64 / G;  // quartets per wave
+  __shared__ double s_I[4][QPW * K::NC];
+  __shared__ int s_bf[4][QPW][4];
+  __shared__ unsigned s_mask[4][QPW];
+  __shared__ int s_mol[4][QPW];
+  __shared__ R s_tab[rys_tab_len(K::NR)];
+  const R *__restrict__ rysn = rys_section<K::NR, R>(a.rys, a.rys32, a.rys_lds, s_tab);
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane & (G - 1), qw = lane / G;
+  // quartet count: the list's device counter for a launch sized to the
+  // list's capacity before the list was built
+  const long long nq = a.nq_dev ? min((long long)*a.nq_dev - a.nq_base, a.nq) : a.nq;
+  const long long q0 = ((long long)blockIdx.x * 4 + wv) * QPW;
+  if (q0 >= nq) return;  // uniform across the wave; no block barriers below
+  const long long q = q0 + qw;
+  const bool valid = q < nq;
+  Quartet qt{0u, 0u};
+  PairRec bp{}, kp{};
+  if (valid) {
+    qt = a.quartets[q];
+    bp = a.bra[qt.bra];
+    kp = a.ket[qt.ket_mask & 0xFFFFFFu];
+  }
+  R I[K::NC];
+  eval_quartet<LA, LB, LC, LD, G, R>(a.prims, rysn, bp, kp, g, I);
+  const double scale = quartet_scale(bp, kp);
   double *sI = s_I[wv] + qw * K::NC;
 #pragma unroll
   for (int o = 0; o < K::NC; ++o)

@@ -97,7 +97,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
     return t;
   };
   R *H = s_H[wv];
-  const R *__restrict__ rys = rys_table<R>(a);
+  const R *__restrict__ rys = rys_table<R>(a.rys, a.rys32);
 
   // the lane's outputs o = lane + 64 k and their 2D-table offsets
   int ox[NOWN], oy[NOWN], oz[NOWN];

@@ -11,6 +11,7 @@
 // integral batch to pure functions.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <mutex>
 
 #include "digest.hpp"
@@ -190,6 +191,65 @@ __global__ __launch_bounds__(64) void eri_team_kernel(EriLaunch a) {
 // ---------------------------------------------------------------------------
 constexpr int kListThreads = 1024;
 
+// The reference's screen of one candidate quartet (bra pair P with its own
+// block norms pb, ket pair Q with pk): the Schwarz product against the max
+// of the six shell-block norms in reference labels (the pair the reference
+// loop visits later is the bra), globally and per matrix (separated
+// screen).  Returns whether any contraction survives and their mask.
+struct ScreenArgs {
+  const double *Bg, *Bm;
+  const long long *mol_boff;
+  const int *mol_ns;
+  int nmat;
+  double thresh;
+  int screen;
+};
+__device__ __forceinline__ bool screen_candidate(const ScreenArgs &a, const PairScr &P, const PairScr &Q,
+                                                 const double *pb, const double *pk, double qq, unsigned &mask) {
+  mask = 0;
+  const int ps1 = (int)(P.s12 & 0xFFFFu), ps2 = (int)(P.s12 >> 16);
+  const int qs1 = (int)(Q.s12 & 0xFFFFu), qs2 = (int)(Q.s12 >> 16);
+  const bool p_first = (ps1 > qs1) || (ps1 == qs1 && ps2 >= qs2);
+  const int s1 = p_first ? ps1 : qs1, s2 = p_first ? ps2 : qs2;
+  const int s3 = p_first ? qs1 : ps1, s4 = p_first ? qs2 : ps2;
+  // lower bounds from the pairs' own block norms: enough to accept
+  if (!a.screen) {
+    mask = (1u << a.nmat) - 1u;
+    return true;
+  }
+  if (!(fmax(pb[0], pk[0]) * qq < a.thresh)) {
+    if (a.nmat == 1) {
+      mask = 1u;
+      return true;
+    }
+    unsigned lm = 0;
+    for (int m = 0; m < a.nmat; ++m)
+      if (!(fmax(pb[1 + m], pk[1 + m]) * qq < a.thresh)) lm |= 1u << m;
+    if (lm == (1u << a.nmat) - 1u) {
+      mask = lm;
+      return true;
+    }
+  }
+  const long long boff = a.mol_boff[P.mol];
+  const int ns = a.mol_ns[P.mol];
+  auto bmax = [&](const double *B, int ld, int m) {
+    auto at = [&](int x, int y) { return B[(boff + x + (size_t)y * ns) * ld + m]; };
+    double v = fmax(at(s1, s2), fmax(at(s1, s3), at(s2, s3)));
+    v = fmax(v, at(s3, s4));
+    v = fmax(v, fmax(at(s1, s4), at(s2, s4)));
+    return v;
+  };
+  if (bmax(a.Bg, 1, 0) * qq < a.thresh) return false;
+  if (a.nmat == 1) {
+    mask = 1u;
+  } else {
+    // per-matrix norms interleaved: the nmat values of one block share a line
+    for (int m = 0; m < a.nmat; ++m)
+      if (!(bmax(a.Bm, a.nmat, m) * qq < a.thresh)) mask |= 1u << m;
+  }
+  return mask != 0u;
+}
+
 // Candidates [base, base + 1024) of one list: the block's bra range is found
 // once, each thread tests one candidate, survivors are appended block-wide
 // (the waves' counts meet in LDS and one thread reserves the block's slots
@@ -236,57 +296,11 @@ __device__ __forceinline__ void list_chunk(const ListLaunch &a, long long base) 
     bi = lo;
     kj = a.ket_lo[bi] + (int)(ta - a.cand_off[lo]);
     const PairScr P = a.bra[bi], Q = a.ket[kj];
-    const int ps1 = (int)(P.s12 & 0xFFFFu), ps2 = (int)(P.s12 >> 16);
-    const int qs1 = (int)(Q.s12 & 0xFFFFu), qs2 = (int)(Q.s12 >> 16);
-    const bool p_first = (ps1 > qs1) || (ps1 == qs1 && ps2 >= qs2);
-    const int s1 = p_first ? ps1 : qs1, s2 = p_first ? ps2 : qs2;
-    const int s3 = p_first ? qs1 : ps1, s4 = p_first ? qs2 : ps2;
     qq = P.Q * Q.Q;
     kk4 = (double)P.kk * (double)Q.kk;
-    // lower bounds from the pairs' own block norms: enough to accept
     const double *pb = a.bra_pn + (size_t)bi * (a.nmat + 1), *pk = a.ket_pn + (size_t)kj * (a.nmat + 1);
-    bool decided = false;
-    if (!a.screen) {
-      pass = true;
-      mask = (1u << a.nmat) - 1u;
-      decided = true;
-    } else if (!(fmax(pb[0], pk[0]) * qq < a.thresh)) {
-      if (a.nmat == 1) {
-        pass = true;
-        mask = 1u;
-        decided = true;
-      } else {
-        unsigned lm = 0;
-        for (int m = 0; m < a.nmat; ++m)
-          if (!(fmax(pb[1 + m], pk[1 + m]) * qq < a.thresh)) lm |= 1u << m;
-        if (lm == (1u << a.nmat) - 1u) {
-          pass = true;
-          mask = lm;
-          decided = true;
-        }
-      }
-    }
-    if (!decided) {
-      const long long boff = a.mol_boff[P.mol];
-      const int ns = a.mol_ns[P.mol];
-      auto bmax = [&](const double *B, int ld, int m) {
-        auto at = [&](int x, int y) { return B[(boff + x + (size_t)y * ns) * ld + m]; };
-        double v = fmax(at(s1, s2), fmax(at(s1, s3), at(s2, s3)));
-        v = fmax(v, at(s3, s4));
-        v = fmax(v, fmax(at(s1, s4), at(s2, s4)));
-        return v;
-      };
-      if (!(bmax(a.Bg, 1, 0) * qq < a.thresh)) {
-        if (a.nmat == 1) {
-          mask = 1u;
-        } else {
-          // per-matrix norms interleaved: the nmat values of one block share a line
-          for (int m = 0; m < a.nmat; ++m)
-            if (!(bmax(a.Bm, a.nmat, m) * qq < a.thresh)) mask |= 1u << m;
-        }
-        pass = mask != 0u;
-      }
-    }
+    const ScreenArgs sa{a.Bg, a.Bm, a.mol_boff, a.mol_ns, a.nmat, a.thresh, a.screen};
+    pass = screen_candidate(sa, P, Q, pb, pk, qq, mask);
   }
   // precision split on the Schwarz product (the FP32 list when below cut32)
   const bool lo32 = pass && a.cut32 > 0.0 && qq < a.cut32;
@@ -356,6 +370,107 @@ __global__ __launch_bounds__(kListThreads) void quartet_lists_kernel(const ListL
     const int k = s_k;
     __syncthreads();
     list_chunk(lists[k], (c - start[k]) * kListThreads);
+  }
+}
+
+// Tile lists: one block of kTileListWaves waves per chunk of kTileChunk
+// sub-blocks of one list, each wave taking its sub-blocks one after the
+// other.  Lane l tests candidate (i, j) = (l / 8, l % 8) of a sub-block
+// with the quartet lists' screen; the survivors of each precision make the
+// sub-block's tile (a bitmap and the masks) at the sub-block's own slot.
+// The block's counts meet in LDS and take one atomic per counter.
+constexpr int kTileListWaves = 4;
+__global__ __launch_bounds__(64 * kTileListWaves) void tile_lists_kernel(const TileListLaunch *lists,
+                                                                         const long long *start, int nlist,
+                                                                         long long nchunk) {
+  __shared__ unsigned long long s_cnt[kTileListWaves][3];
+  __shared__ double s_ps[kTileListWaves][2];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (long long ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
+    const long long t0 = ch * kTileChunk;
+    int lo = 0, hi = nlist - 1;  // the chunk's list: last k with start[k] <= t0
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (start[mid] <= t0) lo = mid; else hi = mid - 1;
+    }
+    const TileListLaunch &a = lists[lo];
+    const long long base = t0 - start[lo];
+    unsigned long long ncand = 0, nq64 = 0, nq32 = 0;  // lane 0's running sums
+    double ps64 = 0.0, ps32 = 0.0;
+    for (int u = wv; u < kTileChunk; u += kTileListWaves) {
+      const long long slot = base + u;
+      const SubBlock sb = a.sub[slot];
+      const int b0 = a.bmem[sb.bra_m0];
+      const int mol = a.bra[b0].mol;
+      // Schwarz pre-cut of the whole sub-block (uniform across the wave);
+      // an empty sub-block has no candidates
+      const bool live = sb.nb > 0 && !(a.screen && sb.qq * a.molmax[mol] < a.thresh);
+      const int i = lane >> 3, j = lane & 7;
+      const bool cand = live && i < sb.nb && j < sb.nk && (!sb.diag || i >= j);
+      bool pass = false;
+      unsigned mask = 0;
+      double qq = 0.0, kk4 = 0.0;
+      if (cand) {
+        const int bi = a.bmem[sb.bra_m0 + i], kj = a.kmem[sb.ket_m0 + j];
+        const PairScr P = a.bra[bi], Q = a.ket[kj];
+        qq = P.Q * Q.Q;
+        kk4 = (double)P.kk * (double)Q.kk;
+        const ScreenArgs sa{a.Bg, a.Bm, a.mol_boff, a.mol_ns, a.nmat, a.thresh, a.screen};
+        pass = screen_candidate(sa, P, Q, a.bra_pn + (size_t)bi * (a.nmat + 1),
+                                a.ket_pn + (size_t)kj * (a.nmat + 1), qq, mask);
+      }
+      const bool lo32 = pass && a.cut32 > 0.0 && qq < a.cut32;
+      const bool hi64 = pass && !lo32;
+      const unsigned long long cbits = __ballot(cand);
+      for (int l = 0; l < (a.out32 ? 2 : 1); ++l) {
+        const bool take = l == 0 ? hi64 : lo32;
+        const unsigned long long bits = __ballot(take);
+        double ps = take ? kk4 : 0.0;
+        for (int off = 32; off > 0; off >>= 1) ps += __shfl_xor(ps, off);
+        Tile *T = (l == 0 ? a.out : a.out32) + slot;
+        if (bits) T->mask[lane] = (unsigned char)(take ? mask : 0u);
+        if (lane == 0) {
+          T->bra_m0 = sb.bra_m0;
+          T->ket_m0 = sb.ket_m0;
+          T->nb = sb.nb;
+          T->nk = sb.nk;
+          T->pad = 0u;
+          T->bits = bits;  // written for every sub-block: an empty tile is skipped
+          T->pad2 = 0ull;
+          if (l == 0) {
+            nq64 += (unsigned long long)__popcll(bits);
+            ps64 += ps;
+          } else {
+            nq32 += (unsigned long long)__popcll(bits);
+            ps32 += ps;
+          }
+        }
+      }
+      if (lane == 0) ncand += (unsigned long long)__popcll(cbits);
+    }
+    if (lane == 0) {
+      s_cnt[wv][0] = ncand;
+      s_cnt[wv][1] = nq64;
+      s_cnt[wv][2] = nq32;
+      s_ps[wv][0] = ps64;
+      s_ps[wv][1] = ps32;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+      const int k = threadIdx.x;
+      if (k < 3) {
+        unsigned long long v = 0;
+        for (int w = 0; w < kTileListWaves; ++w) v += s_cnt[w][k];
+        unsigned long long *dst = k == 0 ? a.ncand : k == 1 ? a.nquart : a.nquart32;
+        if (v && dst) atomicAdd(dst, v);
+      } else {
+        double v = 0.0;
+        for (int w = 0; w < kTileListWaves; ++w) v += s_ps[w][k - 3];
+        double *dst = k == 3 ? a.prim_sum : a.prim_sum32;
+        if (v != 0.0 && dst) atomic_add(dst, v);
+      }
+    }
+    __syncthreads();  // the LDS above is rewritten by the next chunk
   }
 }
 
@@ -556,6 +671,17 @@ hipError_t launch_eri_team(const EriLaunch &a, hipStream_t s) {
   if (attr_err != hipSuccess) return attr_err;
   const long long grid = a.nq < 65536 ? a.nq : 65536;
   hipLaunchKernelGGL(eri_team_kernel, dim3((unsigned)grid), dim3(64), lds, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_lists(const TileListLaunch *lists, const long long *start, int nlist, long long nsub,
+                             hipStream_t s) {
+  if (nsub <= 0 || nlist <= 0) return hipSuccess;
+  if (nsub % kTileChunk) return hipErrorInvalidValue;  // lists are padded to whole chunks
+  const long long nchunk = nsub / kTileChunk;
+  const long long blocks = std::min<long long>(nchunk, 1ll << 20);
+  hipLaunchKernelGGL(tile_lists_kernel, dim3((unsigned)blocks), dim3(64 * kTileListWaves), 0, s, lists, start,
+                     nlist, nchunk);
   return hipGetLastError();
 }
 

@@ -27,6 +27,7 @@ struct EriLaunch {
   double mean_prims;             // mean primitive quartets per listed shell quartet
   int force_g;                   // lanes per quartet of the lane-group kernels (0: chosen per launch)
   int force_qt;                  // quartets per wave of the wave kernels (0: the class default)
+  int rys_lds;                   // lane-group kernels: stage the class's Rys table section in LDS
   PlaneSet planes;
   double *store;                 // STORE mode: contracted Cartesian quartets
   const long long *store_off;    // per quartet offset into store
@@ -39,6 +40,9 @@ hipError_t launch_eri_team(const EriLaunch &a, hipStream_t s);
 // hipErrorNotSupported when the class has no specialised kernel.
 hipError_t launch_eri_fast(const EriLaunch &a, hipStream_t s);
 bool eri_fast_available(int la, int lb, int lc, int ld);
+// position of a lane-group class in (ss|ss), (ps|ss), (ps|ps), (pp|ss),
+// (pp|ps), (ds|ss), (ds|ps), (dp|ss); -1 for the other classes
+int fast_class_index(int la, int lb, int lc, int ld);
 // One-wave-per-quartet templated kernels for the larger s/p/d classes.
 hipError_t launch_eri_wave(const EriLaunch &a, hipStream_t s);
 bool eri_wave_available(int la, int lb, int lc, int ld);
@@ -112,6 +116,62 @@ constexpr int kListChunk = 1024;
 hipError_t launch_gather(const long long *src, const long long *idx, int n, long long *dst, hipStream_t s);
 hipError_t launch_quartet_lists(const ListLaunch *lists, const long long *start, int nlist, long long nchunk,
                                 hipStream_t s);
+// Tile lists of the tile class pairs of a build: sub-block t of a list
+// gives tile t of each precision (FP32 when Q_ab Q_cd < cut32), an empty
+// one (bits = 0) when nothing survives.  The Schwarz pre-cut (qq x the
+// molecule's largest block norm) drops a whole sub-block; the candidates of
+// the rest pass the same screen as the quartet lists.  One block takes
+// kTileChunk sub-blocks of one list and adds its counts once.
+struct TileListLaunch {
+  const SubBlock *sub;
+  const PairScr *bra, *ket;   // screening records of the bra / ket classes
+  const int *bmem, *kmem;     // group-ordered member arrays of the classes
+  const double *bra_pn, *ket_pn;
+  const double *Bg, *Bm;
+  const long long *mol_boff;
+  const int *mol_ns;
+  const double *molmax;
+  int nmat;
+  double thresh;
+  int screen;
+  Tile *out;                   // one tile per sub-block
+  unsigned long long *nquart;  // quartets of the list
+  unsigned long long *ncand;   // candidates of the sub-blocks past the pre-cut
+  double *prim_sum;            // sum over quartets of Ka Kb Kc Kd
+  double cut32;
+  Tile *out32;
+  unsigned long long *nquart32;
+  double *prim_sum32;
+};
+// lists[k] (device) covers sub-blocks [start[k], start[k + 1]) of a common
+// sub-block space of nsub = start[nlist] entries (multiples of kTileChunk)
+hipError_t launch_tile_lists(const TileListLaunch *lists, const long long *start, int nlist, long long nsub,
+                             hipStream_t s);
+
+// ERI + digestion of a tile list (lane-group classes, Hermitian J / K
+// planes only): every quartet's shared targets -- J(a, b), J(d, c), K(a, c),
+// K(d, a), K(b, c) -- are summed in LDS over the tile and added to HBM once
+// per tile; K(d, b) is added per quartet.
+struct TileEriLaunch {
+  int la, lb, lc, ld;
+  const Tile *tiles;
+  long long ntile;    // tiles of the list, empty ones included
+  const int *bmem, *kmem;
+  const PairRec *bra, *ket;
+  const PrimPair *prims;
+  const double *rys;
+  const float *rys32;
+  int fp32;
+  double mean_prims;  // mean primitive quartets per quartet
+  int force_g;        // lanes per quartet (0: chosen per launch)
+  int rys_lds;        // stage the class's Rys table section in LDS
+  PlaneSet planes;
+};
+hipError_t launch_eri_tile(const TileEriLaunch &a, hipStream_t s);
+// LDS accumulator doubles per wave of a tile launch (0: the class has no
+// tile kernel)
+int tile_acc_doubles(int la, int lb, int lc, int ld, const PlaneSet &ps);
+
 // pn[i (nmat + 1) + k] = block norm of pair i's own shells (k = 0 global,
 // k = 1 + m matrix m)
 hipError_t launch_pair_norms(const PairScr *pairs, int npair, const double *Bg, const double *Bm,

@@ -40,7 +40,9 @@ def _density(rng, nb, nocc, cplx=False, scale=1.0):
 
 
 def _pair(basis, opts=()):
-    """The same basis under a recording context and an eager one."""
+    """The same basis under a recording context and an eager one; the eager
+    one builds quartet lists only (a recorded build never takes tiles), so
+    that both count the same candidates."""
     from chronusq_public_amd import cqx
     out = []
     for graph in (1, 0):
@@ -48,6 +50,7 @@ def _pair(basis, opts=()):
         for k, v in opts:
             tpi.set_option(k, v)
         tpi.set_option("graph", graph)
+        tpi.set_option("tiles", 0)
         out.append((tpi, cqx.GTODirectTPIContraction(tpi)))
     return out
 
