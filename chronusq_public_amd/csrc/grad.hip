@@ -2,7 +2,7 @@
 // shell quartet digested into one J or K matrix per nuclear coordinate,
 // DirectGradContraction::directScaffoldGrad (direct.hpp:2198-2580).
 //
-// One 64-lane wave per shell quartet, any s/p/d class.  Per primitive
+// One 64-lane wave per shell quartet, any class through (ff|ff).  Per primitive
 // quartet the Rys 2D integrals are built with one more unit of angular
 // momentum on centres A, B and C; the derivative of a Cartesian component
 // along direction x is 2 alpha I(a + 1_x) - a_x I(a - 1_x) on the x factor,
@@ -10,8 +10,13 @@
 // d/dD = -(d/dA + d/dB + d/dC).  The twelve derivative quartets, scaled by
 // the 8-fold degeneracy, are digested with the Hermitian J / K updates of
 // the reference (direct.hpp:2450-2497) into the Cartesian accumulator of
-// the coordinate (atom of the centre, direction).
+// the coordinate (atom of the centre, direction).  The derivative quartets
+// are built in LDS for a chunk of the Cartesian components at a time (the
+// whole quartet when it fits beside the 2D tables; the primitive loop runs
+// once per chunk).
 #include <hip/hip_runtime.h>
+
+#include <mutex>
 
 #include "kernels.hpp"
 #include "rys.hpp"
@@ -61,8 +66,8 @@ __device__ __forceinline__ int hix(const GDims &D, int ia, int ib, int ic, int i
 // (entries with both ia = la + 1 and ib = lb + 1 are not needed and not set)
 __device__ void rys_2d_grad(const GDims &D, double t2, double p, double q, double PAd, double QCd, double PQd,
                             double ABd, double CDd, double scale, double *__restrict__ H) {
-  double G[8][8];
-  double E[8][4][8];
+  double G[8][8];      // [i <= la + lb + 1][k <= lc + ld + 1]
+  double E[8][5][8];   // bra transfer: [i][j <= lb + 1][k]
   const double pq = p + q, rho = p * q / pq;
   const double B00 = 0.5 * t2 / pq;
   const double B10 = 0.5 / p * (1.0 - rho / p * t2);
@@ -85,7 +90,7 @@ __device__ void rys_2d_grad(const GDims &D, double t2, double p, double q, doubl
   for (int ia = 0; ia <= D.la + 1; ++ia)
     for (int ib = 0; ib <= D.lb + 1; ++ib) {
       if (ia + ib >= D.ni) continue;
-      double F[8][3];
+      double F[8][4];  // ket transfer: [k][l <= ld]
       for (int k = 0; k < D.nk; ++k) F[k][0] = E[ia][ib][k];
       for (int l = 1; l <= D.ld; ++l)
         for (int k = 0; k + l < D.nk; ++k) F[k][l] = F[k + 1][l - 1] + CDd * F[k][l - 1];
@@ -103,10 +108,14 @@ __global__ __launch_bounds__(64) void eri_grad_kernel(GradLaunch a) {
   double *s_x = lds;
   double *s_w = lds + 8;
   double *s_H = lds + 16;
-  double *s_dI = s_H + (size_t)D.nr * 3 * D.nh;  // [9][nc]: centre A, B, C x direction
+  const int chunk = a.chunk;                      // Cartesian components per pass
+  double *s_dI = s_H + (size_t)D.nr * 3 * D.nh;  // [9][chunk]: centre A, B, C x direction
   const int na = ncart_g(D.la), nbb = ncart_g(D.lb), ncc = ncart_g(D.lc), ndd = ncart_g(D.ld);
   const size_t nbc2 = (size_t)a.nbc * a.nbc;
-  for (long long q = blockIdx.x; q < a.nq; q += gridDim.x) {
+  const long long npass = (D.nc + chunk - 1) / chunk;
+  for (long long qp = blockIdx.x; qp < a.nq * npass; qp += gridDim.x) {
+    const long long q = qp / npass;
+    const int o0 = (int)(qp - q * npass) * chunk, o1 = min(D.nc, o0 + chunk);
     int i, j;
     if (a.same_class) {  // j <= i, row-major lower triangle
       long long r = (long long)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
@@ -120,7 +129,7 @@ __global__ __launch_bounds__(64) void eri_grad_kernel(GradLaunch a) {
     }
     const PairRec bp = a.bra[i];
     const PairRec kp = a.ket[j];
-    for (int o = lane; o < 9 * D.nc; o += 64) s_dI[o] = 0.0;
+    for (int o = lane; o < 9 * chunk; o += 64) s_dI[o] = 0.0;
     for (int pi = 0; pi < bp.npp; ++pi) {
       const PrimPair A = a.prims[bp.pp_off + pi];
       const double al_a = a.prim_alpha[bp.pp_off + pi], al_b = A.p - al_a;
@@ -146,7 +155,7 @@ __global__ __launch_bounds__(64) void eri_grad_kernel(GradLaunch a) {
         }
         __syncthreads();
         const double twoal[3] = {2.0 * al_a, 2.0 * al_b, 2.0 * al_c};
-        for (int o = lane; o < D.nc; o += 64) {
+        for (int o = o0 + lane; o < o1; o += 64) {
           int rem = o;
           const int id = rem % ndd; rem /= ndd;
           const int ic = rem % ncc; rem /= ncc;
@@ -176,7 +185,7 @@ __global__ __launch_bounds__(64) void eri_grad_kernel(GradLaunch a) {
               acc[k * 3 + 2] += f[0] * f[1] * df[k][2];
             }
           }
-          for (int t = 0; t < 9; ++t) s_dI[(size_t)t * D.nc + o] += acc[t];
+          for (int t = 0; t < 9; ++t) s_dI[(size_t)t * chunk + (o - o0)] += acc[t];
         }
       }
     }
@@ -189,7 +198,7 @@ __global__ __launch_bounds__(64) void eri_grad_kernel(GradLaunch a) {
     int gc[4];
     for (int k = 0; k < 4; ++k) gc[k] = a.shell_atom[sh[k]] * 3;
     const int N = a.nbc;
-    for (int o = lane; o < D.nc; o += 64) {
+    for (int o = o0 + lane; o < o1; o += 64) {
       int rem = o;
       const int id = rem % ndd; rem /= ndd;
       const int ic = rem % ncc; rem /= ncc;
@@ -198,9 +207,9 @@ __global__ __launch_bounds__(64) void eri_grad_kernel(GradLaunch a) {
       const int f1 = bf[0] + ia, f2 = bf[1] + ib, f3 = bf[2] + ic, f4 = bf[3] + id;
       for (int dir = 0; dir < 3; ++dir) {
         double v[4];
-        v[0] = deg * s_dI[(size_t)(0 + dir) * D.nc + o];
-        v[1] = deg * s_dI[(size_t)(3 + dir) * D.nc + o];
-        v[2] = deg * s_dI[(size_t)(6 + dir) * D.nc + o];
+        v[0] = deg * s_dI[(size_t)(0 + dir) * chunk + (o - o0)];
+        v[1] = deg * s_dI[(size_t)(3 + dir) * chunk + (o - o0)];
+        v[2] = deg * s_dI[(size_t)(6 + dir) * chunk + (o - o0)];
         v[3] = -(v[0] + v[1] + v[2]);  // translational invariance
         for (int k = 0; k < 4; ++k) {
           const double I = v[k];
@@ -231,20 +240,38 @@ __global__ __launch_bounds__(64) void eri_grad_kernel(GradLaunch a) {
 
 }  // namespace
 
-size_t grad_lds_bytes(int la, int lb, int lc, int ld) {
-  auto nca = [](int l) { return (size_t)((l + 1) * (l + 2) / 2); };
+size_t grad_lds_bytes(int la, int lb, int lc, int ld, int chunk) {
   const int nr = (la + lb + lc + ld + 1) / 2 + 1;
   const size_t nh = (size_t)(la + 2) * (lb + 2) * (lc + 2) * (ld + 1);
-  const size_t nc = nca(la) * nca(lb) * nca(lc) * nca(ld);
-  return sizeof(double) * (16 + (size_t)nr * 3 * nh + 9 * nc);
+  return sizeof(double) * (16 + (size_t)nr * 3 * nh + 9 * (size_t)chunk);
+}
+
+int grad_chunk(int la, int lb, int lc, int ld, size_t lds_max) {
+  auto nca = [](int l) { return (l + 1) * (l + 2) / 2; };
+  const int nc = nca(la) * nca(lb) * nca(lc) * nca(ld);
+  const size_t fixed = grad_lds_bytes(la, lb, lc, ld, 0);
+  if (fixed >= lds_max) return 0;
+  const size_t fit = (lds_max - fixed) / (9 * sizeof(double));
+  if (fit >= (size_t)nc) return nc;
+  return fit >= 64 ? (int)(fit / 64 * 64) : 0;
 }
 
 hipError_t launch_eri_grad(const GradLaunch &a, hipStream_t s) {
   if (a.nq <= 0) return hipSuccess;
-  if (a.la > 2 || a.lb > 2 || a.lc > 2 || a.ld > 2) return hipErrorNotSupported;
-  const long long blocks = a.nq < 262144 ? a.nq : 262144;
-  hipLaunchKernelGGL(eri_grad_kernel, dim3((unsigned)blocks), dim3(64), grad_lds_bytes(a.la, a.lb, a.lc, a.ld), s,
-                     a);
+  if (a.la > 3 || a.lb > 3 || a.lc > 3 || a.ld > 3 || a.chunk <= 0) return hipErrorNotSupported;
+  const int nc = ((a.la + 1) * (a.la + 2) / 2) * ((a.lb + 1) * (a.lb + 2) / 2) * ((a.lc + 1) * (a.lc + 2) / 2) *
+                 ((a.ld + 1) * (a.ld + 2) / 2);
+  const long long work = a.nq * ((nc + a.chunk - 1) / a.chunk);
+  const long long blocks = work < 262144 ? work : 262144;
+  const size_t lds = grad_lds_bytes(a.la, a.lb, a.lc, a.ld, a.chunk);
+  static std::once_flag once;  // dynamic LDS beyond the default limit for the f classes
+  static hipError_t attr = hipSuccess;
+  std::call_once(once, [&] {
+    attr = hipFuncSetAttribute(reinterpret_cast<const void *>(eri_grad_kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  });
+  if (attr != hipSuccess) return attr;
+  hipLaunchKernelGGL(eri_grad_kernel, dim3((unsigned)blocks), dim3(64), lds, s, a);
   return hipGetLastError();
 }
 

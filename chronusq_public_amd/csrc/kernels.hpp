@@ -67,9 +67,14 @@ struct GradLaunch {
   double *acc;                // [3 natom][nmat][nbc x nbc] Cartesian accumulators
   int nmat, nbc;
   unsigned kmask;             // bit m: matrix m is an exchange contraction
+  int chunk;                  // Cartesian components of a quartet per pass (grad_chunk)
 };
 hipError_t launch_eri_grad(const GradLaunch &a, hipStream_t s);
-size_t grad_lds_bytes(int la, int lb, int lc, int ld);
+// LDS bytes of a gradient launch that builds chunk Cartesian components per
+// pass, and the largest such chunk (a multiple of 64, or the whole quartet)
+// within lds_max bytes; 0 when even the 2D tables do not fit
+size_t grad_lds_bytes(int la, int lb, int lc, int ld, int chunk);
+int grad_chunk(int la, int lb, int lc, int ld, size_t lds_max);
 
 struct ListLaunch {
   const PairScr *bra, *ket;   // screening records of the bra / ket classes

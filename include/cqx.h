@@ -208,8 +208,8 @@ int cqx_last_stats(const cqx_ctx *ctx, cqx_stats *st);
  * and the NB x NB output AX of coordinate g, which receives
  * 1/2 (A + A^T) (callers zero it).  Real, Hermitian contractions only
  * (the reference instantiates <double, double> and stops on non-Hermitian
- * ones), no screening (as the reference), s/p/d shells, one-molecule
- * contexts; mem as for cqx_two_body_contract. */
+ * ones), no screening (as the reference, direct.hpp:2204-2205), s through f
+ * shells, one-molecule contexts; mem as for cqx_two_body_contract. */
 int cqx_grad_two_body_contract(cqx_ctx *ctx, int ngrad, int nmat, const cqx_contraction *list, int mem);
 
 /* Tuning knobs: "prim_screen" (primitive-pair drop threshold, default

@@ -25,9 +25,12 @@ def _cases():
     yield "water 6-31G(d) converged D", mol, gbs, d["PDM_SCALAR"], False
     yield "water 6-31G(d) Cartesian d", mol, gbs, None, True
     yield "(H2O)2 cc-pVDZ", water_cluster(2, 4.0, seed=5), basis_text("cc-pVDZ"), None, False
+    # f shells: (ff|ff) and the other f classes, pure and Cartesian
+    yield "water cc-pVTZ", water_cluster(1, 3.0, seed=1), basis_text("cc-pVTZ"), None, False
+    yield "water cc-pVTZ Cartesian", water_cluster(1, 3.0, seed=1), basis_text("cc-pVTZ"), None, True
 
 
-@pytest.mark.parametrize("idx", [0, 1, 2])
+@pytest.mark.parametrize("idx", [0, 1, 2, 3, 4])
 def test_grad_contraction_matches_oracle(idx):
     from chronusq_public_amd import cqx
     name, mol, gbs, D, cart = list(_cases())[idx]
@@ -58,10 +61,11 @@ def test_grad_contraction_rejects_unsupported():
     from chronusq_public_amd.chem import basis_text, water_cluster
     say("case: gradient contraction errors")
     mol = water_cluster(1, 3.0, seed=1)
-    tz = cqx.DirectTPI(cqx.BasisSet(basis_text("cc-pVTZ"), mol), 1e-12, device=0)  # f shells
-    nb = tz.basis.nBasis
+    # a batched context: gradient contractions take one molecule
+    parts = [cqx.BasisSet(basis_text("cc-pVDZ"), mol), cqx.BasisSet(basis_text("STO-3G"), mol)]
+    bt = cqx.DirectTPI(cqx.BasisSet.batch(parts), 1e-12, device=0)
     with pytest.raises(cqx.CErr):
-        cqx.getGDGrad(np.eye(nb), tz, 3)
+        cqx.getGDGrad(np.eye(bt.basis.nBasis), bt, 6)
     dz = cqx.DirectTPI(cqx.BasisSet(basis_text("cc-pVDZ"), mol), 1e-12, device=0)
     nb = dz.basis.nBasis
     D = np.eye(nb)
