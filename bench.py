@@ -220,6 +220,8 @@ def mixed_comparison(steps, warmup, cut=1e-4):
     J, K = b.outputs()
     st64 = b.tpi.last_stats()
     b.tpi.set_option("fp32_cut", cut)
+    # the mixed warm-up build tunes the FP32 lists (and the shorter FP64 ones)
+    b.tpi.set_option("autotune", 1)
     dtm, erim, flm, flm32 = b.timed(steps, warmup)
     Jm, Km = b.outputs()
     stm = b.tpi.last_stats()
@@ -331,6 +333,7 @@ def batched_comparison(nmol, steps, cut=1e-4, seed=0):
     singles = []
     for b, D in zip(bases, dens):
         tpi = cqx.DirectTPI(b, 1e-12, device=0)
+        tpi.set_option("autotune", 1)  # tuned in the warm-up round, as the batched context
         X, J, K = make(tpi, b.nBasis, D)
         singles.append((tpi, cqx.GTODirectTPIContraction(tpi), X, J, K))
 

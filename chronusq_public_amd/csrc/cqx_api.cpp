@@ -232,13 +232,19 @@ struct cqx_ctx {
   DevBuf<cqx::ListLaunch> d_glists;
   DevBuf<long long> d_glstart, d_gidx, d_gchk;
 
-  // Tiles (option "tiles", default on): the lane-group class pairs of a
-  // build whose planes are all Hermitian go through tile lists instead of
-  // quartet lists.  Per class, its member array (pair indices grouped by
-  // molecule and first shell); per tile class pair, its sub-blocks, a range
-  // of d_sub.
-  int use_tiles = 0;
-  int tile_classes = 0xFF;  // lane-group classes (bit k: k-th of ss|ss, ps|ss, ps|ps, pp|ss, pp|ps, ds|ss, ds|ps, dp|ss)
+  // Tiles (option "tiles"): the lane-group class pairs of a build whose
+  // planes are all Hermitian go through tile lists instead of quartet
+  // lists; 2 (the default) does so only when the build has at most one
+  // Hermitian J and one Hermitian K plane (a real RHF J / K pair), where
+  // summing the shared targets pays; with the eight exchange planes of a
+  // two-component build the quartet lists are faster.  Per class, its
+  // member array (pair indices grouped by molecule and first shell); per
+  // tile class pair, its sub-blocks, a range of d_sub.
+  int use_tiles = 2;
+  // lane-group classes that take tiles (bit k: k-th of ss|ss, ps|ss, ps|ps,
+  // pp|ss, pp|ps, ds|ss, ds|ps, dp|ss); (ss|ss) and (dp|ss) are faster
+  // through quartet lists
+  int tile_classes = 0x7E;
   int rys_lds = 1;  // lane-group kernels stage their Rys table section in LDS
   DevBuf<int> d_members[kNumClass];
   struct TileCP { int cb, ck; long long off, n; };
@@ -1030,8 +1036,10 @@ int cqx_set_option(cqx_ctx *c, const char *name, double value) {
   }
   if (!std::strcmp(name, "tiles")) {
     // 1: lane-group class pairs of Hermitian builds through tile lists
-    // (shared J / K targets summed per tile); 0 (default): quartet lists
-    c->use_tiles = value != 0.0;
+    // (shared J / K targets summed per tile); 2 (default): only builds with
+    // at most one Hermitian J and one Hermitian K plane; 0: quartet lists
+    if (!(value == 0.0 || value == 1.0 || value == 2.0)) return fail(c, CQX_ERR_ARG, "tiles must be 0, 1 or 2");
+    c->use_tiles = (int)value;
     return CQX_OK;
   }
   if (!std::strcmp(name, "tile_classes")) {
@@ -1427,7 +1435,8 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
       // when every plane of the group is Hermitian and the build is neither
       // recorded nor allocation-free; their quartets come from tile lists
       const bool tiles_ok = c->use_tiles && !scr && !c->use_graph && !c->tile_cps.empty() &&
-                            gp.kcnt[cqx::kJNonHer] == 0 && gp.kcnt[cqx::kKNonHer] == 0;
+                            gp.kcnt[cqx::kJNonHer] == 0 && gp.kcnt[cqx::kKNonHer] == 0 &&
+                            (c->use_tiles == 1 || (gp.kcnt[cqx::kJHer] <= 1 && gp.kcnt[cqx::kKHer] <= 1));
       std::vector<const cqx_ctx::TileCP *> tcps;
       auto lane_group = [&](int cb, int ck) {
         int la_, lb_, lc_, ld_;

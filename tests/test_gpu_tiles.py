@@ -1,10 +1,12 @@
-"""Tile lists (option "tiles", on by default): the lane-group class pairs of
-Hermitian builds evaluate their quartets in tiles that share the bra's and
-the ket's first shells, summing the shared J / K targets per tile.  Each
-build must equal the quartet-list build (option tiles = 0) of the same
-densities to 1e-12 relative, with the same quartets (and FP32 quartets),
-for real RHF, complex two-component, mixed-precision, batched and
-unscreened builds; the tile build also equals the oracle to 1e-10."""
+"""Tile lists (option "tiles"): the lane-group class pairs of Hermitian
+builds evaluate their quartets in tiles that share the bra's and the ket's
+first shells, summing the shared J / K targets per tile.  Each build with
+tiles = 1 on every lane-group class must equal the quartet-list build
+(tiles = 0) of the same densities to 1e-12 relative, with the same quartets
+(and FP32 quartets), for real RHF, complex two-component, mixed-precision,
+batched and unscreened builds; the tile build also equals the oracle to
+1e-10.  The default (tiles = 2) takes tiles for a real J / K pair and
+quartet lists for the eight exchange planes of a two-component build."""
 import sys
 
 import numpy as np
@@ -29,7 +31,10 @@ def _density(rng, nb, nocc, cplx=False):
 def _build(basis, Xs, types, tiles, screen=True, **opts):
     from chronusq_public_amd import cqx
     tpi = cqx.DirectTPI(basis, 1e-12)
-    tpi.set_option("tiles", tiles)
+    if tiles is not None:
+        tpi.set_option("tiles", tiles)
+    if tiles == 1:
+        tpi.set_option("tile_classes", 255)
     for k, v in opts.items():
         tpi.set_option(k, v)
     gto = cqx.GTODirectTPIContraction(tpi)
@@ -114,3 +119,29 @@ def test_tiles_batched_molecules():
     A1, s1 = _build(bb, Xs, types, 1)
     A0, s0 = _build(bb, Xs, types, 0)
     _same(A1, A0, s1, s0)
+
+
+@pytest.mark.parametrize("case", ["rhf", "ghf"])
+def test_default_takes_tiles_for_a_real_pair_only(case):
+    """Default options: an RHF J / K build goes through tile lists (its
+    candidate count is the tile lists', not the quartet lists') and equals
+    the quartet-list build; a GHF build with eight exchange planes stays on
+    quartet lists (the same candidates)."""
+    from chronusq_public_amd import cqx
+    from chronusq_public_amd.chem import basis_text, water_cluster
+    b = cqx.BasisSet(basis_text("cc-pVDZ"), water_cluster(4, 8.0, seed=9))
+    rng = np.random.Generator(np.random.PCG64(31))
+    say(f"case: default tiles, {case}")
+    if case == "ghf":
+        D = _density(rng, b.nBasis, 20, cplx=True)
+        Xs, types = [D, D, 0.3 * D, 0.2 * D.conj(), 0.1 * D], [0, 1, 1, 1, 1]
+    else:
+        D = _density(rng, b.nBasis, 20)
+        Xs, types = [D, D], [0, 1]
+    A2, s2 = _build(b, Xs, types, None)
+    A0, s0 = _build(b, Xs, types, 0)
+    _same(A2, A0, s2, s0)
+    if case == "rhf":
+        assert s2["n_candidates"] != s0["n_candidates"]
+    else:
+        assert s2["n_candidates"] == s0["n_candidates"]
