@@ -498,7 +498,7 @@ def main():
     from chronusq_public_amd import cqx
     global LAUNCH_STREAMS
     LAUNCH_STREAMS = args.streams
-    bench = Build(args.workload)
+    bench = Build(args.workload, nocc=161 if args.workload == "alkane40" else 20)
     for o in args.opt:
         name, value = o.split("=", 1)
         bench.tpi.set_option(name, float(value))

@@ -152,6 +152,7 @@ struct cqx_ctx {
   std::vector<long long> mol_boff;             // molecule m's NS_m^2 block-norm offset
   std::vector<PrimPair> prims;
   std::vector<double> prim_alpha;  // exponent of the first shell of each primitive pair (gradients)
+  std::vector<cqx::PrimC32> primc32;  // the same pairs as the FP32 lists read them
   std::vector<double> schwarz;                 // NS x NS
 
   DevBuf<PairRec> d_pairs[kNumClass];
@@ -159,6 +160,7 @@ struct cqx_ctx {
   DevBuf<double> d_pn[kNumClass];  // per build: each pair's own block norms
   DevBuf<PrimPair> d_prims;
   DevBuf<double> d_prim_alpha;
+  DevBuf<cqx::PrimC32> d_primc32;
   // nuclear-gradient contraction: shell atoms, Cartesian densities,
   // accumulators and outputs
   DevBuf<int> d_shell_atom;
@@ -339,7 +341,8 @@ double class_pair_flops(int la, int lb, int lc, int ld, bool pure_d, double prim
 
 // primitive pairs of shells (a, b) in kernel orientation
 void make_prims(const cqx::Shell &A, const cqx::Shell &B, std::vector<PrimPair> &out,
-                std::vector<double> *bound, std::vector<double> *alpha_a = nullptr) {
+                std::vector<double> *bound, std::vector<double> *alpha_a = nullptr,
+                std::vector<cqx::PrimC32> *compact = nullptr) {
   double AB2 = 0;
   for (int d = 0; d < 3; ++d) AB2 += (A.origin[d] - B.origin[d]) * (A.origin[d] - B.origin[d]);
   for (int i = 0; i < A.nprim(); ++i)
@@ -354,6 +357,7 @@ void make_prims(const cqx::Shell &A, const cqx::Shell &B, std::vector<PrimPair> 
       pp.K = A.coef[i] * B.coef[j] * std::exp(-a * b / p * AB2);
       out.push_back(pp);
       if (alpha_a) alpha_a->push_back(a);
+      if (compact) compact->push_back({(float)p, (float)(b / p), pp.K});
       // |(pp|qq)| <= U_p U_q with U = sqrt(2) pi^(5/4) |K| p^(-5/4)
       if (bound) bound->push_back(std::sqrt(2.0) * std::pow(M_PI, 1.25) * std::fabs(pp.K) * std::pow(p, -1.25));
     }
@@ -413,6 +417,7 @@ PairRec make_pair_rec(const cqx::Basis &B, int a, int b, int s1, int s2, int pp_
   r.Q = 0.0;
   for (int d = 0; d < 3; ++d) r.AB[d] = B.shells[a].origin[d] - B.shells[b].origin[d];
   r.kk = B.shells[a].nprim() * B.shells[b].nprim();
+  for (int d = 0; d < 3; ++d) r.A[d] = B.shells[a].origin[d];
   return r;
 }
 
@@ -489,6 +494,8 @@ void build_pairs(cqx_ctx *c) {
   c->cls.assign(kNumClass, {});
   c->prims.clear();
   c->prim_alpha.clear();
+  c->primc32.clear();
+  std::vector<cqx::PrimC32> cmp;
   for (auto &pr : c->ref_pairs) {
     int s1 = pr.first, s2 = pr.second;
     int a = s1, b = s2;
@@ -496,12 +503,14 @@ void build_pairs(cqx_ctx *c) {
     tmp.clear();
     bnd.clear();
     alp.clear();
-    make_prims(B.shells[a], B.shells[b], tmp, &bnd, &alp);
+    cmp.clear();
+    make_prims(B.shells[a], B.shells[b], tmp, &bnd, &alp, &cmp);
     int off = (int)c->prims.size(), n = 0;
     for (size_t k = 0; k < tmp.size(); ++k)
       if (bnd[k] * umax >= c->prim_thresh) {
         c->prims.push_back(tmp[k]);
         c->prim_alpha.push_back(alp[k]);
+        c->primc32.push_back(cmp[k]);
         ++n;
       }
     c->cls[class_of(B.shells[a].l, B.shells[b].l)].push_back(make_pair_rec(B, a, b, s1, s2, off, n));
@@ -579,6 +588,7 @@ void upload_pairs(cqx_ctx *c) {
   c->d_seg.upload(segs, c->stream);
   c->d_prims.upload(c->prims, c->stream);
   c->d_prim_alpha.upload(c->prim_alpha, c->stream);
+  c->d_primc32.upload(c->primc32, c->stream);
   for (int k = 0; k < kNumClass; ++k) {
     c->d_pairs[k].upload(c->cls[k], c->stream);
     std::vector<cqx::PairScr> scr;
@@ -801,6 +811,7 @@ int ctx_merge(cqx_ctx *const *parts, int n, cqx_ctx **out) {
         }
       c->prims.insert(c->prims.end(), p->prims.begin(), p->prims.end());
       c->prim_alpha.insert(c->prim_alpha.end(), p->prim_alpha.begin(), p->prim_alpha.end());
+      c->primc32.insert(c->primc32.end(), p->primc32.begin(), p->primc32.end());
       prim0 = (int)c->prims.size();
     }
     for (int k = 0; k < kNumClass; ++k)
@@ -1638,6 +1649,7 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
         E.bra = c->d_pairs[cp.cb].p + i0;
         E.ket = c->d_pairs[cp.ck].p;
         E.prims = c->d_prims.p;
+        E.primc32 = c->d_primc32.p;
         E.rys = c->d_rys.p;
         E.rys32 = c->d_rys32.p;
         E.fp32 = f32 ? 1 : 0;
@@ -1803,6 +1815,7 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
             T.bra = c->d_pairs[tc.cb].p;
             T.ket = c->d_pairs[tc.ck].p;
             T.prims = c->d_prims.p;
+            T.primc32 = c->d_primc32.p;
             T.rys = c->d_rys.p;
             T.rys32 = c->d_rys32.p;
             T.fp32 = f;

@@ -21,6 +21,7 @@ struct PairRec {
   double AB[3];    // A - B
   int kk;          // Ka * Kb (contraction lengths, before primitive screening)
   int mol;         // molecule of a batched context (0 for one molecule)
+  double A[3];     // centre of shell a
 };
 
 // What the quartet screen reads of a shell pair, 16 bytes: shells s1, s2
@@ -38,6 +39,17 @@ struct PrimPair {
   double p;        // alpha_a + alpha_b
   double P[3];     // Gaussian product centre
   double PA[3];    // P - A
+  double K;        // c_a c_b exp(-alpha_a alpha_b / p |AB|^2)
+};
+
+// A primitive pair as the FP32 lists read it, in one 16-byte load: with b
+// the exponent of shell b, P - A = -(b / p) AB follows from the shell pair's
+// AB, and P - Q = (A - C) + (P - A) - (Q - C) from the quartet's A - C, so
+// neither centre is stored (K stays FP64: products of tiny prefactors stay
+// representable).  Same indexing as PrimPair.
+struct alignas(16) PrimC32 {
+  float p;         // alpha_a + alpha_b
+  float bp;        // alpha_b / p
   double K;        // c_a c_b exp(-alpha_a alpha_b / p |AB|^2)
 };
 

@@ -10,6 +10,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "digest.hpp"
 #include "digest_tile.hpp"
 #include "kernels.hpp"
@@ -172,12 +174,20 @@ __device__ __forceinline__ const R *rys_section(const double *r64, const float *
 // combined by a butterfly of lane shuffles, so every lane of the group ends
 // with the whole quartet in I (unscaled).  R is the evaluation precision.
 template <int LA, int LB, int LC, int LD, int G, class R>
-__device__ __forceinline__ void eval_quartet(const PrimPair *__restrict__ prims, const R *__restrict__ rysn,
-                                             const PairRec &bp, const PairRec &kp, int g,
-                                             R (&I)[Cls<LA, LB, LC, LD>::NC]) {
+__device__ __forceinline__ void eval_quartet(const PrimPair *__restrict__ prims, const PrimC32 *__restrict__ pc32,
+                                             const R *__restrict__ rysn, const PairRec &bp, const PairRec &kp,
+                                             int g, R (&I)[Cls<LA, LB, LC, LD>::NC]) {
   using K = Cls<LA, LB, LC, LD>;
+  constexpr bool F32 = std::is_same<R, float>::value;
 #pragma unroll
   for (int o = 0; o < K::NC; ++o) I[o] = 0;
+  R AB[3], CD[3], AC[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    AB[d] = (R)bp.AB[d];
+    CD[d] = (R)kp.AB[d];
+    AC[d] = F32 ? (R)(bp.A[d] - kp.A[d]) : (R)0;  // A - C in FP64 before the rounding
+  }
   const int nk = kp.npp, ntot = bp.npp * kp.npp;
   int i = 0, j = g;
   while (nk > 0 && j >= nk) {
@@ -185,19 +195,42 @@ __device__ __forceinline__ void eval_quartet(const PrimPair *__restrict__ prims,
     ++i;
   }
   for (int k = g; k < ntot; k += G) {
-    const PrimPair A = prims[bp.pp_off + i];
-    const PrimPair B = prims[kp.pp_off + j];
+    R ap, bq, PA[3], QC[3], PQ[3];
+    double kk;
+    if constexpr (F32) {
+      // 16-byte records: P - A = -(b/p) AB, Q - C = -(d/q) CD
+      const PrimC32 A = pc32[bp.pp_off + i];
+      const PrimC32 B = pc32[kp.pp_off + j];
+      ap = A.p;
+      bq = B.p;
+      kk = A.K * B.K;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        PA[d] = -A.bp * AB[d];
+        QC[d] = -B.bp * CD[d];
+        PQ[d] = AC[d] + PA[d] - QC[d];
+      }
+    } else {
+      const PrimPair A = prims[bp.pp_off + i];
+      const PrimPair B = prims[kp.pp_off + j];
+      ap = A.p;
+      bq = B.p;
+      kk = A.K * B.K;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        PQ[d] = A.P[d] - B.P[d];
+        PA[d] = A.PA[d];
+        QC[d] = B.PA[d];
+      }
+    }
     j += G;
     while (j >= nk) {
       j -= nk;
       ++i;
     }
-    const R ap = (R)A.p, bq = (R)B.p;
-    const PrimRecip<R> pr = prim_recip<R>(ap, bq, A.K * B.K);
+    const PrimRecip<R> pr = prim_recip<R>(ap, bq, kk);
     const R rpq = pr.rpq, pref = pr.pref;
     const R rho = ap * bq * rpq;
-    // centre differences in FP64 before any rounding to R
-    const R PQ[3] = {(R)(A.P[0] - B.P[0]), (R)(A.P[1] - B.P[1]), (R)(A.P[2] - B.P[2])};
     const R T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
     R x[K::NR], w[K::NR];
     rys_roots_sec<K::NR, R>(T, rysn, x, w);
@@ -217,9 +250,9 @@ __device__ __forceinline__ void eval_quartet(const PrimPair *__restrict__ prims,
         R H[3][K::NH];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          const R C00 = (R)A.PA[d] - bq * rpq * t2 * PQ[d];
-          const R D00 = (R)B.PA[d] + ap * rpq * t2 * PQ[d];
-          rys_2d_full<LA, LB, LC, LD, R>(C00, D00, B10, B01, B00, (R)bp.AB[d], (R)kp.AB[d],
+          const R C00 = PA[d] - bq * rpq * t2 * PQ[d];
+          const R D00 = QC[d] + ap * rpq * t2 * PQ[d];
+          rys_2d_full<LA, LB, LC, LD, R>(C00, D00, B10, B01, B00, AB[d], CD[d],
                                          d == 2 ? pref * w[r] : (R)1, H[d]);
         }
         constexpr HTab<LA, LB, LC, LD> HT = make_htab<LA, LB, LC, LD>();
@@ -277,7 +310,7 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
     kp = a.ket[qt.ket_mask & 0xFFFFFFu];
   }
   R I[K::NC];
-  eval_quartet<LA, LB, LC, LD, G, R>(a.prims, rysn, bp, kp, g, I);
+  eval_quartet<LA, LB, LC, LD, G, R>(a.prims, a.primc32, rysn, bp, kp, g, I);
   const double scale = quartet_scale(bp, kp);
   double *sI = s_I[wv] + qw * K::NC;
 #pragma unroll

@@ -165,7 +165,7 @@ __global__ __launch_bounds__(256) void eri_tile_kernel(TileEriLaunch a) {
       kp = a.ket[a.kmem[km0 + (loc & 7)]];
     }
     R I[K::NC];
-    eval_quartet<LA, LB, LC, LD, G, R>(a.prims, rysn, bp, kp, g, I);
+    eval_quartet<LA, LB, LC, LD, G, R>(a.prims, a.primc32, rysn, bp, kp, g, I);
     const double scale = quartet_scale(bp, kp);
     double *sI = s_I[wv] + qw * K::NC;
 #pragma unroll

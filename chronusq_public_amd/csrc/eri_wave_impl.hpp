@@ -130,14 +130,38 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
       const PairRec &kp = s_pr[wv][t][1];
       const int loc = kq - off[t], nk = kp.npp;
       const int i = loc / nk, j = loc - i * nk;
-      const PrimPair A = a.prims[bp.pp_off + i];
-      const PrimPair B = a.prims[kp.pp_off + j];
-      const R ap = (R)A.p, bq = (R)B.p;
-      const PrimRecip<R> pr = prim_recip<R>(ap, bq, A.K * B.K);
+      R ap, bq, PQ[3], PAv[3], QCv[3];
+      double kk;
+      if constexpr (std::is_same<R, float>::value) {
+        // 16-byte records: P - A = -(b/p) AB, Q - C = -(d/q) CD,
+        // P - Q = (A - C) + (P - A) - (Q - C) with A - C in FP64 before the rounding
+        const PrimC32 A = a.primc32[bp.pp_off + i];
+        const PrimC32 B = a.primc32[kp.pp_off + j];
+        ap = A.p;
+        bq = B.p;
+        kk = A.K * B.K;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          PAv[d] = -A.bp * (R)bp.AB[d];
+          QCv[d] = -B.bp * (R)kp.AB[d];
+          PQ[d] = (R)(bp.A[d] - kp.A[d]) + PAv[d] - QCv[d];
+        }
+      } else {
+        const PrimPair A = a.prims[bp.pp_off + i];
+        const PrimPair B = a.prims[kp.pp_off + j];
+        ap = A.p;
+        bq = B.p;
+        kk = A.K * B.K;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          PQ[d] = A.P[d] - B.P[d];
+          PAv[d] = A.PA[d];
+          QCv[d] = B.PA[d];
+        }
+      }
+      const PrimRecip<R> pr = prim_recip<R>(ap, bq, kk);
       const R rpq = pr.rpq, pref = pr.pref;
       const R rho = ap * bq * rpq;
-      // centre differences in FP64 before any rounding to R
-      const R PQ[3] = {(R)(A.P[0] - B.P[0]), (R)(A.P[1] - B.P[1]), (R)(A.P[2] - B.P[2])};
       const R T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
       R t2, wr;
       rys_root<K::NR, R>(T, rys, my_r, t2, wr);
@@ -146,8 +170,8 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
       const R B10 = (R)0.5 * rp * ((R)1 - rho * rp * t2);
       const R B01 = (R)0.5 * rq * ((R)1 - rho * rq * t2);
       const R PQd = my_d == 0 ? PQ[0] : (my_d == 1 ? PQ[1] : PQ[2]);
-      const R PAd = (R)(my_d == 0 ? A.PA[0] : (my_d == 1 ? A.PA[1] : A.PA[2]));
-      const R QCd = (R)(my_d == 0 ? B.PA[0] : (my_d == 1 ? B.PA[1] : B.PA[2]));
+      const R PAd = my_d == 0 ? PAv[0] : (my_d == 1 ? PAv[1] : PAv[2]);
+      const R QCd = my_d == 0 ? QCv[0] : (my_d == 1 ? QCv[1] : QCv[2]);
       const R ABd = (R)(my_d == 0 ? bp.AB[0] : (my_d == 1 ? bp.AB[1] : bp.AB[2]));
       const R CDd = (R)(my_d == 0 ? kp.AB[0] : (my_d == 1 ? kp.AB[1] : kp.AB[2]));
       const R C00 = PAd - bq * rpq * t2 * PQd;

@@ -21,6 +21,7 @@ struct EriLaunch {
   long long nq_base;
   const PairRec *bra, *ket;
   const PrimPair *prims;
+  const PrimC32 *primc32;        // the same pairs for the FP32 lists of the templated kernels
   const double *rys;
   const float *rys32;            // the same table rounded to FP32
   int fp32;                      // evaluate the quartets in FP32 (digestion stays FP64)
@@ -164,6 +165,7 @@ struct TileEriLaunch {
   const int *bmem, *kmem;
   const PairRec *bra, *ket;
   const PrimPair *prims;
+  const PrimC32 *primc32;
   const double *rys;
   const float *rys32;
   int fp32;
