@@ -26,6 +26,7 @@ BASIS_FILES = {
     "6-31G*": "6-31gs.gbs",
     "CC-PVDZ": "cc-pvdz.gbs",
     "CC-PVTZ": "cc-pvtz.gbs",
+    "CC-PVQZ": "cc-pvqz.gbs",
 }
 
 

@@ -183,7 +183,10 @@ static void renormalise(Shell &sh) {
 
 Basis Basis::from_g94(const std::string &text, const std::vector<int> &Z,
                       const std::vector<double> &xyz, bool force_cart) {
-  struct Rec { int z; std::vector<Shell> sh; };
+  // bad: the first shell of the record beyond kMaxL (reported only when an
+  // atom of the element is asked for; basis files carry heavier elements
+  // with higher shells)
+  struct Rec { int z; std::vector<Shell> sh; std::string bad; };
   std::vector<Rec> recs;
   std::istringstream in(text);
   std::string line;
@@ -208,7 +211,7 @@ Basis Basis::from_g94(const std::string &text, const std::vector<int> &Z,
     auto t = tokens(s);
     if (t.empty()) continue;
     if (new_record) {
-      recs.push_back({element_number(t[0]), {}});
+      recs.push_back({element_number(t[0]), {}, {}});
       new_record = false;
       continue;
     }
@@ -216,8 +219,7 @@ Basis Basis::from_g94(const std::string &text, const std::vector<int> &Z,
     bool sp = (t[0] == "SP" || t[0] == "sp");
     int depth = std::stoi(t[1]);
     int L = sp ? 0 : l_of_symbol(t[0]);
-    if (L < 0 || L > kMaxL || depth <= 0)
-      throw std::runtime_error("G94: unsupported shell " + t[0]);
+    if (L < 0 || depth <= 0) throw std::runtime_error("G94: unsupported shell " + t[0]);
     Shell a, p;
     for (int i = 0; i < depth; ++i) {
       std::string pl;
@@ -230,6 +232,10 @@ Basis Basis::from_g94(const std::string &text, const std::vector<int> &Z,
     }
     a.l = L;
     a.pure = (L > 1) && !force_cart;
+    if (L > kMaxL) {
+      if (recs.back().bad.empty()) recs.back().bad = t[0];
+      continue;
+    }
     recs.back().sh.push_back(a);
     if (sp) { p.l = 1; p.pure = false; recs.back().sh.push_back(p); }
   }
@@ -239,6 +245,7 @@ Basis Basis::from_g94(const std::string &text, const std::vector<int> &Z,
     for (auto &x : recs)
       if (x.z == Z[at]) r = &x;  // a later record for the same element wins
     if (!r) throw std::runtime_error("Cannot find Z=" + std::to_string(Z[at]) + " in Basis Definition");
+    if (!r->bad.empty()) throw std::runtime_error("G94: unsupported shell " + r->bad);
     for (auto sh : r->sh) {
       for (int d = 0; d < 3; ++d) sh.origin[d] = xyz[3 * at + d];
       sh.atom = (int)at;

@@ -75,7 +75,7 @@ using cqx::Quartet;
 
 namespace {
 
-constexpr int kNumClass = 10;  // (la, lb), la >= lb, la <= 3
+constexpr int kNumClass = (cqx::kMaxLDev + 1) * (cqx::kMaxLDev + 2) / 2;  // (la, lb), la >= lb
 inline int class_of(int la, int lb) { return la * (la + 1) / 2 + lb; }
 inline void class_ls(int c, int &la, int &lb) {
   la = 0;
@@ -435,7 +435,7 @@ void setup_device_maps(cqx_ctx *c) {
     sh_nf.push_back(sh.nfunc());
   }
   std::vector<double> sph;
-  for (int l = 0; l <= 3; ++l) {
+  for (int l = 0; l <= cqx::kMaxLDev; ++l) {
     auto T = cqx::solid_harmonics(l);
     sph.insert(sph.end(), T.begin(), T.end());
   }
@@ -947,7 +947,7 @@ int cqx_basis_shell_prims(const cqx_basis *b, int s, double *alpha, double *coef
 int cqx_ctx_create(const cqx_basis *b, double thresh, int device, cqx_ctx **out) {
   if (!b || !out) return fail(nullptr, CQX_ERR_ARG, "cqx_ctx_create: bad arguments");
   if (b->b.max_l > cqx::kMaxLDev)
-    return fail(nullptr, CQX_ERR_UNSUPPORTED, "device kernels cover shells up to l = 3");
+    return fail(nullptr, CQX_ERR_UNSUPPORTED, "device kernels cover shells up to l = 4 (g)");
   auto c = std::make_unique<cqx_ctx>();
   try {
     init_ctx(c.get(), b->b, thresh, device);
@@ -1663,8 +1663,9 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
       // per quartet of a lane-group kernel, the quartets per wave of a wave
       // kernel
       auto tune_key = [&](const cqx::EriLaunch &E) {
-        const int k = ((((E.la * 4 + E.lb) * 4 + E.lc) * 4 + E.ld) << 1) | (E.fp32 ? 1 : 0);
-        return route(E.la, E.lb, E.lc, E.ld) == 1 ? k + 4096 : k;
+        constexpr int B = cqx::kMaxLDev + 1;
+        const int k = ((((E.la * B + E.lb) * B + E.lc) * B + E.ld) << 1) | (E.fp32 ? 1 : 0);
+        return route(E.la, E.lb, E.lc, E.ld) == 1 ? k + 2 * B * B * B * B : k;
       };
       auto apply_tuned = [&](cqx::EriLaunch &E) {
         const int r = route(E.la, E.lb, E.lc, E.ld);

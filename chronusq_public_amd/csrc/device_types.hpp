@@ -5,7 +5,7 @@
 
 namespace cqx {
 
-constexpr int kMaxLDev = 3;    // device kernels cover s..f shells
+constexpr int kMaxLDev = 4;    // device kernels cover s..g shells
 constexpr int kMaxPlanes = 16; // real accumulation planes per launch
 constexpr int kMaxMat = 8;     // contractions sharing one ERI evaluation
 

@@ -46,6 +46,12 @@ __device__ __forceinline__ void atomic_add(double *p, double v) { unsafeAtomicAd
 // (r, dir) < 3 NR build the 2D integrals I_dir(ia, ib, ic, id) (vertical
 // recursion on the Rys variables, then horizontal transfer to b and d) into
 // LDS, and every lane accumulates its share of the Cartesian quartet in LDS.
+// Quartets too large for LDS (classes with g shells) are built in passes
+// over ranges of the first shell's components (a is the slowest index of
+// the quartet layout, so a range is a smaller quartet with a's first
+// function shifted) and with the 2D tables of a few roots at a time; each
+// pass is digested (or stored) on its own, since every J/K update is a sum
+// of terms linear in the integrals.
 // ---------------------------------------------------------------------------
 struct TeamDims {
   int la, lb, lc, ld, nca, ncb, ncc, ncd, nc, ni, nk, nr, nh;
@@ -63,12 +69,14 @@ __device__ TeamDims team_dims(int la, int lb, int lc, int ld) {
   return d;
 }
 
+constexpr int kTeamMaxI = 2 * kMaxLDev + 1;  // la + lb + 1
+
 // 2D integrals of one Rys root along one direction, written to H[nh].
 __device__ void rys_2d(const TeamDims &D, double t2, double p, double q, double PAd,
                        double QCd, double PQd, double ABd, double CDd, double scale,
                        double *__restrict__ H) {
-  double G[7][7];   // [i <= la+lb][k <= lc+ld]
-  double E[7][4][7];  // bra transfer: [i][j][k]
+  double G[kTeamMaxI][kTeamMaxI];                 // [i <= la+lb][k <= lc+ld]
+  double E[kTeamMaxI][kMaxLDev + 1][kTeamMaxI];   // bra transfer: [i][j][k]
   const double pq = p + q, rho = p * q / pq;
   const double B00 = 0.5 * t2 / pq;
   const double B10 = 0.5 / p * (1.0 - rho / p * t2);
@@ -92,7 +100,7 @@ __device__ void rys_2d(const TeamDims &D, double t2, double p, double q, double 
   // horizontal transfer on the ket for every (ia, ib)
   for (int ia = 0; ia <= D.la; ++ia)
     for (int ib = 0; ib <= D.lb; ++ib) {
-      double F[7][4];
+      double F[kTeamMaxI][kMaxLDev + 1];
       for (int k = 0; k < D.nk; ++k) F[k][0] = E[ia][ib][k];
       for (int l = 1; l <= D.ld; ++l)
         for (int k = 0; k + l < D.nk; ++k) F[k][l] = F[k + 1][l - 1] + CDd * F[k][l - 1];
@@ -102,14 +110,23 @@ __device__ void rys_2d(const TeamDims &D, double t2, double p, double q, double 
     }
 }
 
+// LDS of the generic kernel, in doubles: Rys nodes and weights of up to 16
+// roots, the 2D tables of rpp roots and the quartet components of ca
+// components of the first shell
+__host__ __device__ constexpr int team_lds_doubles(int rpp, int nh, int ca, int nbcd) {
+  return 32 + rpp * 3 * nh + ca * nbcd;
+}
+
 __global__ __launch_bounds__(64) void eri_team_kernel(EriLaunch a) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x;
   const TeamDims D = team_dims(a.la, a.lb, a.lc, a.ld);
+  const int CA = a.team_ca, RPP = a.team_rpp;  // first-shell components per pass, roots per table pass
+  const int nbcd = D.ncb * D.ncc * D.ncd;
   double *s_x = lds;
-  double *s_w = lds + 8;
-  double *s_H = lds + 16;
-  double *s_I = s_H + D.nr * 3 * D.nh;
+  double *s_w = lds + 16;
+  double *s_H = lds + 32;
+  double *s_I = s_H + RPP * 3 * D.nh;
   // quartet count: the list's device counter for a launch sized to the
   // list's capacity before the list was built
   const long long nq = a.nq_dev ? min((long long)*a.nq_dev - a.nq_base, a.nq) : a.nq;
@@ -117,69 +134,77 @@ __global__ __launch_bounds__(64) void eri_team_kernel(EriLaunch a) {
     const Quartet qt = a.quartets[q];
     const PairRec bp = a.bra[qt.bra];
     const PairRec kp = a.ket[qt.ket_mask & 0xFFFFFFu];
-    for (int o = lane; o < D.nc; o += 64) s_I[o] = 0.0;
-    for (int i = 0; i < bp.npp; ++i) {
-      const PrimPair A = a.prims[bp.pp_off + i];
-      for (int j = 0; j < kp.npp; ++j) {
-        const PrimPair B = a.prims[kp.pp_off + j];
-        const double pq = A.p + B.p, rho = A.p * B.p / pq;
-        const double PQ[3] = {A.P[0] - B.P[0], A.P[1] - B.P[1], A.P[2] - B.P[2]};
-        const double T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
-        const double pref = A.K * B.K * kTwoPi52 / (A.p * B.p * sqrt(pq));
-        __syncthreads();
-        if (lane < D.nr) {
-          double x, w;
-          rys_root_rt(D.nr, lane, T, a.rys, x, w);
-          s_x[lane] = x;
-          s_w[lane] = w * pref;
-        }
-        __syncthreads();
-        if (lane < 3 * D.nr) {
-          const int r = lane / 3, dir = lane % 3;
-          rys_2d(D, s_x[r], A.p, B.p, A.PA[dir], B.PA[dir], PQ[dir], bp.AB[dir], kp.AB[dir],
-                 dir == 2 ? s_w[r] : 1.0, s_H + (r * 3 + dir) * D.nh);
-        }
-        __syncthreads();
-        for (int o = lane; o < D.nc; o += 64) {
-          int rem = o;
-          const int id = rem % D.ncd; rem /= D.ncd;
-          const int ic = rem % D.ncc; rem /= D.ncc;
-          const int ib = rem % D.ncb; rem /= D.ncb;
-          const int ia = rem;
-          int e[4][3];
-          cart_exp(D.la, ia, e[0][0], e[0][1], e[0][2]);
-          cart_exp(D.lb, ib, e[1][0], e[1][1], e[1][2]);
-          cart_exp(D.lc, ic, e[2][0], e[2][1], e[2][2]);
-          cart_exp(D.ld, id, e[3][0], e[3][1], e[3][2]);
-          int h[3];
-          for (int dir = 0; dir < 3; ++dir)
-            h[dir] = ((e[0][dir] * (D.lb + 1) + e[1][dir]) * (D.lc + 1) + e[2][dir]) * (D.ld + 1) +
-                     e[3][dir];
-          double s = 0.0;
-          for (int r = 0; r < D.nr; ++r) {
-            const double *Hr = s_H + r * 3 * D.nh;
-            s += Hr[h[0]] * Hr[D.nh + h[1]] * Hr[2 * D.nh + h[2]];
+    for (int ia0 = 0; ia0 < D.nca; ia0 += CA) {
+      const int na = D.nca - ia0 < CA ? D.nca - ia0 : CA;
+      const int nsub = na * nbcd;
+      for (int o = lane; o < nsub; o += 64) s_I[o] = 0.0;
+      for (int i = 0; i < bp.npp; ++i) {
+        const PrimPair A = a.prims[bp.pp_off + i];
+        for (int j = 0; j < kp.npp; ++j) {
+          const PrimPair B = a.prims[kp.pp_off + j];
+          const double pq = A.p + B.p, rho = A.p * B.p / pq;
+          const double PQ[3] = {A.P[0] - B.P[0], A.P[1] - B.P[1], A.P[2] - B.P[2]};
+          const double T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+          const double pref = A.K * B.K * kTwoPi52 / (A.p * B.p * sqrt(pq));
+          __syncthreads();
+          if (lane < D.nr) {
+            double x, w;
+            rys_root_rt(D.nr, lane, T, a.rys, x, w);
+            s_x[lane] = x;
+            s_w[lane] = w * pref;
           }
-          s_I[o] += s;
+          for (int r0 = 0; r0 < D.nr; r0 += RPP) {
+            const int nrp = D.nr - r0 < RPP ? D.nr - r0 : RPP;
+            __syncthreads();
+            if (lane < 3 * nrp) {
+              const int lr = lane / 3, dir = lane % 3, r = r0 + lr;
+              rys_2d(D, s_x[r], A.p, B.p, A.PA[dir], B.PA[dir], PQ[dir], bp.AB[dir], kp.AB[dir],
+                     dir == 2 ? s_w[r] : 1.0, s_H + (lr * 3 + dir) * D.nh);
+            }
+            __syncthreads();
+            for (int o = lane; o < nsub; o += 64) {
+              int rem = o;
+              const int id = rem % D.ncd; rem /= D.ncd;
+              const int ic = rem % D.ncc; rem /= D.ncc;
+              const int ib = rem % D.ncb; rem /= D.ncb;
+              const int ia = ia0 + rem;
+              int e[4][3];
+              cart_exp(D.la, ia, e[0][0], e[0][1], e[0][2]);
+              cart_exp(D.lb, ib, e[1][0], e[1][1], e[1][2]);
+              cart_exp(D.lc, ic, e[2][0], e[2][1], e[2][2]);
+              cart_exp(D.ld, id, e[3][0], e[3][1], e[3][2]);
+              int h[3];
+              for (int dir = 0; dir < 3; ++dir)
+                h[dir] = ((e[0][dir] * (D.lb + 1) + e[1][dir]) * (D.lc + 1) + e[2][dir]) * (D.ld + 1) +
+                         e[3][dir];
+              double sum = 0.0;
+              for (int lr = 0; lr < nrp; ++lr) {
+                const double *Hr = s_H + lr * 3 * D.nh;
+                sum += Hr[h[0]] * Hr[D.nh + h[1]] * Hr[2 * D.nh + h[2]];
+              }
+              s_I[o] += sum;
+            }
+          }
         }
       }
-    }
-    __syncthreads();
-    if (a.store_mode) {
-      double *out = a.store + a.store_off[q];
-      for (int o = lane; o < D.nc; o += 64) out[o] = s_I[o];
-    } else {
-      // degeneracy of the quartet in the 8-fold loop (direct.hpp:1664-1667,
-      // 1793-1802), times the 1/2 of direct.hpp:1849-1850
-      const bool same_pair = (bp.a == kp.a && bp.b == kp.b);
-      const double deg = (bp.a == bp.b ? 1.0 : 2.0) * (kp.a == kp.b ? 1.0 : 2.0) * (same_pair ? 1.0 : 2.0);
-      for (int o = lane; o < D.nc; o += 64) s_I[o] *= 0.5 * deg;
       __syncthreads();
-      const int n[4] = {D.nca, D.ncb, D.ncc, D.ncd};
-      const int bf[4] = {bp.bfa, bp.bfb, kp.bfa, kp.bfb};
-      digest_wave(s_I, n, bf, a.planes, qt.ket_mask >> 24, bp.mol, lane, 64);
+      if (a.store_mode) {
+        double *out = a.store + a.store_off[q] + (size_t)ia0 * nbcd;
+        for (int o = lane; o < nsub; o += 64) out[o] = s_I[o];
+      } else {
+        // degeneracy of the quartet in the 8-fold loop (direct.hpp:1664-1667,
+        // 1793-1802), times the 1/2 of direct.hpp:1849-1850
+        const bool same_pair = (bp.a == kp.a && bp.b == kp.b);
+        const double deg = (bp.a == bp.b ? 1.0 : 2.0) * (kp.a == kp.b ? 1.0 : 2.0) * (same_pair ? 1.0 : 2.0);
+        for (int o = lane; o < nsub; o += 64) s_I[o] *= 0.5 * deg;
+        __syncthreads();
+        // the pass is the quartet of a's components ia0 .. ia0 + na - 1
+        const int n[4] = {na, D.ncb, D.ncc, D.ncd};
+        const int bf[4] = {bp.bfa + ia0, bp.bfb, kp.bfa, kp.bfb};
+        digest_wave(s_I, n, bf, a.planes, qt.ket_mask >> 24, bp.mol, lane, 64);
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
@@ -653,14 +678,31 @@ inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / 
 
 }  // namespace
 
-hipError_t launch_eri_team(const EriLaunch &a, hipStream_t s) {
-  if (a.nq <= 0) return hipSuccess;
-  const int nr = (a.la + a.lb + a.lc + a.ld) / 2 + 1;
+bool team_shape(int la, int lb, int lc, int ld, int lds_max_bytes, int &ca, int &rpp) {
+  const int nr = (la + lb + lc + ld) / 2 + 1;
+  const int nh = (la + 1) * (lb + 1) * (lc + 1) * (ld + 1);
+  auto nc = [](int l) { return (l + 1) * (l + 2) / 2; };
+  const int nca = nc(la), nbcd = nc(lb) * nc(lc) * nc(ld);
+  const int budget = lds_max_bytes / (int)sizeof(double);
+  // the most first-shell components per pass (fewest passes over the
+  // primitives), then the most roots per table pass that still fit
+  for (ca = nca; ca >= 1; --ca) {
+    if (team_lds_doubles(1, nh, ca, nbcd) > budget) continue;
+    rpp = nr;
+    while (rpp > 1 && team_lds_doubles(rpp, nh, ca, nbcd) > budget) --rpp;
+    return true;
+  }
+  return false;
+}
+
+hipError_t launch_eri_team(const EriLaunch &a0, hipStream_t s) {
+  if (a0.nq <= 0) return hipSuccess;
+  if (a0.la > kMaxLDev || a0.lb > kMaxLDev || a0.lc > kMaxLDev || a0.ld > kMaxLDev) return hipErrorInvalidValue;
+  EriLaunch a = a0;
+  if (!team_shape(a.la, a.lb, a.lc, a.ld, 160 * 1024, a.team_ca, a.team_rpp)) return hipErrorInvalidValue;
   const int nh = (a.la + 1) * (a.lb + 1) * (a.lc + 1) * (a.ld + 1);
   auto nc = [](int l) { return (l + 1) * (l + 2) / 2; };
-  const int ncq = nc(a.la) * nc(a.lb) * nc(a.lc) * nc(a.ld);
-  const size_t lds = sizeof(double) * (16 + (size_t)nr * 3 * nh + ncq);
-  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  const size_t lds = sizeof(double) * (size_t)team_lds_doubles(a.team_rpp, nh, a.team_ca, nc(a.lb) * nc(a.lc) * nc(a.ld));
   // once per process (contexts on several host threads launch concurrently)
   static std::once_flag attr_once;
   static hipError_t attr_err = hipSuccess;

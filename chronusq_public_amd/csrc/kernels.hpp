@@ -29,14 +29,19 @@ struct EriLaunch {
   int force_g;                   // lanes per quartet of the lane-group kernels (0: chosen per launch)
   int force_qt;                  // quartets per wave of the wave kernels (0: the class default)
   int rys_lds;                   // lane-group kernels: stage the class's Rys table section in LDS
+  int team_ca, team_rpp;         // generic kernel: first-shell components per pass, roots per table pass (set by its launcher)
   PlaneSet planes;
   double *store;                 // STORE mode: contracted Cartesian quartets
   const long long *store_off;    // per quartet offset into store
   int store_mode;
 };
 
-// Generic one-wave-per-quartet kernel (any class up to (ff|ff)).
+// Generic one-wave-per-quartet kernel (any class up to (gg|gg)).
 hipError_t launch_eri_team(const EriLaunch &a, hipStream_t s);
+// Passes of the generic kernel within lds_max_bytes of LDS: first-shell
+// components per pass (ca) and roots per 2D-table pass (rpp); false when
+// not even one of each fits
+bool team_shape(int la, int lb, int lc, int ld, int lds_max_bytes, int &ca, int &rpp);
 // Fast thread-per-quartet kernels for the low classes; returns
 // hipErrorNotSupported when the class has no specialised kernel.
 hipError_t launch_eri_fast(const EriLaunch &a, hipStream_t s);

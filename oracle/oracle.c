@@ -490,6 +490,7 @@ typedef struct {
   double *V;    /* [nE][nF][Lt+1] */
   double *C;    /* contracted [e in la..Le][f in lc..Lf] */
   double *H1, *H2, *H3, *T1, *T2; /* HRR / transform scratch */
+  double *O;    /* a quartet in the evaluation order (eri_shell) */
 } eri_work;
 
 static size_t work_elems(int maxl) {
@@ -510,9 +511,10 @@ static void work_alloc(eri_work *w, int maxl) {
   w->H3 = malloc(sizeof(double) * big);
   w->T1 = malloc(sizeof(double) * big);
   w->T2 = malloc(sizeof(double) * big);
+  w->O = malloc(sizeof(double) * nc * nc * nc * nc);
 }
 static void work_free(eri_work *w) {
-  free(w->V); free(w->C); free(w->H1); free(w->H2); free(w->H3); free(w->T1); free(w->T2);
+  free(w->V); free(w->C); free(w->H1); free(w->H2); free(w->H3); free(w->T1); free(w->T2); free(w->O);
 }
 
 static inline int first_dir(ijk_t e) { return e.x ? 0 : (e.y ? 1 : 2); }
@@ -672,9 +674,27 @@ static int eri_shell_p(const oshell *A, const oshell *B, const oshell *Cc, const
   return n;
 }
 
+/* The HRR moves angular momentum from the first shell of a pair onto the
+ * second over the distance between them; moving the smaller one loses less
+ * to cancellation (a g shell second in a pair costs 1e-10 relative at O-H
+ * distances), so each pair is evaluated with its higher-l shell first and
+ * the quartet is transposed back to the order asked for. */
 static int eri_shell(const oracle_basis *bs, int s1, int s2, int s3, int s4,
                      double *out, eri_work *w) {
-  return eri_shell_p(&bs->sh[s1], &bs->sh[s2], &bs->sh[s3], &bs->sh[s4], out, w);
+  const int swb = bs->sh[s1].l < bs->sh[s2].l, swk = bs->sh[s3].l < bs->sh[s4].l;
+  if (!swb && !swk) return eri_shell_p(&bs->sh[s1], &bs->sh[s2], &bs->sh[s3], &bs->sh[s4], out, w);
+  const int a = swb ? s2 : s1, b = swb ? s1 : s2, c = swk ? s4 : s3, d = swk ? s3 : s4;
+  const int n = eri_shell_p(&bs->sh[a], &bs->sh[b], &bs->sh[c], &bs->sh[d], w->O, w);
+  const int n1 = bs->sh[s1].nfunc, n2 = bs->sh[s2].nfunc, n3 = bs->sh[s3].nfunc, n4 = bs->sh[s4].nfunc;
+  const int nb = bs->sh[b].nfunc, nc = bs->sh[c].nfunc, nd = bs->sh[d].nfunc;
+  for (int i1 = 0; i1 < n1; i1++)
+    for (int i2 = 0; i2 < n2; i2++)
+      for (int i3 = 0; i3 < n3; i3++)
+        for (int i4 = 0; i4 < n4; i4++) {
+          const int ia = swb ? i2 : i1, ib = swb ? i1 : i2, ic = swk ? i4 : i3, id = swk ? i3 : i4;
+          out[(((size_t)i1 * n2 + i2) * n3 + i3) * n4 + i4] = w->O[(((size_t)ia * nb + ib) * nc + ic) * nd + id];
+        }
+  return n;
 }
 
 int oracle_eri_quartet(const oracle_basis *b, int s1, int s2, int s3, int s4,
@@ -694,7 +714,9 @@ void oracle_eri_full(const oracle_basis *b, double *out) {
   {
     eri_work w;
     work_alloc(&w, b->maxl);
-    double *buf = malloc(sizeof(double) * 20000);
+    size_t mx = 1;
+    for (int i = 0; i < 4; i++) mx *= NCART(b->maxl);
+    double *buf = malloc(sizeof(double) * mx);
 #pragma omp for schedule(dynamic) collapse(2)
     for (int s1 = 0; s1 < ns; s1++)
       for (int s2 = 0; s2 < ns; s2++)

@@ -100,8 +100,10 @@ def test_ssss_closed_form():
 
 
 def test_eri_permutational_symmetry_through_f():
-    """(12|34) = (21|34) = (12|43) = (34|12): the four index orders run
-    the recursions along different paths (bra/ket VRR, HRR direction)."""
+    """(12|34) = (21|34) = (12|43) = (34|12): the bra-ket swap runs the
+    recursions along different paths (bra/ket VRR), and so do the swaps
+    within a pair of equal l (HRR direction); a pair of unequal l is always
+    evaluated with its higher l first."""
     from chronusq_public_amd.chem import WATER, Molecule, basis_text
     mol = Molecule.from_angstrom(WATER)
     ob = OracleBasis(basis_text("cc-pVTZ"), mol.Z, mol.xyz)
