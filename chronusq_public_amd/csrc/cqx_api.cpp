@@ -234,15 +234,13 @@ struct cqx_ctx {
   DevBuf<cqx::ListLaunch> d_glists;
   DevBuf<long long> d_glstart, d_gidx, d_gchk;
 
-  // Tiles (option "tiles"): the lane-group class pairs of a build whose
-  // planes are all Hermitian go through tile lists instead of quartet
-  // lists; 2 (the default) does so only when the build has at most one
-  // Hermitian J and one Hermitian K plane (a real RHF J / K pair), where
-  // summing the shared targets pays; with the eight exchange planes of a
-  // two-component build the quartet lists are faster.  Per class, its
-  // member array (pair indices grouped by molecule and first shell); per
-  // tile class pair, its sub-blocks, a range of d_sub.
-  int use_tiles = 2;
+  // Tiles (option "tiles", default 0): the lane-group class pairs of a
+  // build whose planes are all Hermitian go through tile lists instead of
+  // quartet lists (1); 2 does so only when the build has at most one
+  // Hermitian J and one Hermitian K plane (a real RHF J / K pair).  Per
+  // class, its member array (pair indices grouped by molecule and first
+  // shell); per tile class pair, its sub-blocks, a range of d_sub.
+  int use_tiles = 0;
   // lane-group classes that take tiles (bit k: k-th of ss|ss, ps|ss, ps|ps,
   // pp|ss, pp|ps, ds|ss, ds|ps, dp|ss); (ss|ss) and (dp|ss) are faster
   // through quartet lists
@@ -1047,8 +1045,8 @@ int cqx_set_option(cqx_ctx *c, const char *name, double value) {
   }
   if (!std::strcmp(name, "tiles")) {
     // 1: lane-group class pairs of Hermitian builds through tile lists
-    // (shared J / K targets summed per tile); 2 (default): only builds with
-    // at most one Hermitian J and one Hermitian K plane; 0: quartet lists
+    // (shared J / K targets summed per tile); 2: only builds with at most
+    // one Hermitian J and one Hermitian K plane; 0 (default): quartet lists
     if (!(value == 0.0 || value == 1.0 || value == 2.0)) return fail(c, CQX_ERR_ARG, "tiles must be 0, 1 or 2");
     c->use_tiles = (int)value;
     return CQX_OK;

@@ -245,10 +245,10 @@ int cqx_grad_two_body_contract(cqx_ctx *ctx, int ngrad, int nmat, const cqx_cont
  * lane-group class pairs of a build whose contractions are all Hermitian
  * evaluate their quartets in tiles of up to 8 bra pairs sharing their first
  * shell against up to 8 ket pairs sharing theirs, and sum the shared J / K
- * targets of a tile in LDS before adding them to HBM once; 2, the default:
- * the same, only for builds with at most one Hermitian J and one Hermitian
- * K plane per molecule (real RHF J / K); 0: never; not used by recorded or
- * allocation-free builds), "tile_classes" (bit k: the k-th lane-group class
+ * targets of a tile in LDS before adding them to HBM once; 2: the same,
+ * only for builds with at most one Hermitian J and one Hermitian K plane
+ * per molecule (real RHF J / K); 0, the default: never; not used by
+ * recorded or allocation-free builds), "tile_classes" (bit k: the k-th lane-group class
  * of (ss|ss), (ps|ss), (ps|ps), (pp|ss), (pp|ps), (ds|ss), (ds|ps), (dp|ss)
  * takes tiles; default 126, all but (ss|ss) and (dp|ss)), "rys_lds" (1, the
  * default: the lane-group kernels read the Rys table section of their root

@@ -5,8 +5,8 @@ tiles = 1 on every lane-group class must equal the quartet-list build
 (tiles = 0) of the same densities to 1e-12 relative, with the same quartets
 (and FP32 quartets), for real RHF, complex two-component, mixed-precision,
 batched and unscreened builds; the tile build also equals the oracle to
-1e-10.  The default (tiles = 2) takes tiles for a real J / K pair and
-quartet lists for the eight exchange planes of a two-component build."""
+1e-10.  tiles = 2 takes tiles for a real J / K pair and quartet lists for
+the eight exchange planes of a two-component build."""
 import sys
 
 import numpy as np
@@ -122,11 +122,11 @@ def test_tiles_batched_molecules():
 
 
 @pytest.mark.parametrize("case", ["rhf", "ghf"])
-def test_default_takes_tiles_for_a_real_pair_only(case):
-    """Default options: an RHF J / K build goes through tile lists (its
-    candidate count is the tile lists', not the quartet lists') and equals
-    the quartet-list build; a GHF build with eight exchange planes stays on
-    quartet lists (the same candidates)."""
+def test_tiles_2_takes_tiles_for_a_real_pair_only(case):
+    """tiles = 2: an RHF J / K build goes through tile lists (its candidate
+    count is the tile lists', not the quartet lists') and equals the
+    quartet-list build; a GHF build with eight exchange planes stays on
+    quartet lists (the same candidates), as does the default."""
     from chronusq_public_amd import cqx
     from chronusq_public_amd.chem import basis_text, water_cluster
     b = cqx.BasisSet(basis_text("cc-pVDZ"), water_cluster(4, 8.0, seed=9))
@@ -138,9 +138,11 @@ def test_default_takes_tiles_for_a_real_pair_only(case):
     else:
         D = _density(rng, b.nBasis, 20)
         Xs, types = [D, D], [0, 1]
-    A2, s2 = _build(b, Xs, types, None)
+    A2, s2 = _build(b, Xs, types, 2)
     A0, s0 = _build(b, Xs, types, 0)
+    Ad, sd = _build(b, Xs, types, None)
     _same(A2, A0, s2, s0)
+    assert sd["n_candidates"] == s0["n_candidates"]
     if case == "rhf":
         assert s2["n_candidates"] != s0["n_candidates"]
     else:
