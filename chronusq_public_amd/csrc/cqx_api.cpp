@@ -2292,7 +2292,7 @@ int cqx_grad_two_body_contract(cqx_ctx *c, int ngrad, int nmat, const cqx_contra
   int natom = 0;
   for (const auto &sh : BS.shells) {
     natom = std::max(natom, sh.atom + 1);
-    if (sh.l > 3) return fail(c, CQX_ERR_UNSUPPORTED, "gradient contractions cover s, p, d and f shells");
+    if (sh.l > cqx::kMaxLDev) return fail(c, CQX_ERR_UNSUPPORTED, "gradient contractions cover s through g shells");
   }
   if (ngrad < 3 * natom) return fail(c, CQX_ERR_ARG, "fewer gradient components than 3 x atoms of the basis");
   unsigned kmask = 0;
@@ -2341,7 +2341,7 @@ int cqx_grad_two_body_contract(cqx_ctx *c, int ngrad, int nmat, const cqx_contra
         class_ls(cb, G.la, G.lb);
         class_ls(ck, G.lc, G.ld);
         // the largest LDS a block may take (dynamic LDS up to the device's 160 KB)
-        G.chunk = cqx::grad_chunk(G.la, G.lb, G.lc, G.ld, std::max<size_t>((size_t)c->lds_max, 160 * 1024));
+        G.chunk = cqx::grad_chunk(G.la, G.lb, G.lc, G.ld, std::max<size_t>((size_t)c->lds_max, 160 * 1024), &G.rpp);
         if (G.chunk <= 0) throw Unsupported("gradient kernel LDS beyond the device limit");
         G.bra = c->d_pairs[cb].p;
         G.ket = c->d_pairs[ck].p;

@@ -2,7 +2,7 @@
 // shell quartet digested into one J or K matrix per nuclear coordinate,
 // DirectGradContraction::directScaffoldGrad (direct.hpp:2198-2580).
 //
-// One 64-lane wave per shell quartet, any class through (ff|ff).  Per primitive
+// One 64-lane wave per shell quartet, any class through (gg|gg).  Per primitive
 // quartet the Rys 2D integrals are built with one more unit of angular
 // momentum on centres A, B and C; the derivative of a Cartesian component
 // along direction x is 2 alpha I(a + 1_x) - a_x I(a - 1_x) on the x factor,
@@ -13,7 +13,8 @@
 // the coordinate (atom of the centre, direction).  The derivative quartets
 // are built in LDS for a chunk of the Cartesian components at a time (the
 // whole quartet when it fits beside the 2D tables; the primitive loop runs
-// once per chunk).
+// once per chunk), with the 2D tables of rpp roots at a time (all of them
+// unless the tables of a g class would not fit).
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -66,8 +67,9 @@ __device__ __forceinline__ int hix(const GDims &D, int ia, int ib, int ic, int i
 // (entries with both ia = la + 1 and ib = lb + 1 are not needed and not set)
 __device__ void rys_2d_grad(const GDims &D, double t2, double p, double q, double PAd, double QCd, double PQd,
                             double ABd, double CDd, double scale, double *__restrict__ H) {
-  double G[8][8];      // [i <= la + lb + 1][k <= lc + ld + 1]
-  double E[8][5][8];   // bra transfer: [i][j <= lb + 1][k]
+  constexpr int NI = 2 * kMaxLDev + 2;
+  double G[NI][NI];                 // [i <= la + lb + 1][k <= lc + ld + 1]
+  double E[NI][kMaxLDev + 2][NI];   // bra transfer: [i][j <= lb + 1][k]
   const double pq = p + q, rho = p * q / pq;
   const double B00 = 0.5 * t2 / pq;
   const double B10 = 0.5 / p * (1.0 - rho / p * t2);
@@ -90,7 +92,7 @@ __device__ void rys_2d_grad(const GDims &D, double t2, double p, double q, doubl
   for (int ia = 0; ia <= D.la + 1; ++ia)
     for (int ib = 0; ib <= D.lb + 1; ++ib) {
       if (ia + ib >= D.ni) continue;
-      double F[8][4];  // ket transfer: [k][l <= ld]
+      double F[NI][kMaxLDev + 1];  // ket transfer: [k][l <= ld]
       for (int k = 0; k < D.nk; ++k) F[k][0] = E[ia][ib][k];
       for (int l = 1; l <= D.ld; ++l)
         for (int k = 0; k + l < D.nk; ++k) F[k][l] = F[k + 1][l - 1] + CDd * F[k][l - 1];
@@ -106,10 +108,11 @@ __global__ __launch_bounds__(64) void eri_grad_kernel(GradLaunch a) {
   const int lane = threadIdx.x;
   const GDims D = gdims(a.la, a.lb, a.lc, a.ld);
   double *s_x = lds;
-  double *s_w = lds + 8;
-  double *s_H = lds + 16;
+  double *s_w = lds + 16;
+  double *s_H = lds + 32;
   const int chunk = a.chunk;                      // Cartesian components per pass
-  double *s_dI = s_H + (size_t)D.nr * 3 * D.nh;  // [9][chunk]: centre A, B, C x direction
+  const int RPP = a.rpp;                          // roots per 2D-table pass
+  double *s_dI = s_H + (size_t)RPP * 3 * D.nh;    // [9][chunk]: centre A, B, C x direction
   const int na = ncart_g(D.la), nbb = ncart_g(D.lb), ncc = ncart_g(D.lc), ndd = ncart_g(D.ld);
   const size_t nbc2 = (size_t)a.nbc * a.nbc;
   const long long npass = (D.nc + chunk - 1) / chunk;
@@ -147,14 +150,16 @@ __global__ __launch_bounds__(64) void eri_grad_kernel(GradLaunch a) {
           s_x[lane] = x;
           s_w[lane] = w * pref;
         }
+        const double twoal[3] = {2.0 * al_a, 2.0 * al_b, 2.0 * al_c};
+        for (int r0 = 0; r0 < D.nr; r0 += RPP) {
+        const int nrp = D.nr - r0 < RPP ? D.nr - r0 : RPP;
         __syncthreads();
-        if (lane < 3 * D.nr) {
-          const int r = lane / 3, dir = lane % 3;
+        if (lane < 3 * nrp) {
+          const int lr = lane / 3, dir = lane % 3, r = r0 + lr;
           rys_2d_grad(D, s_x[r], A.p, B.p, A.PA[dir], B.PA[dir], PQ[dir], bp.AB[dir], kp.AB[dir],
-                      dir == 2 ? s_w[r] : 1.0, s_H + (size_t)(r * 3 + dir) * D.nh);
+                      dir == 2 ? s_w[r] : 1.0, s_H + (size_t)(lr * 3 + dir) * D.nh);
         }
         __syncthreads();
-        const double twoal[3] = {2.0 * al_a, 2.0 * al_b, 2.0 * al_c};
         for (int o = o0 + lane; o < o1; o += 64) {
           int rem = o;
           const int id = rem % ndd; rem /= ndd;
@@ -168,8 +173,8 @@ __global__ __launch_bounds__(64) void eri_grad_kernel(GradLaunch a) {
           cart_exp_g(D.ld, id, e[3]);
           double acc[9];
           for (int t = 0; t < 9; ++t) acc[t] = 0.0;
-          for (int r = 0; r < D.nr; ++r) {
-            const double *Hr = s_H + (size_t)r * 3 * D.nh;
+          for (int lr = 0; lr < nrp; ++lr) {
+            const double *Hr = s_H + (size_t)lr * 3 * D.nh;
             double f[3], df[3][3];  // plain factor per direction; derivative per centre and direction
             for (int dir = 0; dir < 3; ++dir) {
               const double *Hd = Hr + (size_t)dir * D.nh;
@@ -186,6 +191,7 @@ __global__ __launch_bounds__(64) void eri_grad_kernel(GradLaunch a) {
             }
           }
           for (int t = 0; t < 9; ++t) s_dI[(size_t)t * chunk + (o - o0)] += acc[t];
+        }
         }
       }
     }
@@ -240,30 +246,44 @@ __global__ __launch_bounds__(64) void eri_grad_kernel(GradLaunch a) {
 
 }  // namespace
 
-size_t grad_lds_bytes(int la, int lb, int lc, int ld, int chunk) {
-  const int nr = (la + lb + lc + ld + 1) / 2 + 1;
+size_t grad_lds_bytes(int la, int lb, int lc, int ld, int chunk, int rpp) {
   const size_t nh = (size_t)(la + 2) * (lb + 2) * (lc + 2) * (ld + 1);
-  return sizeof(double) * (16 + (size_t)nr * 3 * nh + 9 * (size_t)chunk);
+  return sizeof(double) * (32 + (size_t)rpp * 3 * nh + 9 * (size_t)chunk);
 }
 
-int grad_chunk(int la, int lb, int lc, int ld, size_t lds_max) {
+int grad_chunk(int la, int lb, int lc, int ld, size_t lds_max, int *rpp_out) {
   auto nca = [](int l) { return (l + 1) * (l + 2) / 2; };
   const int nc = nca(la) * nca(lb) * nca(lc) * nca(ld);
-  const size_t fixed = grad_lds_bytes(la, lb, lc, ld, 0);
-  if (fixed >= lds_max) return 0;
-  const size_t fit = (lds_max - fixed) / (9 * sizeof(double));
-  if (fit >= (size_t)nc) return nc;
-  return fit >= 64 ? (int)(fit / 64 * 64) : 0;
+  const int nr = (la + lb + lc + ld + 1) / 2 + 1;
+  // the fewest passes over the components, then the most roots per table pass
+  int best = 0, best_rpp = 0;
+  long long best_passes = 0;
+  for (int rpp = nr; rpp >= 1; --rpp) {
+    const size_t fixed = grad_lds_bytes(la, lb, lc, ld, 0, rpp);
+    if (fixed >= lds_max) continue;
+    const size_t fit = (lds_max - fixed) / (9 * sizeof(double));
+    const int chunk = fit >= (size_t)nc ? nc : (fit >= 64 ? (int)(fit / 64 * 64) : 0);
+    if (chunk <= 0) continue;
+    const long long passes = (nc + chunk - 1) / chunk;
+    if (best == 0 || passes < best_passes) {
+      best = chunk;
+      best_rpp = rpp;
+      best_passes = passes;
+    }
+  }
+  if (rpp_out) *rpp_out = best_rpp;
+  return best;
 }
 
 hipError_t launch_eri_grad(const GradLaunch &a, hipStream_t s) {
   if (a.nq <= 0) return hipSuccess;
-  if (a.la > 3 || a.lb > 3 || a.lc > 3 || a.ld > 3 || a.chunk <= 0) return hipErrorNotSupported;
+  if (a.la > kMaxLDev || a.lb > kMaxLDev || a.lc > kMaxLDev || a.ld > kMaxLDev || a.chunk <= 0 || a.rpp <= 0)
+    return hipErrorNotSupported;
   const int nc = ((a.la + 1) * (a.la + 2) / 2) * ((a.lb + 1) * (a.lb + 2) / 2) * ((a.lc + 1) * (a.lc + 2) / 2) *
                  ((a.ld + 1) * (a.ld + 2) / 2);
   const long long work = a.nq * ((nc + a.chunk - 1) / a.chunk);
   const long long blocks = work < 262144 ? work : 262144;
-  const size_t lds = grad_lds_bytes(a.la, a.lb, a.lc, a.ld, a.chunk);
+  const size_t lds = grad_lds_bytes(a.la, a.lb, a.lc, a.ld, a.chunk, a.rpp);
   static std::once_flag once;  // dynamic LDS beyond the default limit for the f classes
   static hipError_t attr = hipSuccess;
   std::call_once(once, [&] {

@@ -74,13 +74,16 @@ struct GradLaunch {
   int nmat, nbc;
   unsigned kmask;             // bit m: matrix m is an exchange contraction
   int chunk;                  // Cartesian components of a quartet per pass (grad_chunk)
+  int rpp;                    // Rys roots whose 2D tables are built at a time (grad_chunk)
 };
 hipError_t launch_eri_grad(const GradLaunch &a, hipStream_t s);
 // LDS bytes of a gradient launch that builds chunk Cartesian components per
-// pass, and the largest such chunk (a multiple of 64, or the whole quartet)
-// within lds_max bytes; 0 when even the 2D tables do not fit
-size_t grad_lds_bytes(int la, int lb, int lc, int ld, int chunk);
-int grad_chunk(int la, int lb, int lc, int ld, size_t lds_max);
+// pass with the 2D tables of rpp roots at a time, and the chunk (a multiple
+// of 64, or the whole quartet) and rpp within lds_max bytes that take the
+// fewest passes (then the most roots per table pass); 0 when even one
+// root's tables do not fit
+size_t grad_lds_bytes(int la, int lb, int lc, int ld, int chunk, int rpp);
+int grad_chunk(int la, int lb, int lc, int ld, size_t lds_max, int *rpp);
 
 struct ListLaunch {
   const PairScr *bra, *ket;   // screening records of the bra / ket classes

@@ -208,7 +208,7 @@ int cqx_last_stats(const cqx_ctx *ctx, cqx_stats *st);
  * and the NB x NB output AX of coordinate g, which receives
  * 1/2 (A + A^T) (callers zero it).  Real, Hermitian contractions only
  * (the reference instantiates <double, double> and stops on non-Hermitian
- * ones), no screening (as the reference, direct.hpp:2204-2205), s through f
+ * ones), no screening (as the reference, direct.hpp:2204-2205), s through g
  * shells, one-molecule contexts; mem as for cqx_two_body_contract. */
 int cqx_grad_two_body_contract(cqx_ctx *ctx, int ngrad, int nmat, const cqx_contraction *list, int mem);
 

@@ -679,14 +679,14 @@ static int eri_shell_p(const oshell *A, const oshell *B, const oshell *Cc, const
  * to cancellation (a g shell second in a pair costs 1e-10 relative at O-H
  * distances), so each pair is evaluated with its higher-l shell first and
  * the quartet is transposed back to the order asked for. */
-static int eri_shell(const oracle_basis *bs, int s1, int s2, int s3, int s4,
-                     double *out, eri_work *w) {
-  const int swb = bs->sh[s1].l < bs->sh[s2].l, swk = bs->sh[s3].l < bs->sh[s4].l;
-  if (!swb && !swk) return eri_shell_p(&bs->sh[s1], &bs->sh[s2], &bs->sh[s3], &bs->sh[s4], out, w);
-  const int a = swb ? s2 : s1, b = swb ? s1 : s2, c = swk ? s4 : s3, d = swk ? s3 : s4;
-  const int n = eri_shell_p(&bs->sh[a], &bs->sh[b], &bs->sh[c], &bs->sh[d], w->O, w);
-  const int n1 = bs->sh[s1].nfunc, n2 = bs->sh[s2].nfunc, n3 = bs->sh[s3].nfunc, n4 = bs->sh[s4].nfunc;
-  const int nb = bs->sh[b].nfunc, nc = bs->sh[c].nfunc, nd = bs->sh[d].nfunc;
+static int eri_oriented(const oshell *S1, const oshell *S2, const oshell *S3, const oshell *S4,
+                        double *out, eri_work *w) {
+  const int swb = S1->l < S2->l, swk = S3->l < S4->l;
+  if (!swb && !swk) return eri_shell_p(S1, S2, S3, S4, out, w);
+  const oshell *A = swb ? S2 : S1, *B = swb ? S1 : S2, *C = swk ? S4 : S3, *D = swk ? S3 : S4;
+  const int n = eri_shell_p(A, B, C, D, w->O, w);
+  const int n1 = S1->nfunc, n2 = S2->nfunc, n3 = S3->nfunc, n4 = S4->nfunc;
+  const int nb = B->nfunc, nc = C->nfunc, nd = D->nfunc;
   for (int i1 = 0; i1 < n1; i1++)
     for (int i2 = 0; i2 < n2; i2++)
       for (int i3 = 0; i3 < n3; i3++)
@@ -695,6 +695,11 @@ static int eri_shell(const oracle_basis *bs, int s1, int s2, int s3, int s4,
           out[(((size_t)i1 * n2 + i2) * n3 + i3) * n4 + i4] = w->O[(((size_t)ia * nb + ib) * nc + ic) * nd + id];
         }
   return n;
+}
+
+static int eri_shell(const oracle_basis *bs, int s1, int s2, int s3, int s4,
+                     double *out, eri_work *w) {
+  return eri_oriented(&bs->sh[s1], &bs->sh[s2], &bs->sh[s3], &bs->sh[s4], out, w);
 }
 
 int oracle_eri_quartet(const oracle_basis *b, int s1, int s2, int s3, int s4,
@@ -1079,11 +1084,11 @@ static void eri_centre_deriv(const oracle_basis *bs, const int s[4], int k, doub
   const oshell *Q[4] = {S[0], S[1], S[2], S[3]};
   Q[k] = &up;
   double *P = scr, *M = scr + scr_n, *T = scr + 2 * scr_n;
-  eri_shell_p(Q[0], Q[1], Q[2], Q[3], P, w);
+  eri_oriented(Q[0], Q[1], Q[2], Q[3], P, w);
   if (K->l > 0) {
     shifted_shell(K, 0, &dn);
     Q[k] = &dn;
-    eri_shell_p(Q[0], Q[1], Q[2], Q[3], M, w);
+    eri_oriented(Q[0], Q[1], Q[2], Q[3], M, w);
   }
   int pre = 1, post = 1;
   for (int j = 0; j < k; j++) pre *= S[j]->nfunc;

@@ -28,9 +28,11 @@ def _cases():
     # f shells: (ff|ff) and the other f classes, pure and Cartesian
     yield "water cc-pVTZ", water_cluster(1, 3.0, seed=1), basis_text("cc-pVTZ"), None, False
     yield "water cc-pVTZ Cartesian", water_cluster(1, 3.0, seed=1), basis_text("cc-pVTZ"), None, True
+    # g shells: (gg|gg) needs the 2D tables one root at a time and 28 passes
+    yield "water cc-pVQZ", water_cluster(1, 3.0, seed=1), basis_text("cc-pVQZ"), None, False
 
 
-@pytest.mark.parametrize("idx", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("idx", [0, 1, 2, 3, 4, 5])
 def test_grad_contraction_matches_oracle(idx):
     from chronusq_public_amd import cqx
     name, mol, gbs, D, cart = list(_cases())[idx]

@@ -13,17 +13,19 @@ from conftest import fixture_molecule, load_fixture
 from oracle.oracle import COULOMB, EXCHANGE, OracleBasis
 
 
-def test_grad_contraction_through_f_shells():
-    """Water / cc-pVTZ (f functions on O): every coordinate's contraction
-    equals the central finite difference of the forward contraction, and
-    the atoms' contractions sum to zero along each axis."""
+@pytest.mark.parametrize("basis,lmax", [("cc-pVTZ", 3), ("cc-pVQZ", 4)])
+def test_grad_contraction_through_f_and_g_shells(basis, lmax):
+    """Water / cc-pVTZ (f functions on O) and cc-pVQZ (g on O, f on H):
+    every coordinate's contraction equals the central finite difference of
+    the forward contraction, and the atoms' contractions sum to zero along
+    each axis."""
     from chronusq_public_amd.chem import basis_text, water_cluster
     mol = water_cluster(1, 3.0, seed=1)
-    gbs = basis_text("cc-pVTZ")
+    gbs = basis_text(basis)
     Z = np.asarray(mol.Z, dtype=np.int32)
     xyz = np.asarray(mol.xyz, dtype=np.float64)
     ob = OracleBasis(gbs, Z, xyz)
-    assert max(s["l"] for s in ob.shells()) == 3
+    assert max(s["l"] for s in ob.shells()) == lmax
     nb = ob.nbf
     rng = np.random.Generator(np.random.PCG64(43))
     C_ = rng.normal(0, 1 / np.sqrt(nb), size=(nb, 5))
