@@ -1535,19 +1535,6 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
           HIPCHK(cqx::launch_candidates(CL, w.scan, b, s));
         }
       };
-      // --- back to pure functions, symmetrise, accumulate into AX ---
-      auto back = [&]() {
-        for (int m = 0; m < nmol; ++m) {
-          const cqx::CartMap M = mol_map(m);
-          for (int j = 0; j < nmat; ++j) {
-            const bool two = is_complex && !(L[j].her && L[j].type == CQX_COULOMB);
-            const int pr = slot_re[j], pi = slot_im[j];
-            const double *are = mp[m].ka[pd[pr].kind] + pidx[pr];
-            const double *aim = two ? mp[m].ka[pd[pi].kind] + pidx[pi] : nullptr;
-            HIPCHK(cqx::launch_from_cart(M, are, aim, ps.kps[pd[pr].kind], L[j].her, is_complex, ax_of(m, j), s));
-          }
-        }
-      };
       front();
       const bool mixed = c->fp32_cut > 0.0;
       const int qmul = mixed ? 2 : 1;  // quartet lists per class pair
@@ -1712,6 +1699,25 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
         }
       };
       auto stream_of = [&](int launch, int nst) { return launch % nst == 0 ? s : c->side[launch % nst - 1]; };
+      // --- back to pure functions, symmetrise, accumulate into AX: the
+      // contractions' transforms are independent and are dealt over nst
+      // streams like the ERI launches (side streams already created) ---
+      auto back = [&](int nst) {
+        fork_side(nst);
+        int q = 0;
+        for (int m = 0; m < nmol; ++m) {
+          const cqx::CartMap M = mol_map(m);
+          for (int j = 0; j < nmat; ++j) {
+            const bool two = is_complex && !(L[j].her && L[j].type == CQX_COULOMB);
+            const int pr = slot_re[j], pi = slot_im[j];
+            const double *are = mp[m].ka[pd[pr].kind] + pidx[pr];
+            const double *aim = two ? mp[m].ka[pd[pi].kind] + pidx[pi] : nullptr;
+            HIPCHK(cqx::launch_from_cart(M, are, aim, ps.kps[pd[pr].kind], L[j].her, is_complex, ax_of(m, j),
+                                         stream_of(q++, nst)));
+          }
+        }
+        join_side(nst);
+      };
       // bra-range pieces of at most list_cap candidates, grouped into batches
       // of at most list_cap candidates: the quartet lists of one batch are
       // built, counted (one host synchronisation) and evaluated before the
@@ -1986,7 +1992,8 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
         HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
         eri_ms_total += ms;
       }
-      back();
+      ensure_side(nst_all);
+      back(nst_all);
       if (!scr) {
         const hipMemcpyKind dk = mem == CQX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
         for (int m = 0; m < nmol; ++m)
@@ -2101,7 +2108,7 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
           }
           join_side(nst);
           forked = false;
-          back();
+          back(nst);
           HIPCHK(cqx::launch_gather(w.cand, c->d_gidx.p, (int)ncp, c->d_gchk.p, s));
         } catch (...) {
           if (forked)  // an unjoined side stream would leave the capture open
