@@ -556,6 +556,10 @@ def main():
             "traffic": None,
             "flops_per_build": flops_avg,
             "eri_ms_per_build": eri_avg,
+            # cross-check of the flop model: primitive quartets of the evaluated
+            # shell quartets (Ka Kb Kc Kd, before primitive-pair screening) per second
+            "prim_quartets_per_build": st["n_prim_quartets_max"],
+            "prim_quartets_per_s": st["n_prim_quartets_max"] / (eri_avg * 1e-3),
         },
     }
     pc = profile_counters(args.workload)
