@@ -93,3 +93,54 @@ def test_qz_contraction_against_oracle(case):
         say(f"  max |d| {err:.2e} (max |AX| {np.abs(r).max():.2e})")
         assert err < 1e-10
     assert tpi.last_stats()["n_quartets"] > 0
+
+
+def test_qz_batched_mixed_and_recorded():
+    """g shells through the other paths: a batched context of water
+    cc-pVQZ and (H2O)2 cc-pVDZ gives each molecule its oracle J / K
+    (1e-10); mixed precision keeps the classes with f or g shells in FP64
+    and stays within 1e-7 of FP64 (max |dF|, dF = 2 dJ - dK); a recorded
+    build replays to the eager result (1e-11)."""
+    from chronusq_public_amd import cqx
+    from chronusq_public_amd.chem import WATER, Molecule, basis_text, water_cluster
+    say("case: cc-pVQZ batched, mixed, recorded")
+    mols = [(Molecule.from_angstrom(WATER), basis_text("cc-pVQZ")), (water_cluster(2, 4.0, seed=3), basis_text("cc-pVDZ"))]
+    parts = [cqx.BasisSet(g, m) for m, g in mols]
+    rng = np.random.Generator(np.random.PCG64(8))
+    dens = []
+    for p in parts:
+        C = rng.normal(0, 1 / np.sqrt(p.nBasis), (p.nBasis, 5))
+        dens.append(np.asfortranarray(C @ C.T))
+
+    def run(tpi, Ds):
+        gto = cqx.GTODirectTPIContraction(tpi)
+        out, cont = [], []
+        for D in Ds:
+            J, K = np.zeros(D.shape, order="F"), np.zeros(D.shape, order="F")
+            out.append((J, K))
+            cont += [cqx.TwoBodyContraction(D, J, True, COULOMB), cqx.TwoBodyContraction(D, K, True, EXCHANGE)]
+        gto.twoBodyContract(cont)
+        return out
+    bb = cqx.BasisSet.batch(parts)
+    got = run(cqx.DirectTPI(bb, 1e-12), dens)
+    for (m, g), D, (J, K) in zip(mols, dens, got):
+        Jr, Kr = OracleBasis(g, m.Z, m.xyz).contract([D, D], [COULOMB, EXCHANGE], [True, True])
+        assert np.abs(J - Jr).max() < 1e-10 and np.abs(K - Kr).max() < 1e-10
+    # mixed precision on the QZ molecule alone
+    t64 = cqx.DirectTPI(parts[0], 1e-12)
+    (J64, K64), = run(t64, dens[:1])
+    t32 = cqx.DirectTPI(parts[0], 1e-12)
+    t32.set_option("fp32_cut", 1e-4)
+    (J32, K32), = run(t32, dens[:1])
+    st = t32.last_stats()
+    assert 0 < st["n_quartets_f32"] < st["n_quartets"] == t64.last_stats()["n_quartets"]
+    dF = np.abs((2 * J32 - K32) - (2 * J64 - K64)).max()
+    say(f"  mixed max|dF| {dF:.2e}, FP32 quartets {st['n_quartets_f32']} of {st['n_quartets']}")
+    assert dF < 1e-7
+    # recorded build and its replays
+    tg = cqx.DirectTPI(parts[0], 1e-12)
+    tg.set_option("graph", 1)
+    for it in range(3):
+        (Jg, Kg), = run(tg, dens[:1])
+        assert tg.last_stats()["graph"] == (1 if it == 0 else 2)
+        assert np.abs(Jg - J64).max() < 1e-11 and np.abs(Kg - K64).max() < 1e-11
