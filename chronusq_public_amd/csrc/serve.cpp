@@ -11,13 +11,17 @@
 //  * per-job streams: each job iterates on its own context and stream, so
 //    the Fock builds, eigensolves and GEMMs of the jobs in flight overlap on
 //    the device;
-//  * cross-request batching (opts->batch_fock): the Fock builds of every
-//    job in flight go into one batched contraction per round, one launch
+//  * cross-request batching (opts->batch_fock): the Fock builds of the
+//    jobs in flight go into one batched contraction per round, one launch
 //    per class pair for all of them, over a context merged from the jobs'
 //    own pair data (rebuilt only when the set of jobs changes); each job
 //    then finishes its iteration (energy, convergence, CDIIS, eigensolve,
 //    density) on its own stream, concurrently with the others.  A job that
-//    finishes set-up joins the next round; a converged job leaves.
+//    finishes set-up joins the next round; a converged job leaves.  Jobs
+//    are batched with jobs of similar size only (basis sizes within a
+//    factor of two, kBucketBase below), each size class in rounds of its
+//    own on a coordinator thread of its own, so that a small job does not
+//    iterate at the pace of the largest one in flight.
 // The reference has no serving layer; each job is its SingleSlater SCF.
 #include <hip/hip_runtime.h>
 
@@ -57,8 +61,18 @@ double job_footprint(int nb, int nshell, long long list_cap, bool mixed, bool ba
          pairs * (72.0 + 64.0 * 9.0) * (batched ? 2.0 : 1.0) + (64 << 20);
 }
 
+// size classes of cross-request batching: class b holds the jobs with
+// kBucketBase 2^b <= NB < kBucketBase 2^(b+1) (smaller ones in class 0)
+constexpr int kBucketBase = 96;
+constexpr int kBuckets = 8;
+int size_bucket(int nb) {
+  int b = 0;
+  while (b + 1 < kBuckets && nb >= (kBucketBase << (b + 1))) ++b;
+  return b;
+}
+
 // State shared by the dispatcher, the job threads and (batched) the round
-// coordinator; every field is guarded by mu.
+// coordinators; every field is guarded by mu.
 struct Served {
   const cqx_job *jobs;
   int njobs;
@@ -80,6 +94,7 @@ struct Served {
   std::vector<char> leaving;     // finished (converged or failed) in this round
   std::vector<int> round_rc;     // status of this round's batched build for the job
   std::vector<double> round_ms;  // this round's build time
+  std::vector<int> bucket;       // size class of the job
   bool stop = false;
 
   void finish(int j, int rc) {  // with mu held
@@ -191,9 +206,10 @@ void run_job_batched(Served &S, int j) {
   S.finish(j, rc);
 }
 
-// Cross-request batching, coordinator: one batched J/K build per round for
-// every active job, over a context merged from their pair data.
-void coordinate(Served &S) {
+// Cross-request batching, coordinator of size class b: one batched J/K
+// build per round for every active job of the class, over a context merged
+// from their pair data.
+void coordinate(Served &S, int b) {
   (void)hipSetDevice(S.opts->device);
   cqx_ctx *merged = nullptr;
   std::vector<int> prev;
@@ -202,13 +218,13 @@ void coordinate(Served &S) {
     S.cv.wait(lk, [&] {
       if (S.stop) return true;
       for (int j = 0; j < S.njobs; ++j)
-        if (S.active[j]) return true;
+        if (S.active[j] && S.bucket[j] == b) return true;
       return false;
     });
     if (S.stop) break;
     std::vector<int> P;
     for (int j = 0; j < S.njobs; ++j)
-      if (S.active[j]) P.push_back(j);
+      if (S.active[j] && S.bucket[j] == b) P.push_back(j);
     std::vector<cqx_ctx *> parts;
     std::vector<const double *> D(P.size());
     std::vector<double *> J(P.size()), K(P.size());
@@ -306,6 +322,7 @@ int cqx_serve(const cqx_job *jobs, int njobs, const cqx_serve_opts *opts, cqx_jo
   S.leaving.assign(njobs, 0);
   S.round_rc.assign(njobs, CQX_OK);
   S.round_ms.assign(njobs, 0.0);
+  S.bucket.assign(njobs, 0);
   std::vector<int> order(njobs);
   for (int i = 0; i < njobs; ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(),
@@ -316,8 +333,9 @@ int cqx_serve(const cqx_job *jobs, int njobs, const cqx_serve_opts *opts, cqx_jo
   }
   const bool batched = opts->batch_fock != 0;
   std::vector<std::thread> threads;
-  std::thread coordinator;
-  if (batched) coordinator = std::thread(coordinate, std::ref(S));
+  std::vector<std::thread> coordinators;
+  if (batched)
+    for (int b = 0; b < kBuckets; ++b) coordinators.emplace_back(coordinate, std::ref(S), b);
   std::deque<int> waiting;
   size_t next = 0;
   int rc_all = CQX_OK;
@@ -334,6 +352,7 @@ int cqx_serve(const cqx_job *jobs, int njobs, const cqx_serve_opts *opts, cqx_jo
         int nsh = 0, nb = 0, mp = 0, ml = 0;
         if (rc == CQX_OK) rc = cqx_basis_dims(S.basis[j], &nsh, &nb, &mp, &ml);
         if (rc == CQX_OK) {
+          S.bucket[j] = size_bucket(nb);
           S.foot[j] = job_footprint(nb, nsh, opts->list_cap, opts->scf.fp32_cut > 0.0, batched);
           out[j].footprint_bytes = S.foot[j];
           if (S.foot[j] > opts->hbm_budget_bytes) rc = CQX_ERR_MEM;  // refused: beyond the whole budget
@@ -372,7 +391,7 @@ int cqx_serve(const cqx_job *jobs, int njobs, const cqx_serve_opts *opts, cqx_jo
     S.cv.notify_all();
   }
   for (auto &t : threads) t.join();
-  if (coordinator.joinable()) coordinator.join();
+  for (auto &t : coordinators) t.join();
   for (auto *b : S.basis)
     if (b) cqx_basis_destroy(b);
   for (int i = 0; i < njobs && rc_all == CQX_OK; ++i)

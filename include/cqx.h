@@ -376,10 +376,11 @@ const char *cqx_scf_last_error(const cqx_scf *s);
  * and keeps its state on the device until it converges.  Its iterations
  * run either on its own stream (batch_fock = 0: the jobs' Fock builds
  * overlap through their streams) or in rounds (batch_fock = 1): every
- * round builds the J and K of all jobs in flight in one batched
- * contraction over a context merged from the jobs' pair data, then each
- * job finishes its iteration on its own stream; jobs join at the next
- * round after set-up and leave once converged.  A Fock matrix with a
+ * round builds the J and K of the jobs in flight of one size class (basis
+ * sizes within a factor of two) in one batched contraction over a context
+ * merged from their pair data, then each job finishes its iteration on its
+ * own stream; each size class runs its rounds on a thread of its own; jobs
+ * join at the next round after set-up and leave once converged.  A Fock matrix with a
  * non-finite entry stops its job (CQX_ERR_ARG).  Blocks until every job
  * has finished; returns the first non-zero job status, if any.
  * ------------------------------------------------------------------- */
