@@ -3,8 +3,8 @@ generic kernel, which builds quartets too large for LDS in passes over the
 first shell's components and the roots (Rys rules up to nine roots).
 Against the oracle (its g integrals pinned in test_oracle_g.py): the shell
 pairs and Schwarz bounds (1e-12 relative), sampled contracted quartets of
-every class up to (gg|gg) (1e-12 relative), and RHF, complex two-component
-and non-Hermitian J / K (1e-10)."""
+every class up to (gg|gg) (1e-12 relative), and RHF, complex two-component,
+non-Hermitian and Cartesian-g J / K (1e-10)."""
 import sys
 
 import numpy as np
@@ -64,9 +64,13 @@ def test_qz_eri_quartets_every_class_through_g():
         assert np.abs(v - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), q
 
 
-@pytest.mark.parametrize("case", ["rhf", "ghf", "nonhermitian"])
+@pytest.mark.parametrize("case", ["rhf", "ghf", "nonhermitian", "cartesian"])
 def test_qz_contraction_against_oracle(case):
     cqx, mol, gbs, b, tpi, ob = _setup()
+    if case == "cartesian":  # 15 Cartesian g functions: (gg|gg) of 50625 components
+        b = cqx.BasisSet(gbs, mol, force_cart=True)
+        tpi = cqx.DirectTPI(b, 1e-12)
+        ob = OracleBasis(gbs, mol.Z, mol.xyz, force_cart=True)
     gto = cqx.GTODirectTPIContraction(tpi)
     nb = b.nBasis
     rng = np.random.Generator(np.random.PCG64(5))
@@ -75,7 +79,7 @@ def test_qz_contraction_against_oracle(case):
         C = rng.normal(0, 1 / np.sqrt(nb), (nb, 5)) + 1j * rng.normal(0, 1 / np.sqrt(nb), (nb, 5))
         D = C @ C.conj().T
         Xs, types, hers = [D, D, 0.3 * D, 0.2 * D.conj(), 0.1 * D], [COULOMB] + [EXCHANGE] * 4, [True] * 5
-    elif case == "rhf":
+    elif case in ("rhf", "cartesian"):
         C = rng.normal(0, 1 / np.sqrt(nb), (nb, 5))
         D = C @ C.T
         Xs, types, hers = [D, D], [COULOMB, EXCHANGE], [True, True]
