@@ -1625,6 +1625,11 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
         if (c->use_wave && cqx::eri_wave_available(la, lb, lc, ld)) return 1;
         return 2;
       };
+      // classes with an FP32 kernel: the lane-group and wave kernels of the
+      // s/p/d classes (quartets with an f or g shell stay in FP64)
+      auto fp32_class = [&](int la, int lb, int lc, int ld) {
+        return route(la, lb, lc, ld) < 2 && std::max(std::max(la, lb), std::max(lc, ld)) <= 2;
+      };
       // the ERI launch of class pair k's bras from i0 on, in one precision
       // (the count, list and lane choice are set by the caller)
       auto eri_base = [&](const CP &cp, int i0, bool f32) {
@@ -1899,7 +1904,7 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
           int la_, lb_, lc_, ld_;
           class_ls(cp.cb, la_, lb_);
           class_ls(cp.ck, lc_, ld_);
-          if (mixed && route(la_, lb_, lc_, ld_) < 2) {
+          if (mixed && fp32_class(la_, lb_, lc_, ld_)) {
             LL.cut32 = c->fp32_cut;
             LL.out32 = w.quart + cap_need + pc.seg;
             LL.count32 = w.count + np + k;
@@ -2020,7 +2025,7 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
         class_ls(cps[k].cb, la_, lb_);
         class_ls(cps[k].ck, lc_, ld_);
         lcap[k] = (long long)std::ceil(cp_count[k] * c->graph_margin) + 1024;
-        if (mixed && route(la_, lb_, lc_, ld_) < 2)
+        if (mixed && fp32_class(la_, lb_, lc_, ld_))
           lcap[ncp + k] = (long long)std::ceil(cp_count[ncp + k] * c->graph_margin) + 1024;
       }
       for (size_t kk = 0; kk < 2 * ncp; ++kk) lseg[kk + 1] = lseg[kk] + lcap[kk];
@@ -2065,7 +2070,7 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
           class_ls(cp.cb, la_, lb_);
           class_ls(cp.ck, lc_, ld_);
           slots[k] = {la_, lb_, lc_, ld_, 0};
-          if (mixed && route(la_, lb_, lc_, ld_) < 2) {
+          if (mixed && fp32_class(la_, lb_, lc_, ld_)) {
             LL.cut32 = c->fp32_cut;
             LL.out32 = w.quart + lseg[ncp + k];
             LL.cap32 = lcap[ncp + k];

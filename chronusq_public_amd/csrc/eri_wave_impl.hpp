@@ -1,5 +1,6 @@
-// Wave kernels for the larger s/p/d classes (Cartesian quartet size > 27),
-// templated on the class and on QT, the quartets a wave takes together.
+// Wave kernels for the larger s/p/d classes (Cartesian quartet size > 27)
+// and the classes with an f shell up to 1080 Cartesian components, templated
+// on the class and on QT, the quartets a wave takes together.
 //
 // The primitive quartets of the wave's QT quartets form one list.  Each
 // round takes up to PP of them: lane group g (3 n_r lanes) builds the 2D
@@ -7,8 +8,9 @@
 // direction), each lane evaluating only its own Rys node and weight, and
 // stores them to the wave's LDS slice; then every lane adds the round's
 // tables to its share of the contracted Cartesian quartet they belong to.
-// After the last round the QT quartets go to LDS and the lanes digest them
-// together (digest_tile.hpp).  Instantiated per QT (eri_wave_q*.hip).
+// After the last round the QT quartets go to LDS, in the space the tables
+// took, and the lanes digest them together (digest_tile.hpp).  Instantiated
+// per QT (eri_wave_q*.hip; the f classes, FP64 only, in eri_wave_f*.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -34,11 +36,18 @@ constexpr int prims_per_round() {
   return p < by_lds ? (p < 1 ? 1 : p) : (by_lds < 1 ? 1 : by_lds);
 }
 
+// A wave's LDS buffer in doubles: the 2D tables of one round, and after
+// the last round the QT contracted quartets, share it
+template <class K, class R, int QT>
+constexpr int wave_buf_doubles() {
+  constexpr int tables = (prims_per_round<K, R>() * 3 * K::NR * K::NH * (int)sizeof(R) + 7) / 8;
+  return tables > QT * K::NC ? tables : QT * K::NC;
+}
+
 // LDS of a block of the kernel below with QT quartets per wave
 template <class K, class R, int QT>
 constexpr int wave_lds_bytes() {
-  return kWavesPerBlock * (prims_per_round<K, R>() * 3 * K::NR * K::NH * (int)sizeof(R) + QT * K::NC * 8 + QT * 24 +
-                          QT * 2 * (int)sizeof(PairRec));
+  return kWavesPerBlock * (wave_buf_doubles<K, R, QT>() * 8 + QT * 24 + QT * 2 * (int)sizeof(PairRec));
 }
 
 // QT: quartets a wave takes together.  Their primitive quartets form one
@@ -54,8 +63,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
   constexpr int NOWN = (K::NC + 63) / 64;
   constexpr int HSZ = 3 * K::NR * K::NH;
   constexpr int PP = prims_per_round<K, R>();
-  __shared__ R s_H[kWavesPerBlock][PP * HSZ];
-  __shared__ double s_I[kWavesPerBlock][QT * K::NC];
+  __shared__ double s_buf[kWavesPerBlock][wave_buf_doubles<K, R, QT>()];
   __shared__ int s_bf[kWavesPerBlock][QT][4];
   __shared__ unsigned s_mask[kWavesPerBlock][QT];
   __shared__ int s_mol[kWavesPerBlock][QT];
@@ -96,7 +104,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
       if (kq >= off[u]) t = u;
     return t;
   };
-  R *H = s_H[wv];
+  R *H = reinterpret_cast<R *>(s_buf[wv]);  // the round's 2D tables
   const R *__restrict__ rys = rys_table<R>(a.rys, a.rys32);
 
   // the lane's outputs o = lane + 64 k and their 2D-table offsets
@@ -201,7 +209,10 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
     }
     wave_sync();
   }
-  // degeneracy (direct.hpp:1664-1667, 1793-1802) and the 1/2 of :1849-1850
+  // degeneracy (direct.hpp:1664-1667, 1793-1802) and the 1/2 of :1849-1850;
+  // the quartets overwrite the tables, which the last round's wave_sync
+  // has retired
+  double *sI = s_buf[wv];
 #pragma unroll
   for (int t = 0; t < QT; ++t) {
     if (t >= ntile) break;
@@ -211,10 +222,10 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void eri_wave_kernel(EriLaunch
     const double scale = 0.5 * (bp.a == bp.b ? 1.0 : 2.0) * (kp.a == kp.b ? 1.0 : 2.0) * (same_pair ? 1.0 : 2.0);
 #pragma unroll
     for (int k = 0; k < NOWN; ++k)
-      if (lane + 64 * k < K::NC) s_I[wv][t * K::NC + lane + 64 * k] = (double)acc[t][k] * scale;
+      if (lane + 64 * k < K::NC) sI[t * K::NC + lane + 64 * k] = (double)acc[t][k] * scale;
   }
   wave_sync();
-  digest_tile<K>(s_I[wv], s_bf[wv], s_mask[wv], s_mol[wv], ntile, a.planes, lane);
+  digest_tile<K>(sI, s_bf[wv], s_mask[wv], s_mol[wv], ntile, a.planes, lane);
 }
 
 // LDS a block may take on gfx950 (160 KB per CU): variants beyond it are
@@ -270,7 +281,50 @@ hipError_t launch_wave_q(const EriLaunch &a, hipStream_t s) {
   return hipErrorNotSupported;
 }
 
-// whether the QT variant of a class is built, and its LDS per block
+// classes with an f shell and at most 1080 Cartesian components (the
+// larger ones and the g classes take the generic kernel); FP64 only, since
+// the FP32 lists hold s/p/d classes alone
+#define CQX_WAVE_F_CLASSES(X)                                                                   \
+  X(3, 0, 0, 0) X(3, 0, 1, 0) X(3, 0, 1, 1) X(3, 0, 2, 0) X(3, 0, 2, 1) X(3, 0, 2, 2) X(3, 0, 3, 0) \
+  X(3, 1, 0, 0) X(3, 1, 1, 0) X(3, 1, 1, 1) X(3, 1, 2, 0) X(3, 1, 2, 1) X(3, 1, 2, 2) X(3, 1, 3, 0) \
+  X(3, 1, 3, 1) X(3, 2, 0, 0) X(3, 2, 1, 0) X(3, 2, 1, 1) X(3, 2, 2, 0) X(3, 2, 2, 1) X(3, 2, 3, 0) \
+  X(3, 3, 0, 0) X(3, 3, 1, 0) X(3, 3, 1, 1) X(3, 3, 2, 0) X(3, 3, 3, 0)
+
+template <int LA, int LB, int LC, int LD, int QT>
+hipError_t launch_wave_f_cls(const EriLaunch &a, hipStream_t s) {
+  using K = Cls<LA, LB, LC, LD>;
+  constexpr long long per_block = (long long)kWavesPerBlock * QT;
+  const long long blocks = (a.nq + per_block - 1) / per_block;
+  if (a.fp32) return hipErrorNotSupported;
+  if constexpr (wave_lds_bytes<K, double, QT>() <= kWaveLdsMax)
+    hipLaunchKernelGGL((eri_wave_kernel<LA, LB, LC, LD, double, QT>), dim3((unsigned)blocks),
+                       dim3(64 * kWavesPerBlock), 0, s, a);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+// every f wave class with QT quartets per wave
+template <int QT>
+hipError_t launch_wave_fq(const EriLaunch &a, hipStream_t s) {
+#define CQX_CASE(A, B, C, D) \
+  if (a.la == A && a.lb == B && a.lc == C && a.ld == D) return launch_wave_f_cls<A, B, C, D, QT>(a, s);
+  CQX_WAVE_F_CLASSES(CQX_CASE)
+#undef CQX_CASE
+  return hipErrorNotSupported;
+}
+
+// whether the QT variant of an f class is built (FP64)
+template <int QT>
+bool wave_fq_built(int la, int lb, int lc, int ld) {
+#define CQX_CASE(A, B, C, D) \
+  if (la == A && lb == B && lc == C && ld == D) return wave_lds_bytes<Cls<A, B, C, D>, double, QT>() <= kWaveLdsMax;
+  CQX_WAVE_F_CLASSES(CQX_CASE)
+#undef CQX_CASE
+  return false;
+}
+
+// whether the QT variant of an s/p/d class is built, and its LDS per block
 template <int QT>
 bool wave_q_built(int la, int lb, int lc, int ld, bool fp32) {
 #define CQX_CASE(A, B, C, D)                                                                  \

@@ -202,6 +202,31 @@ def test_fast_and_generic_kernels_agree():
         assert np.abs(w - g).max() < 1e-12
 
 
+def test_f_wave_and_generic_kernels_agree():
+    """Water cc-pVTZ (f on O, d on H): the templated wave kernels of the f
+    classes up to 1080 Cartesian components against the generic kernel for
+    every class, and both against the oracle.  Complex Hermitian J and K and
+    a non-Hermitian K: 1e-12 relative to the largest element between the
+    kernels, 1e-10 max abs against the oracle."""
+    meta, _ = load_fixture("contract_water_ccpvtz")
+    say("case: f wave vs generic, water cc-pVTZ")
+    mol, gbs, b, tpi, gto = make(meta)
+    nb = b.nBasis
+    rng = np.random.Generator(np.random.PCG64(31))
+    C_ = rng.normal(0, 1 / np.sqrt(nb), (nb, 5)) + 1j * rng.normal(0, 1 / np.sqrt(nb), (nb, 5))
+    D = C_ @ C_.conj().T
+    Xs, types, hers = [D, D, D], [COULOMB, EXCHANGE, EXCHANGE], [True, True, False]
+    wave = contract(gto, Xs, types, hers)   # f classes on the wave kernels
+    tpi.set_option("use_wave", 0)
+    gen = contract(gto, Xs, types, hers)    # every class beyond the lane-group ones generic
+    ref = OracleBasis(gbs, mol.Z, mol.xyz).contract(Xs, types, hers)
+    for w, g, r in zip(wave, gen, ref):
+        scale = np.abs(g).max()
+        say(f"  wave vs generic {np.abs(w - g).max() / scale:.2e} rel, vs oracle {np.abs(w - r).max():.2e}")
+        assert np.abs(w - g).max() < 1e-12 * scale
+        assert np.abs(w - r).max() < 1e-10
+
+
 @pytest.mark.parametrize("cplx", [False, True])
 def test_screening_keeps_exactly_the_reference_quartets(cplx):
     """The quartets evaluated on the GPU are exactly as many as the reference
