@@ -15,6 +15,7 @@
 #include "digest.hpp"
 #include "digest_tile.hpp"
 #include "kernels.hpp"
+#include "limits.hpp"
 #include "rys.hpp"
 
 namespace cqx {
@@ -280,7 +281,9 @@ __device__ __forceinline__ double quartet_scale(const PairRec &bp, const PairRec
 // quartet and combine the contracted Cartesian integrals with a butterfly of
 // lane shuffles; the wave's 64 / G quartets then go to LDS and all 64 lanes
 // digest them (digest_tile.hpp), so the registers of the evaluation and of
-// the digestion are never live together.
+// the digestion are never live together.  A block takes a run of
+// consecutive batches of 4 x 64 / G quartets (lane_group_blocks in
+// limits.hpp), one after the other, over one copy of the Rys table section.
 // R is the evaluation precision (Rys nodes, 2D integrals, the contracted
 // quartet); the digestion and the accumulators are FP64 either way.
 template <int LA, int LB, int LC, int LD, int G, class R>
@@ -298,42 +301,51 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
   // quartet count: the list's device counter for a launch sized to the
   // list's capacity before the list was built
   const long long nq = a.nq_dev ? min((long long)*a.nq_dev - a.nq_base, a.nq) : a.nq;
-  const long long q0 = ((long long)blockIdx.x * 4 + wv) * QPW;
-  if (q0 >= nq) return;  // uniform across the wave; no block barriers below
-  const long long q = q0 + qw;
-  const bool valid = q < nq;
-  Quartet qt{0u, 0u};
-  PairRec bp{}, kp{};
-  if (valid) {
-    qt = a.quartets[q];
-    bp = a.bra[qt.bra];
-    kp = a.ket[qt.ket_mask & 0xFFFFFFu];
-  }
-  R I[K::NC];
-  eval_quartet<LA, LB, LC, LD, G, R>(a.prims, a.primc32, rysn, bp, kp, g, I);
-  const double scale = quartet_scale(bp, kp);
-  double *sI = s_I[wv] + qw * K::NC;
+  // this block's run of batches: the launch's capacity a.nq in batches,
+  // split into gridDim.x runs of consecutive batches whose lengths differ
+  // by at most one
+  const long long nbatch = (a.nq + 4 * QPW - 1) / (4 * QPW);
+  const long long b0 = (long long)blockIdx.x * nbatch / gridDim.x;
+  const long long b1 = ((long long)blockIdx.x + 1) * nbatch / gridDim.x;
+  for (long long bt = b0; bt < b1; ++bt) {
+    const long long q0 = (bt * 4 + wv) * QPW;
+    if (q0 >= nq) break;  // uniform across the wave; no block barriers below
+    if (bt > b0) wave_sync();  // the previous batch's digestion has read the tile
+    const long long q = q0 + qw;
+    const bool valid = q < nq;
+    Quartet qt{0u, 0u};
+    PairRec bp{}, kp{};
+    if (valid) {
+      qt = a.quartets[q];
+      bp = a.bra[qt.bra];
+      kp = a.ket[qt.ket_mask & 0xFFFFFFu];
+    }
+    R I[K::NC];
+    eval_quartet<LA, LB, LC, LD, G, R>(a.prims, a.primc32, rysn, bp, kp, g, I);
+    const double scale = quartet_scale(bp, kp);
+    double *sI = s_I[wv] + qw * K::NC;
 #pragma unroll
-  for (int o = 0; o < K::NC; ++o)
-    if (o % G == g) sI[o] = (double)I[o] * scale;
-  if (g == 0) {
-    s_bf[wv][qw][0] = bp.bfa;
-    s_bf[wv][qw][1] = bp.bfb;
-    s_bf[wv][qw][2] = kp.bfa;
-    s_bf[wv][qw][3] = kp.bfb;
-    s_mask[wv][qw] = valid ? (qt.ket_mask >> 24) : 0u;
-    s_mol[wv][qw] = bp.mol;
+    for (int o = 0; o < K::NC; ++o)
+      if (o % G == g) sI[o] = (double)I[o] * scale;
+    if (g == 0) {
+      s_bf[wv][qw][0] = bp.bfa;
+      s_bf[wv][qw][1] = bp.bfb;
+      s_bf[wv][qw][2] = kp.bfa;
+      s_bf[wv][qw][3] = kp.bfb;
+      s_mask[wv][qw] = valid ? (qt.ket_mask >> 24) : 0u;
+      s_mol[wv][qw] = bp.mol;
+    }
+    wave_sync();
+    const long long left = nq - q0;
+    digest_tile<K>(s_I[wv], s_bf[wv], s_mask[wv], s_mol[wv], left < QPW ? (int)left : QPW, a.planes, lane);
   }
-  wave_sync();
-  const long long left = nq - q0;
-  digest_tile<K>(s_I[wv], s_bf[wv], s_mask[wv], s_mol[wv], left < QPW ? (int)left : QPW, a.planes, lane);
 }
 
 template <int LA, int LB, int LC, int LD, int G, class R>
 hipError_t launch_cls(const EriLaunch &a, hipStream_t s) {
-  constexpr long long per_block = 4 * (64 / G);
-  hipLaunchKernelGGL((eri_fast_kernel<LA, LB, LC, LD, G, R>), dim3((unsigned)((a.nq + per_block - 1) / per_block)),
-                     dim3(256), 0, s, a);
+  constexpr long long per_batch = 4 * (64 / G);
+  const long long blocks = lane_group_blocks((a.nq + per_batch - 1) / per_batch);
+  hipLaunchKernelGGL((eri_fast_kernel<LA, LB, LC, LD, G, R>), dim3((unsigned)blocks), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

@@ -34,12 +34,26 @@ inline long long quartets_per_block(int family, int shape) {
   return 1;  // team kernel: one wave per quartet, grid-stride
 }
 
+// A lane-group block takes a run of consecutive batches (a batch: the
+// quartets its 4 waves take at once), so that its copy of the Rys table
+// section into LDS serves more quartets: the batches are split into runs of
+// at most kLaneGroupRun (lengths differing by at most one), and a launch
+// keeps kLaneGroupMinBlocks blocks when it has that many batches.
+constexpr long long kLaneGroupRun = 16;
+constexpr long long kLaneGroupMinBlocks = 2048;
+inline long long lane_group_blocks(long long nbatch) {
+  if (nbatch <= kLaneGroupMinBlocks) return nbatch;
+  const long long by_run = (nbatch + kLaneGroupRun - 1) / kLaneGroupRun;
+  return by_run > kLaneGroupMinBlocks ? by_run : kLaneGroupMinBlocks;
+}
+
 // Blocks of one launch of nq quartets (nq <= kLaunchPiece).
 inline long long launch_blocks(long long nq, int family, int shape) {
   if (nq <= 0) return 0;
   if (family == kFamilyTeam) return nq < kTeamGridMax ? nq : kTeamGridMax;
   const long long per = quartets_per_block(family, shape);
-  return (nq + per - 1) / per;
+  const long long nbatch = (nq + per - 1) / per;
+  return family == kFamilyLaneGroup ? lane_group_blocks(nbatch) : nbatch;
 }
 
 // Pieces a list of nq quartets is launched in, and the largest launch's

@@ -28,6 +28,23 @@ int main() {
   expect(launch_blocks(n28, kFamilyTeam, 1) == kTeamGridMax, "team_grid_clamped");
   expect(plan_launches(0, kFamilyWave, 1).pieces == 0, "empty_list_no_launch");
   expect(plan_launches(1, kFamilyLaneGroup, 16).max_items == 256, "one_quartet_one_block");
+  // lane-group runs: short lists keep a block per batch, long ones give a
+  // block up to kLaneGroupRun batches and keep kLaneGroupMinBlocks blocks
+  expect(lane_group_blocks(100) == 100, "lanegroup_short_list_block_per_batch");
+  expect(lane_group_blocks(kLaneGroupMinBlocks + 1) == kLaneGroupMinBlocks, "lanegroup_min_blocks");
+  expect(lane_group_blocks(kLaneGroupMinBlocks * kLaneGroupRun * 4) == kLaneGroupMinBlocks * 4, "lanegroup_run_cap");
+  // the kernel's split: block i takes batches [i nb / b, (i + 1) nb / b)
+  for (long long nb : {1ll, 5000ll, 8193ll, 100000ll, 131073ll, 1ll << 21}) {
+    const long long b = lane_group_blocks(nb);
+    bool ok = b >= 1 && b <= nb;
+    long long next = 0;
+    for (long long i = 0; i < b && ok; ++i) {
+      const long long b0 = i * nb / b, b1 = (i + 1) * nb / b;
+      ok = b0 == next && b1 - b0 >= 1 && b1 - b0 <= kLaneGroupRun;
+      next = b1;
+    }
+    expect(ok && next == nb, "lanegroup_runs_cover_batches");
+  }
   // ket indices: a class of 2^24 pairs fits, one more does not
   expect(ket_index_fits((size_t)1 << 24), "ket_2^24_fits");
   expect(!ket_index_fits(((size_t)1 << 24) + 1), "ket_2^24+1_rejected");
