@@ -307,6 +307,13 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
   const long long nbatch = (a.nq + 4 * QPW - 1) / (4 * QPW);
   const long long b0 = (long long)blockIdx.x * nbatch / gridDim.x;
   const long long b1 = ((long long)blockIdx.x + 1) * nbatch / gridDim.x;
+  // each lane's list entry of the next batch, loaded before the current
+  // batch's digestion so that its pair-record loads need not wait on it
+  Quartet qn{0u, 0u};
+  {
+    const long long qf = (b0 * 4 + wv) * QPW + qw;
+    if (b0 < b1 && qf < nq) qn = a.quartets[qf];
+  }
   for (long long bt = b0; bt < b1; ++bt) {
     const long long q0 = (bt * 4 + wv) * QPW;
     if (q0 >= nq) break;  // uniform across the wave; no block barriers below
@@ -316,7 +323,7 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
     Quartet qt{0u, 0u};
     PairRec bp{}, kp{};
     if (valid) {
-      qt = a.quartets[q];
+      qt = qn;
       bp = a.bra[qt.bra];
       kp = a.ket[qt.ket_mask & 0xFFFFFFu];
     }
@@ -334,6 +341,10 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(EriLaunch a) {
       s_bf[wv][qw][3] = kp.bfb;
       s_mask[wv][qw] = valid ? (qt.ket_mask >> 24) : 0u;
       s_mol[wv][qw] = bp.mol;
+    }
+    {
+      const long long qf = ((bt + 1) * 4 + wv) * QPW + qw;
+      if (bt + 1 < b1 && qf < nq) qn = a.quartets[qf];
     }
     wave_sync();
     const long long left = nq - q0;
