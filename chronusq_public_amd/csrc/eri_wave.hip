@@ -1,8 +1,10 @@
 // Dispatch of the wave-per-quartet ERI kernels (eri_wave_impl.hpp): the
 // class, and the number of quartets a wave evaluates before digesting them
-// (forced, e.g. by the autotuner, or the class default).  The f classes are
-// built for FP64 with 1 and 2 quartets per wave.
+// (forced, e.g. by the autotuner, or the class default).  The f and g
+// classes are built for FP64 with 1 and 2 quartets per wave.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "kernels.hpp"
 
@@ -16,19 +18,27 @@ template <int QT>
 hipError_t launch_wave_fq(const EriLaunch &a, hipStream_t s);
 template <int QT>
 bool wave_fq_built(int la, int lb, int lc, int ld);
+template <int QT>
+hipError_t launch_wave_gq(const EriLaunch &a, hipStream_t s);
+template <int QT>
+bool wave_gq_built(int la, int lb, int lc, int ld);
 
-static bool has_f(int la, int lb, int lc, int ld) { return la == 3 || lb == 3 || lc == 3 || ld == 3; }
+static int max_l(int la, int lb, int lc, int ld) { return std::max(std::max(la, lb), std::max(lc, ld)); }
+
+static bool fg_built(int la, int lb, int lc, int ld, int qt) {
+  const bool g = max_l(la, lb, lc, ld) == 4;
+  if (qt == 1) return g ? wave_gq_built<1>(la, lb, lc, ld) : wave_fq_built<1>(la, lb, lc, ld);
+  if (qt == 2) return g ? wave_gq_built<2>(la, lb, lc, ld) : wave_fq_built<2>(la, lb, lc, ld);
+  return false;
+}
 
 bool eri_wave_available(int la, int lb, int lc, int ld) {
-  if (has_f(la, lb, lc, ld)) return wave_fq_built<1>(la, lb, lc, ld);
+  if (max_l(la, lb, lc, ld) >= 3) return fg_built(la, lb, lc, ld, 1);
   return wave_q_built<1>(la, lb, lc, ld, false);
 }
 
 bool eri_wave_qt_built(int la, int lb, int lc, int ld, int fp32, int qt) {
-  if (has_f(la, lb, lc, ld)) {
-    if (fp32) return false;
-    return qt == 1 ? wave_fq_built<1>(la, lb, lc, ld) : qt == 2 ? wave_fq_built<2>(la, lb, lc, ld) : false;
-  }
+  if (max_l(la, lb, lc, ld) >= 3) return !fp32 && fg_built(la, lb, lc, ld, qt);
   switch (qt) {
     case 1: return wave_q_built<1>(la, lb, lc, ld, fp32 != 0);
     case 2: return wave_q_built<2>(la, lb, lc, ld, fp32 != 0);
@@ -51,8 +61,9 @@ hipError_t launch_eri_wave(const EriLaunch &a, hipStream_t s) {
   if (a.store_mode) return hipErrorNotSupported;
   int qt = a.force_qt;
   if (!eri_wave_qt_built(a.la, a.lb, a.lc, a.ld, a.fp32, qt)) qt = eri_wave_default_qt(a.la, a.lb, a.lc, a.ld);
-  if (has_f(a.la, a.lb, a.lc, a.ld)) {
+  if (max_l(a.la, a.lb, a.lc, a.ld) >= 3) {
     if (!eri_wave_qt_built(a.la, a.lb, a.lc, a.ld, a.fp32, qt)) qt = 1;
+    if (max_l(a.la, a.lb, a.lc, a.ld) == 4) return qt == 2 ? launch_wave_gq<2>(a, s) : launch_wave_gq<1>(a, s);
     return qt == 2 ? launch_wave_fq<2>(a, s) : launch_wave_fq<1>(a, s);
   }
   switch (qt) {

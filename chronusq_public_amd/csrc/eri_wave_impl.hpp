@@ -1,6 +1,6 @@
 // Wave kernels for the larger s/p/d classes (Cartesian quartet size > 27)
-// and the classes with an f shell up to 1080 Cartesian components, templated
-// on the class and on QT, the quartets a wave takes together.
+// and the classes with an f or g shell up to 1080 Cartesian components,
+// templated on the class and on QT, the quartets a wave takes together.
 //
 // The primitive quartets of the wave's QT quartets form one list.  Each
 // round takes up to PP of them: lane group g (3 n_r lanes) builds the 2D
@@ -10,7 +10,8 @@
 // tables to its share of the contracted Cartesian quartet they belong to.
 // After the last round the QT quartets go to LDS, in the space the tables
 // took, and the lanes digest them together (digest_tile.hpp).  Instantiated
-// per QT (eri_wave_q*.hip; the f classes, FP64 only, in eri_wave_f*.hip).
+// per QT (eri_wave_q*.hip; the f and g classes, FP64 only, in eri_wave_f*.hip
+// and eri_wave_g*.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -282,16 +283,23 @@ hipError_t launch_wave_q(const EriLaunch &a, hipStream_t s) {
 }
 
 // classes with an f shell and at most 1080 Cartesian components (the
-// larger ones and the g classes take the generic kernel); FP64 only, since
-// the FP32 lists hold s/p/d classes alone
+// larger ones take the generic kernel); FP64 only, since the FP32 lists
+// hold s/p/d classes alone
 #define CQX_WAVE_F_CLASSES(X)                                                                   \
   X(3, 0, 0, 0) X(3, 0, 1, 0) X(3, 0, 1, 1) X(3, 0, 2, 0) X(3, 0, 2, 1) X(3, 0, 2, 2) X(3, 0, 3, 0) \
   X(3, 1, 0, 0) X(3, 1, 1, 0) X(3, 1, 1, 1) X(3, 1, 2, 0) X(3, 1, 2, 1) X(3, 1, 2, 2) X(3, 1, 3, 0) \
   X(3, 1, 3, 1) X(3, 2, 0, 0) X(3, 2, 1, 0) X(3, 2, 1, 1) X(3, 2, 2, 0) X(3, 2, 2, 1) X(3, 2, 3, 0) \
   X(3, 3, 0, 0) X(3, 3, 1, 0) X(3, 3, 1, 1) X(3, 3, 2, 0) X(3, 3, 3, 0)
 
+// classes with a g shell and at most 1080 Cartesian components, FP64 only
+#define CQX_WAVE_G_CLASSES(X)                                                                   \
+  X(4, 0, 0, 0) X(4, 0, 1, 0) X(4, 0, 1, 1) X(4, 0, 2, 0) X(4, 0, 2, 1) X(4, 0, 2, 2) X(4, 0, 3, 0) \
+  X(4, 0, 3, 1) X(4, 0, 3, 2) X(4, 0, 4, 0) X(4, 1, 0, 0) X(4, 1, 1, 0) X(4, 1, 1, 1) X(4, 1, 2, 0) \
+  X(4, 1, 2, 1) X(4, 1, 3, 0) X(4, 1, 4, 0) X(4, 2, 0, 0) X(4, 2, 1, 0) X(4, 2, 1, 1) X(4, 2, 2, 0) \
+  X(4, 2, 3, 0) X(4, 3, 0, 0) X(4, 3, 1, 0) X(4, 3, 2, 0) X(4, 4, 0, 0) X(4, 4, 1, 0)
+
 template <int LA, int LB, int LC, int LD, int QT>
-hipError_t launch_wave_f_cls(const EriLaunch &a, hipStream_t s) {
+hipError_t launch_wave_fp64_cls(const EriLaunch &a, hipStream_t s) {
   using K = Cls<LA, LB, LC, LD>;
   constexpr long long per_block = (long long)kWavesPerBlock * QT;
   const long long blocks = (a.nq + per_block - 1) / per_block;
@@ -308,7 +316,7 @@ hipError_t launch_wave_f_cls(const EriLaunch &a, hipStream_t s) {
 template <int QT>
 hipError_t launch_wave_fq(const EriLaunch &a, hipStream_t s) {
 #define CQX_CASE(A, B, C, D) \
-  if (a.la == A && a.lb == B && a.lc == C && a.ld == D) return launch_wave_f_cls<A, B, C, D, QT>(a, s);
+  if (a.la == A && a.lb == B && a.lc == C && a.ld == D) return launch_wave_fp64_cls<A, B, C, D, QT>(a, s);
   CQX_WAVE_F_CLASSES(CQX_CASE)
 #undef CQX_CASE
   return hipErrorNotSupported;
@@ -320,6 +328,25 @@ bool wave_fq_built(int la, int lb, int lc, int ld) {
 #define CQX_CASE(A, B, C, D) \
   if (la == A && lb == B && lc == C && ld == D) return wave_lds_bytes<Cls<A, B, C, D>, double, QT>() <= kWaveLdsMax;
   CQX_WAVE_F_CLASSES(CQX_CASE)
+#undef CQX_CASE
+  return false;
+}
+
+// the same for the g classes
+template <int QT>
+hipError_t launch_wave_gq(const EriLaunch &a, hipStream_t s) {
+#define CQX_CASE(A, B, C, D) \
+  if (a.la == A && a.lb == B && a.lc == C && a.ld == D) return launch_wave_fp64_cls<A, B, C, D, QT>(a, s);
+  CQX_WAVE_G_CLASSES(CQX_CASE)
+#undef CQX_CASE
+  return hipErrorNotSupported;
+}
+
+template <int QT>
+bool wave_gq_built(int la, int lb, int lc, int ld) {
+#define CQX_CASE(A, B, C, D) \
+  if (la == A && lb == B && lc == C && ld == D) return wave_lds_bytes<Cls<A, B, C, D>, double, QT>() <= kWaveLdsMax;
+  CQX_WAVE_G_CLASSES(CQX_CASE)
 #undef CQX_CASE
   return false;
 }

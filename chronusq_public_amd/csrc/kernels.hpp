@@ -50,12 +50,12 @@ bool eri_fast_available(int la, int lb, int lc, int ld);
 // (pp|ps), (ds|ss), (ds|ps), (dp|ss); -1 for the other classes
 int fast_class_index(int la, int lb, int lc, int ld);
 // One-wave-per-quartet templated kernels for the larger s/p/d classes and
-// the f classes of at most 1080 Cartesian components (FP64).
+// the f and g classes of at most 1080 Cartesian components (FP64).
 hipError_t launch_eri_wave(const EriLaunch &a, hipStream_t s);
 bool eri_wave_available(int la, int lb, int lc, int ld);
 // whether the wave kernel of a class is built with qt (1, 2, 4, 8) quartets
 // per wave for the precision (its LDS per block within the limit; the f
-// classes: FP64 with 1 or 2)
+// and g classes: FP64 with 1 or 2)
 bool eri_wave_qt_built(int la, int lb, int lc, int ld, int fp32, int qt);
 
 // Nuclear-gradient J/K contraction over every quartet of one class pair

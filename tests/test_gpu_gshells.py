@@ -1,4 +1,5 @@
 """g shells (l = 4) on the GPU: water cc-pVQZ (g on O, f on H) through the
+templated wave kernels (classes up to 1080 Cartesian components) and the
 generic kernel, which builds quartets too large for LDS in passes over the
 first shell's components and the roots (Rys rules up to nine roots).
 Against the oracle (its g integrals pinned in test_oracle_g.py): the shell
@@ -97,6 +98,33 @@ def test_qz_contraction_against_oracle(case):
         say(f"  max |d| {err:.2e} (max |AX| {np.abs(r).max():.2e})")
         assert err < 1e-10
     assert tpi.last_stats()["n_quartets"] > 0
+
+
+def test_qz_wave_and_generic_kernels_agree():
+    """The templated wave kernels of the f and g classes up to 1080
+    Cartesian components against the generic kernel for every class beyond
+    the lane-group ones: complex Hermitian J and K and a non-Hermitian K of
+    water cc-pVQZ, 1e-12 relative to the largest element."""
+    cqx, mol, gbs, b, tpi, ob = _setup()
+    say("case: water cc-pVQZ wave vs generic")
+    gto = cqx.GTODirectTPIContraction(tpi)
+    nb = b.nBasis
+    rng = np.random.Generator(np.random.PCG64(9))
+    C = rng.normal(0, 1 / np.sqrt(nb), (nb, 5)) + 1j * rng.normal(0, 1 / np.sqrt(nb), (nb, 5))
+    D = np.asfortranarray(C @ C.conj().T)
+    Xs, types, hers = [D, D, D], [COULOMB, EXCHANGE, EXCHANGE], [True, True, False]
+
+    def run():
+        AX = [np.zeros((nb, nb), dtype=np.complex128, order="F") for _ in Xs]
+        gto.twoBodyContract([cqx.TwoBodyContraction(x, a, h, t) for x, a, t, h in zip(Xs, AX, types, hers)])
+        return AX
+    wave = run()
+    tpi.set_option("use_wave", 0)
+    gen = run()
+    for w, g in zip(wave, gen):
+        scale = np.abs(g).max()
+        say(f"  wave vs generic {np.abs(w - g).max() / scale:.2e} rel")
+        assert np.abs(w - g).max() < 1e-12 * scale
 
 
 def test_qz_batched_mixed_and_recorded():
