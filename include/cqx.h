@@ -215,17 +215,19 @@ int cqx_grad_two_body_contract(cqx_ctx *ctx, int ngrad, int nmat, const cqx_cont
 /* Tuning knobs: "prim_screen" (primitive-pair drop threshold, default
  * 1e-18; rebuilds the pair data), "fast_max_nc" (largest Cartesian quartet
  * size routed to the lane-group kernels, default 27), "use_wave" (0 routes
- * the larger s/p/d classes to the generic kernel instead of the templated
+ * the larger s/p/d classes and the f and g classes of at most 1080
+ * Cartesian components to the generic kernel instead of the templated
  * wave-per-quartet ones), "fast_lanes" (lanes per shell quartet of the
  * lane-group kernels: 1, 2, 4, 8 or 16; 0, the default, picks it per launch
  * from the mean primitive count), "autotune" (1: the next contraction
  * times every lane count of every lane-group launch and every number of
  * quartets a wave kernel evaluates before digesting them (1, 2, 4, 8) on
- * scratch accumulators, keeps the fastest per class pair and precision and
+ * scratch accumulators (1 or 2 for the f and g classes), keeps the fastest
+ * per class pair and precision and
  * uses it from then on; 0 forgets the choices), "fp32_cut" (mixed precision: quartets whose
  * Schwarz product Q_ab Q_cd is below this value are evaluated in FP32 with
  * an FP32 Rys table, digested and accumulated in FP64; default 0 = every
- * quartet in FP64; classes with an f shell always stay FP64), "list_cap"
+ * quartet in FP64; classes with an f or g shell always stay FP64), "list_cap"
  * (screened quartet-list entries built per batch, default 2^28: the list
  * workspace is 8 B per entry, twice that with fp32_cut, whatever the
  * molecule size), "graph" (1: a build records its launches, from the
