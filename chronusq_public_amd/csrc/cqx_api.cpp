@@ -1784,14 +1784,22 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
       };
       // time a launch's shapes on scratch accumulators (results discarded)
       // and return the fastest
-      auto time_shapes = [&](const std::vector<int> &shapes, const std::function<void(int)> &launch) {
+      // Each shape is timed with the launch replicated on the build's
+      // streams at once: a build overlaps the launches of several class
+      // pairs, and the shape that is fastest alone on the device is not
+      // always the fastest beside others.
+      auto time_shapes = [&](const std::vector<int> &shapes, const std::function<void(int, hipStream_t)> &launch) {
         float best = 1e30f;
         int best_g = shapes.empty() ? 0 : shapes[0];
+        const int nrep = c->nstreams;
+        ensure_side(nrep);
         for (int g : shapes) {
           float ms = 0.0f;
           for (int rep = 0; rep < 2; ++rep) {  // the second run is timed
             HIPCHK(hipEventRecord(c->ev_t0, s));
-            launch(g);
+            fork_side(nrep);
+            for (int i = 0; i < nrep; ++i) launch(g, stream_of(i, nrep));
+            join_side(nrep);
             HIPCHK(hipEventRecord(c->ev_t1, s));
             HIPCHK(hipEventSynchronize(c->ev_t1));
             HIPCHK(hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1));
@@ -1844,9 +1852,9 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
                   if (!(g == 1 && nc > 9)) shapes.push_back(g);
                 cqx::TileEriLaunch TT = T;
                 TT.planes = ps_tune;
-                c->tuned_lanes[key] = time_shapes(shapes, [&](int g) {
+                c->tuned_lanes[key] = time_shapes(shapes, [&](int g, hipStream_t st) {
                   TT.force_g = g;
-                  HIPCHK(cqx::launch_eri_tile(TT, s));
+                  HIPCHK(cqx::launch_eri_tile(TT, st));
                 });
               }
               auto it = c->tuned_lanes.find(key);
@@ -1958,9 +1966,9 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
                 for (int qt : {1, 2, 4, 8})
                   if (cqx::eri_wave_qt_built(E.la, E.lb, E.lc, E.ld, E.fp32, qt)) shapes.push_back(qt);
               }
-              c->tuned_lanes[key] = time_shapes(shapes, [&](int g) {
+              c->tuned_lanes[key] = time_shapes(shapes, [&](int g, hipStream_t st) {
                 (rt == 0 ? T.force_g : T.force_qt) = g;
-                launch_eri(T, s);
+                launch_eri(T, st);
               });
             }
             apply_tuned(E);
