@@ -1,7 +1,8 @@
 // Dispatch of the wave-per-quartet ERI kernels (eri_wave_impl.hpp): the
 // class, and the number of quartets a wave evaluates before digesting them
 // (forced, e.g. by the autotuner, or the class default).  The f and g
-// classes are built for FP64 with 1 and 2 quartets per wave.
+// classes are built for FP64 with 1 and 2 quartets per wave, and 4 up to
+// 216 Cartesian components.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,6 +30,7 @@ static bool fg_built(int la, int lb, int lc, int ld, int qt) {
   const bool g = max_l(la, lb, lc, ld) == 4;
   if (qt == 1) return g ? wave_gq_built<1>(la, lb, lc, ld) : wave_fq_built<1>(la, lb, lc, ld);
   if (qt == 2) return g ? wave_gq_built<2>(la, lb, lc, ld) : wave_fq_built<2>(la, lb, lc, ld);
+  if (qt == 4) return g ? wave_gq_built<4>(la, lb, lc, ld) : wave_fq_built<4>(la, lb, lc, ld);
   return false;
 }
 
@@ -63,8 +65,9 @@ hipError_t launch_eri_wave(const EriLaunch &a, hipStream_t s) {
   if (!eri_wave_qt_built(a.la, a.lb, a.lc, a.ld, a.fp32, qt)) qt = eri_wave_default_qt(a.la, a.lb, a.lc, a.ld);
   if (max_l(a.la, a.lb, a.lc, a.ld) >= 3) {
     if (!eri_wave_qt_built(a.la, a.lb, a.lc, a.ld, a.fp32, qt)) qt = 1;
-    if (max_l(a.la, a.lb, a.lc, a.ld) == 4) return qt == 2 ? launch_wave_gq<2>(a, s) : launch_wave_gq<1>(a, s);
-    return qt == 2 ? launch_wave_fq<2>(a, s) : launch_wave_fq<1>(a, s);
+    if (max_l(a.la, a.lb, a.lc, a.ld) == 4)
+      return qt == 4 ? launch_wave_gq<4>(a, s) : qt == 2 ? launch_wave_gq<2>(a, s) : launch_wave_gq<1>(a, s);
+    return qt == 4 ? launch_wave_fq<4>(a, s) : qt == 2 ? launch_wave_fq<2>(a, s) : launch_wave_fq<1>(a, s);
   }
   switch (qt) {
     case 1: return launch_wave_q<1>(a, s);

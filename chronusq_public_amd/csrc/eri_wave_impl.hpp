@@ -11,7 +11,7 @@
 // After the last round the QT quartets go to LDS, in the space the tables
 // took, and the lanes digest them together (digest_tile.hpp).  Instantiated
 // per QT (eri_wave_q*.hip; the f and g classes, FP64 only, in eri_wave_f*.hip
-// and eri_wave_g*.hip).
+// and eri_wave_g*.hip, 1, 2 and, up to 216 components, 4 quartets per wave).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -254,6 +254,27 @@ hipError_t launch_wave_cls(const EriLaunch &a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// FP64-only variants (the f and g classes): 1 and 2 quartets per wave for
+// every class, 4 for those of at most 216 Cartesian components
+template <class K, int QT>
+constexpr bool fp64_variant_built() {
+  return (QT <= 2 || K::NC <= 216) && wave_lds_bytes<K, double, QT>() <= kWaveLdsMax;
+}
+
+template <int LA, int LB, int LC, int LD, int QT>
+hipError_t launch_wave_fp64_cls(const EriLaunch &a, hipStream_t s) {
+  using K = Cls<LA, LB, LC, LD>;
+  constexpr long long per_block = (long long)kWavesPerBlock * QT;
+  const long long blocks = (a.nq + per_block - 1) / per_block;
+  if (a.fp32) return hipErrorNotSupported;
+  if constexpr (fp64_variant_built<K, QT>())
+    hipLaunchKernelGGL((eri_wave_kernel<LA, LB, LC, LD, double, QT>), dim3((unsigned)blocks),
+                       dim3(64 * kWavesPerBlock), 0, s, a);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
 }  // namespace
 
 // s/p/d classes with more than 27 Cartesian components
@@ -298,20 +319,6 @@ hipError_t launch_wave_q(const EriLaunch &a, hipStream_t s) {
   X(4, 1, 2, 1) X(4, 1, 3, 0) X(4, 1, 4, 0) X(4, 2, 0, 0) X(4, 2, 1, 0) X(4, 2, 1, 1) X(4, 2, 2, 0) \
   X(4, 2, 3, 0) X(4, 3, 0, 0) X(4, 3, 1, 0) X(4, 3, 2, 0) X(4, 4, 0, 0) X(4, 4, 1, 0)
 
-template <int LA, int LB, int LC, int LD, int QT>
-hipError_t launch_wave_fp64_cls(const EriLaunch &a, hipStream_t s) {
-  using K = Cls<LA, LB, LC, LD>;
-  constexpr long long per_block = (long long)kWavesPerBlock * QT;
-  const long long blocks = (a.nq + per_block - 1) / per_block;
-  if (a.fp32) return hipErrorNotSupported;
-  if constexpr (wave_lds_bytes<K, double, QT>() <= kWaveLdsMax)
-    hipLaunchKernelGGL((eri_wave_kernel<LA, LB, LC, LD, double, QT>), dim3((unsigned)blocks),
-                       dim3(64 * kWavesPerBlock), 0, s, a);
-  else
-    return hipErrorInvalidValue;
-  return hipGetLastError();
-}
-
 // every f wave class with QT quartets per wave
 template <int QT>
 hipError_t launch_wave_fq(const EriLaunch &a, hipStream_t s) {
@@ -326,7 +333,7 @@ hipError_t launch_wave_fq(const EriLaunch &a, hipStream_t s) {
 template <int QT>
 bool wave_fq_built(int la, int lb, int lc, int ld) {
 #define CQX_CASE(A, B, C, D) \
-  if (la == A && lb == B && lc == C && ld == D) return wave_lds_bytes<Cls<A, B, C, D>, double, QT>() <= kWaveLdsMax;
+  if (la == A && lb == B && lc == C && ld == D) return fp64_variant_built<Cls<A, B, C, D>, QT>();
   CQX_WAVE_F_CLASSES(CQX_CASE)
 #undef CQX_CASE
   return false;
@@ -345,7 +352,7 @@ hipError_t launch_wave_gq(const EriLaunch &a, hipStream_t s) {
 template <int QT>
 bool wave_gq_built(int la, int lb, int lc, int ld) {
 #define CQX_CASE(A, B, C, D) \
-  if (la == A && lb == B && lc == C && ld == D) return wave_lds_bytes<Cls<A, B, C, D>, double, QT>() <= kWaveLdsMax;
+  if (la == A && lb == B && lc == C && ld == D) return fp64_variant_built<Cls<A, B, C, D>, QT>();
   CQX_WAVE_G_CLASSES(CQX_CASE)
 #undef CQX_CASE
   return false;

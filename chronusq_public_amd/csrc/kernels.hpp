@@ -55,7 +55,7 @@ hipError_t launch_eri_wave(const EriLaunch &a, hipStream_t s);
 bool eri_wave_available(int la, int lb, int lc, int ld);
 // whether the wave kernel of a class is built with qt (1, 2, 4, 8) quartets
 // per wave for the precision (its LDS per block within the limit; the f
-// and g classes: FP64 with 1 or 2)
+// and g classes: FP64 with 1 or 2, and 4 up to 216 Cartesian components)
 bool eri_wave_qt_built(int la, int lb, int lc, int ld, int fp32, int qt);
 
 // Nuclear-gradient J/K contraction over every quartet of one class pair

@@ -222,7 +222,8 @@ int cqx_grad_two_body_contract(cqx_ctx *ctx, int ngrad, int nmat, const cqx_cont
  * from the mean primitive count), "autotune" (1: the next contraction
  * times every lane count of every lane-group launch and every number of
  * quartets a wave kernel evaluates before digesting them (1, 2, 4, 8) on
- * scratch accumulators (1 or 2 for the f and g classes), each replicated
+ * scratch accumulators (1, 2 or, up to 216 Cartesian components, 4 for
+ * the f and g classes), each replicated
  * on the build's streams at once, keeps the fastest
  * per class pair and precision and
  * uses it from then on; 0 forgets the choices), "fp32_cut" (mixed precision: quartets whose
