@@ -1618,10 +1618,28 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
       }
       c->stats.n_candidates += total;
       for (int k = 0; k < nt; ++k) c->stats.n_candidates += (long long)tcount[3 * k + 2];
-      // kernel family of a class: 0 lane-group, 1 wave-per-quartet, 2 generic
-      auto route = [&](int la, int lb, int lc, int ld) {
+      // autotuned shapes: key of a class and precision, and the value tag of
+      // a wave shape for the classes built in both families
+      constexpr int kB = cqx::kMaxLDev + 1;
+      constexpr int kWaveTag = 64;
+      auto class_key = [&](int la, int lb, int lc, int ld, int f32) {
+        return ((((la * kB + lb) * kB + lc) * kB + ld) << 1) | (f32 ? 1 : 0);
+      };
+      // classes with both a lane-group and a wave kernel, whose family the
+      // autotuner picks (without a tuned choice: the lane-group kernels up to
+      // fast_max_nc components)
+      auto dual = [&](int la, int lb, int lc, int ld) {
+        return c->use_wave && cqx::eri_lane_group_dual(la, lb, lc, ld) && cqx::eri_wave_available(la, lb, lc, ld);
+      };
+      // kernel family of a class: 0 lane-group, 1 wave-per-quartet, 2 generic;
+      // f32 < 0 asks for the family without a tuned choice
+      auto route = [&](int la, int lb, int lc, int ld, int f32 = -1) {
         const int ncq = cqx::ncart(la) * cqx::ncart(lb) * cqx::ncart(lc) * cqx::ncart(ld);
-        if (ncq <= c->fast_max_nc && cqx::eri_fast_available(la, lb, lc, ld)) return 0;
+        if (f32 >= 0 && dual(la, lb, lc, ld)) {
+          auto it = c->tuned_lanes.find(class_key(la, lb, lc, ld, f32));
+          if (it != c->tuned_lanes.end()) return it->second >= kWaveTag ? 1 : 0;
+        }
+        if (ncq <= c->fast_max_nc && cqx::eri_lane_group_built(la, lb, lc, ld)) return 0;
         if (c->use_wave && cqx::eri_wave_available(la, lb, lc, ld)) return 1;
         return 2;
       };
@@ -1652,22 +1670,24 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
       // autotuned shape of a launch, per class pair and precision: the lanes
       // per quartet of a lane-group kernel, the quartets per wave of a wave
       // kernel
+      // (a class built in both families keeps its choice under the plain key,
+      // tagged kWaveTag + QT for a wave shape)
       auto tune_key = [&](const cqx::EriLaunch &E) {
-        constexpr int B = cqx::kMaxLDev + 1;
-        const int k = ((((E.la * B + E.lb) * B + E.lc) * B + E.ld) << 1) | (E.fp32 ? 1 : 0);
-        return route(E.la, E.lb, E.lc, E.ld) == 1 ? k + 2 * B * B * B * B : k;
+        const int k = class_key(E.la, E.lb, E.lc, E.ld, E.fp32);
+        if (dual(E.la, E.lb, E.lc, E.ld)) return k;
+        return route(E.la, E.lb, E.lc, E.ld) == 1 ? k + 2 * kB * kB * kB * kB : k;
       };
       auto apply_tuned = [&](cqx::EriLaunch &E) {
-        const int r = route(E.la, E.lb, E.lc, E.ld);
+        const int r = route(E.la, E.lb, E.lc, E.ld, E.fp32);
         auto it = c->tuned_lanes.find(tune_key(E));
         if (it == c->tuned_lanes.end()) return;
-        if (r == 0 && !c->fast_lanes) E.force_g = it->second;
-        if (r == 1) E.force_qt = it->second;
+        const int v = it->second >= kWaveTag ? it->second - kWaveTag : it->second;
+        if (r == 0 && !c->fast_lanes) E.force_g = v;
+        if (r == 1) E.force_qt = v;
       };
       // one list in launches of at most kLaunchPiece quartets, so that no
-      // grid reaches 2^32 work-items (limits.hpp)
-      auto launch_eri = [&](const cqx::EriLaunch &E, hipStream_t st) {
-        const int r = route(E.la, E.lb, E.lc, E.ld);
+      // grid reaches 2^32 work-items (limits.hpp), on the kernels of family r
+      auto launch_family = [&](const cqx::EriLaunch &E, int r, hipStream_t st) {
         if (r == 2 && E.fp32) throw std::runtime_error("FP32 quartet list for a class without an FP32 kernel");
         // the shape a launch picks for itself is at worst the smallest tile
         const int shape = r == 0 ? (E.force_g > 0 ? E.force_g : 16) : r == 1 ? (E.force_qt > 0 ? E.force_qt : 1) : 1;
@@ -1679,6 +1699,9 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
           P.nq_base = b;
           HIPCHK(r == 0 ? cqx::launch_eri_fast(P, st) : r == 1 ? cqx::launch_eri_wave(P, st) : cqx::launch_eri_team(P, st));
         }
+      };
+      auto launch_eri = [&](const cqx::EriLaunch &E, hipStream_t st) {
+        launch_family(E, route(E.la, E.lb, E.lc, E.ld, E.fp32), st);
       };
       // side streams for concurrent class-pair launches: created outside any
       // capture, forked from the context stream before the launches and
@@ -1952,23 +1975,32 @@ int contract(cqx_ctx *c, int screen, int ncon, const cqx_contraction *list, int 
           const int rt = route(E.la, E.lb, E.lc, E.ld);
           if (rt <= 1) {
             const int key = tune_key(E);
-            if (tuning && !(rt == 0 && c->fast_lanes) && !c->tuned_lanes.count(key)) {
+            const bool both = dual(E.la, E.lb, E.lc, E.ld);  // the family is tuned too
+            if (tuning && (both || !(rt == 0 && c->fast_lanes)) && !c->tuned_lanes.count(key)) {
               // time each shape on scratch accumulators (results discarded)
               prepare_tune();
               cqx::EriLaunch T = E;
               T.planes = ps_tune;
               const int nc = cqx::ncart(E.la) * cqx::ncart(E.lb) * cqx::ncart(E.lc) * cqx::ncart(E.ld);
+              // lane counts of the lane-group kernel (a value 1-16) and quartets
+              // per wave of the wave kernel (kWaveTag + QT for a class built in
+              // both families, QT otherwise)
               std::vector<int> shapes;
-              if (rt == 0) {
-                for (int g : {1, 2, 4, 8, 16})
-                  if (!(g == 1 && nc > 9)) shapes.push_back(g);  // one lane per quartet: LDS tile too large
-              } else {
-                for (int qt : {1, 2, 4, 8})
-                  if (cqx::eri_wave_qt_built(E.la, E.lb, E.lc, E.ld, E.fp32, qt)) shapes.push_back(qt);
+              if (rt == 0 || both) {
+                if (c->fast_lanes) shapes.push_back(c->fast_lanes);
+                else
+                  for (int g : {1, 2, 4, 8, 16})
+                    if (!(g == 1 && nc > 9)) shapes.push_back(g);  // one lane per quartet: LDS tile too large
               }
-              c->tuned_lanes[key] = time_shapes(shapes, [&](int g, hipStream_t st) {
-                (rt == 0 ? T.force_g : T.force_qt) = g;
-                launch_eri(T, st);
+              if (rt == 1 || both)
+                for (int qt : {1, 2, 4, 8})
+                  if (cqx::eri_wave_qt_built(E.la, E.lb, E.lc, E.ld, E.fp32, qt))
+                    shapes.push_back(both ? kWaveTag + qt : qt);
+              c->tuned_lanes[key] = time_shapes(shapes, [&](int v, hipStream_t st) {
+                const bool wave = both ? v >= kWaveTag : rt == 1;
+                if (wave) T.force_qt = both ? v - kWaveTag : v;
+                else T.force_g = v;
+                launch_family(T, wave ? 1 : 0, st);
               });
             }
             apply_tuned(E);

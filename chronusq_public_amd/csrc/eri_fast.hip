@@ -29,6 +29,17 @@ int fast_class_index(int la, int lb, int lc, int ld) {
 
 bool eri_fast_available(int la, int lb, int lc, int ld) { return fast_class_index(la, lb, lc, ld) >= 0; }
 
+bool eri_lane_group_dual(int la, int lb, int lc, int ld) {
+  const int q[4][4] = {{2, 0, 2, 0}, {2, 2, 0, 0}, {2, 1, 1, 0}, {2, 0, 1, 1}};
+  for (int k = 0; k < 4; ++k)
+    if (q[k][0] == la && q[k][1] == lb && q[k][2] == lc && q[k][3] == ld) return true;
+  return false;
+}
+
+bool eri_lane_group_built(int la, int lb, int lc, int ld) {
+  return eri_fast_available(la, lb, lc, ld) || eri_lane_group_dual(la, lb, lc, ld);
+}
+
 hipError_t launch_eri_fast(const EriLaunch &a, hipStream_t s) {
   if (a.nq <= 0) return hipSuccess;
   if (a.store_mode) return hipErrorNotSupported;

@@ -1,6 +1,7 @@
 // Lane-group Rys ERI + J/K digestion kernels for the low angular-momentum
-// classes (Cartesian quartet size <= 27), templated on the class and on the
-// number G of lanes that share one shell quartet.
+// classes (Cartesian quartet size <= 27, and four classes of 36 and 54
+// components that also have wave kernels), templated on the class and on
+// the number G of lanes that share one shell quartet.
 //
 // Every size is a compile-time constant, so the Rys nodes/weights, the 2D
 // recursions and the contracted Cartesian quartet stay in registers; the
@@ -374,12 +375,22 @@ hipError_t launch_cls(const EriLaunch &a, hipStream_t s) {
   X(2, 0, 1, 0)             \
   X(2, 1, 0, 0)
 
-// all classes of the list for one lane-group size G and precision R
+// classes of 36 and 54 components built in both families: the autotuner
+// picks the lane-group or the wave kernel for each (the lane-group kernels
+// win with one density plane per kind, the wave kernels with eight)
+#define CQX_FAST_DUAL_CLASSES(X) \
+  X(2, 0, 2, 0)                  \
+  X(2, 2, 0, 0)                  \
+  X(2, 1, 1, 0)                  \
+  X(2, 0, 1, 1)
+
+// all classes of the lists for one lane-group size G and precision R
 template <int G, class R>
 hipError_t launch_fast_g(const EriLaunch &a, hipStream_t s) {
 #define CQX_CASE(A, B, C, D) \
   if (a.la == A && a.lb == B && a.lc == C && a.ld == D) return launch_cls<A, B, C, D, G, R>(a, s);
   CQX_FAST_CLASSES(CQX_CASE)
+  CQX_FAST_DUAL_CLASSES(CQX_CASE)
 #undef CQX_CASE
   return hipErrorNotSupported;
 }

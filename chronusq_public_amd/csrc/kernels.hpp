@@ -46,6 +46,11 @@ bool team_shape(int la, int lb, int lc, int ld, int lds_max_bytes, int &ca, int 
 // hipErrorNotSupported when the class has no specialised kernel.
 hipError_t launch_eri_fast(const EriLaunch &a, hipStream_t s);
 bool eri_fast_available(int la, int lb, int lc, int ld);
+// (ds|ds), (dd|ss), (dp|ps), (ds|pp): lane-group kernels beside their wave
+// kernels (the autotuner picks the family); eri_lane_group_built is true for
+// these and the eight classes of eri_fast_available (which alone take tiles)
+bool eri_lane_group_dual(int la, int lb, int lc, int ld);
+bool eri_lane_group_built(int la, int lb, int lc, int ld);
 // position of a lane-group class in (ss|ss), (ps|ss), (ps|ps), (pp|ss),
 // (pp|ps), (ds|ss), (ds|ps), (dp|ss); -1 for the other classes
 int fast_class_index(int la, int lb, int lc, int ld);

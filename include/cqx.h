@@ -214,7 +214,8 @@ int cqx_grad_two_body_contract(cqx_ctx *ctx, int ngrad, int nmat, const cqx_cont
 
 /* Tuning knobs: "prim_screen" (primitive-pair drop threshold, default
  * 1e-18; rebuilds the pair data), "fast_max_nc" (largest Cartesian quartet
- * size routed to the lane-group kernels, default 27), "use_wave" (0 routes
+ * size routed to the lane-group kernels without a tuned choice, default
+ * 27), "use_wave" (0 routes
  * the larger s/p/d classes and the f and g classes of at most 1080
  * Cartesian components to the generic kernel instead of the templated
  * wave-per-quartet ones), "fast_lanes" (lanes per shell quartet of the
@@ -223,10 +224,11 @@ int cqx_grad_two_body_contract(cqx_ctx *ctx, int ngrad, int nmat, const cqx_cont
  * times every lane count of every lane-group launch and every number of
  * quartets a wave kernel evaluates before digesting them (1, 2, 4, 8) on
  * scratch accumulators (1, 2 or, up to 216 Cartesian components, 4 for
- * the f and g classes), each replicated
- * on the build's streams at once, keeps the fastest
- * per class pair and precision and
- * uses it from then on; 0 forgets the choices), "fp32_cut" (mixed precision: quartets whose
+ * the f and g classes), each replicated on the build's streams at once;
+ * for (ds|ds), (dd|ss), (dp|ps) and (ds|pp), built in both families, the
+ * lane-group and wave shapes compete; keeps the fastest per class pair and
+ * precision and uses it from then on; 0 forgets the choices), "fp32_cut"
+ * (mixed precision: quartets whose
  * Schwarz product Q_ab Q_cd is below this value are evaluated in FP32 with
  * an FP32 Rys table, digested and accumulated in FP64; default 0 = every
  * quartet in FP64; classes with an f or g shell always stay FP64), "list_cap"

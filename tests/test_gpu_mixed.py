@@ -78,8 +78,9 @@ def test_mixed_alkane_rhf_within_microhartree():
 def test_all_fp32_every_kernel_and_plane_kind():
     """Every s/p/d quartet in FP32 (cut above every Schwarz product) on
     (H2O)_2 / 6-31G(d), complex density, Hermitian and non-Hermitian J and
-    K, through the lane-group and the wave kernels: within 1e-5 of the
-    largest element of the FP64 result."""
+    K, through the lane-group and the wave kernels (the classes of 36 and 54
+    components on either): within 1e-5 of the largest element of the FP64
+    result."""
     from chronusq_public_amd.chem import basis_text, water_cluster
     mol, gbs = water_cluster(2, 4.0, seed=5), basis_text("6-31G(d)")
     say("case: all-FP32 water2 6-31G(d)")
@@ -92,7 +93,9 @@ def test_all_fp32_every_kernel_and_plane_kind():
     Xs = [D, D, X, X]
     types, hers = [COULOMB, EXCHANGE, COULOMB, EXCHANGE], [True, True, False, False]
     f64 = _contract(gto, Xs, types, hers)
-    for fast_max_nc in (27, 0):  # lane-group + wave kernels, then wave kernels only
+    # lane-group + wave kernels; the lane-group kernels for every class they
+    # are built for (through (dp|ps) and (ds|pp), 54 components); wave only
+    for fast_max_nc in (27, 54, 0):
         tpi.set_option("fast_max_nc", fast_max_nc)
         tpi.set_option("fp32_cut", 1e300)
         f32 = _contract(gto, Xs, types, hers)

@@ -202,6 +202,37 @@ def test_fast_and_generic_kernels_agree():
         assert np.abs(w - g).max() < 1e-12
 
 
+def test_lane_group_and_wave_kernels_agree_on_the_shared_classes():
+    """(ds|ds), (dd|ss), (dp|ps) and (ds|pp) are built in both the lane-group
+    and the wave family (the autotuner picks one): (H2O)_2 / 6-31G(d), complex
+    Hermitian J and K and non-Hermitian J and K, on the lane-group kernels
+    (fast_max_nc 54), on the wave kernels (the default without a tuned
+    choice) and with the autotuner's choice: 1e-12 max abs between them, and
+    1e-10 against the oracle."""
+    mol, gbs = _cluster(2, 4.0, 5, "6-31G(d)")
+    say("case: lane-group vs wave on the shared classes")
+    _, _, b, tpi, gto = make(mol, gbs)
+    nb = b.nBasis
+    rng = np.random.Generator(np.random.PCG64(13))
+    C_ = rng.normal(size=(nb, 5)) + 1j * rng.normal(size=(nb, 5))
+    D = C_ @ C_.conj().T
+    X = rng.normal(size=(nb, nb)) + 1j * rng.normal(size=(nb, nb))
+    Xs, types, hers = [D, D, X, X], [COULOMB, EXCHANGE, COULOMB, EXCHANGE], [True, True, False, False]
+    wave = contract(gto, Xs, types, hers)
+    tpi.set_option("fast_max_nc", 54)
+    lane = contract(gto, Xs, types, hers)
+    tpi.set_option("fast_max_nc", 27)
+    tpi.set_option("autotune", 1)
+    contract(gto, Xs, types, hers)  # the tuning build
+    tuned = contract(gto, Xs, types, hers)
+    ref = OracleBasis(gbs, mol.Z, mol.xyz).contract(Xs, types, hers)
+    for w, l, t, r in zip(wave, lane, tuned, ref):
+        say(f"  lane vs wave {np.abs(l - w).max():.2e}, tuned vs wave {np.abs(t - w).max():.2e}, vs oracle {np.abs(w - r).max():.2e}")
+        assert np.abs(l - w).max() < 1e-12
+        assert np.abs(t - w).max() < 1e-12
+        assert np.abs(w - r).max() < 1e-10
+
+
 def test_f_wave_and_generic_kernels_agree():
     """Water cc-pVTZ (f on O, d on H): the templated wave kernels of the f
     classes up to 1080 Cartesian components against the generic kernel for
